@@ -1,0 +1,706 @@
+// Host side of libfc_lbm.so: the C ABI declared in include/flowcraft/fc_lbm.h.
+//
+// One handle = one lattice resident in HBM on the current HIP device.  All
+// device work is enqueued on the handle's stream; only the readback/stats/probe
+// calls synchronise.  See lbm_kernels.hpp for the data layout.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/flowcraft/fc_lbm.h"
+#include "lbm_aux.hpp"
+#include "lbm_kernels.hpp"
+
+using namespace fc;
+
+struct fc_lbm {
+    int nx = 0, ny = 0, nz = 0, Q = 19, storage = FC_LBM_STORE_F32, layout = FC_LBM_LAYOUT_PINGPONG;
+    int pery = 0, perz = 0;
+    long long N = 0, P = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    size_t tbytes = 4, rbytes = 4;  // storage element / compute element sizes
+    void* pops[2] = {nullptr, nullptr};
+    uint8_t* flags = nullptr;
+    uint8_t* dsolid = nullptr;
+    int* bidx = nullptr;
+    void* recs = nullptr;
+    int* nearlist = nullptr;
+    int nrec = 0;
+    int* surf = nullptr;
+    int nsurf = 0;
+    void* side_mom = nullptr;
+    void* sforce = nullptr;
+    double* dforces = nullptr;
+    double* stat_part = nullptr;
+    double* stat_out = nullptr;
+    int* dflag = nullptr;   // sticky divergence flag
+    int* dprobe = nullptr;
+    float* staging = nullptr;
+    size_t staging_bytes = 0;
+    fc_lbm_params params{};
+    long long t = 0;
+    bool has_geometry = false;
+    bool has_state = false;
+    std::vector<uint8_t> solid_host;
+    std::string err;
+};
+
+namespace {
+
+const char* kNoHandle = "null fc_lbm handle";
+
+void DevFree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+#define FC_HIP(h, call)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);                       \
+            return FC_LBM_EDEVICE;                                                              \
+        }                                                                                       \
+    } while (0)
+
+int Fail(fc_lbm* h, int code, const std::string& msg) {
+    h->err = msg;
+    return code;
+}
+
+template <int Q, class S, class F>
+void DispatchL(const fc_lbm* h, F& f) {
+    if (h->layout == FC_LBM_LAYOUT_INPLACE)
+        f.template operator()<Q, S, InPlace>();
+    else
+        f.template operator()<Q, S, PingPong>();
+}
+
+template <int Q, class F>
+void DispatchS(const fc_lbm* h, F& f) {
+    switch (h->storage) {
+        case FC_LBM_STORE_F16: DispatchL<Q, StoreF16<Q>>(h, f); break;
+        case FC_LBM_STORE_BF16: DispatchL<Q, StoreBF16<Q>>(h, f); break;
+        case FC_LBM_STORE_F64: DispatchL<Q, StoreF64<Q>>(h, f); break;
+        default: DispatchL<Q, StoreF32<Q>>(h, f); break;
+    }
+}
+
+template <class F>
+void Dispatch(const fc_lbm* h, F&& f) {
+    if (h->Q == 15)
+        DispatchS<15>(h, f);
+    else
+        DispatchS<19>(h, f);
+}
+
+unsigned Blocks(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
+
+void FreeGeometry(fc_lbm* h) {
+    DevFree(h->flags);
+    DevFree(h->dsolid);
+    DevFree(h->bidx);
+    DevFree(h->recs);
+    DevFree(h->nearlist);
+    DevFree(h->surf);
+    DevFree(h->sforce);
+    h->flags = h->dsolid = nullptr;
+    h->bidx = h->nearlist = h->surf = nullptr;
+    h->recs = h->sforce = nullptr;
+    h->nrec = h->nsurf = 0;
+    h->has_geometry = false;
+    h->has_state = false;
+}
+
+// Deterministic compaction of the cells whose flag has `mask` (padded indices, in
+// global cell order).  Allocates *out (at least one element).
+int Compact(fc_lbm* h, uint8_t mask, int** out, int* count) {
+    const unsigned chunks = Blocks(h->N, kChunk);
+    int* dcounts = nullptr;
+    FC_HIP(h, hipMalloc(&dcounts, sizeof(int) * chunks));
+    hipLaunchKernelGGL(KCountFlags, dim3(chunks), dim3(256), 0, h->stream, h->flags, mask, h->nx, h->ny, h->nz,
+                       dcounts);
+    std::vector<int> counts(chunks), offs(chunks);
+    FC_HIP(h, hipMemcpyAsync(counts.data(), dcounts, sizeof(int) * chunks, hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    long long total = 0;
+    for (unsigned i = 0; i < chunks; ++i) {
+        offs[i] = static_cast<int>(total);
+        total += counts[i];
+    }
+    FC_HIP(h, hipMemcpyAsync(dcounts, offs.data(), sizeof(int) * chunks, hipMemcpyHostToDevice, h->stream));
+    FC_HIP(h, hipMalloc(out, sizeof(int) * std::max<long long>(total, 1)));
+    hipLaunchKernelGGL(KScatterFlags, dim3(chunks), dim3(256), 0, h->stream, h->flags, mask, h->nx, h->ny, h->nz,
+                       dcounts, *out);
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    DevFree(dcounts);
+    *count = static_cast<int>(total);
+    return FC_LBM_OK;
+}
+
+int EnsureStaging(fc_lbm* h, size_t bytes) {
+    if (bytes <= h->staging_bytes) return FC_LBM_OK;
+    DevFree(h->staging);
+    h->staging = nullptr;
+    h->staging_bytes = 0;
+    FC_HIP(h, hipMalloc(&h->staging, bytes));
+    h->staging_bytes = bytes;
+    return FC_LBM_OK;
+}
+
+int CheckReady(fc_lbm* h) {
+    if (!h->has_geometry) return Fail(h, FC_LBM_ESTATE, "no geometry: call fc_lbm_set_geometry first");
+    if (!h->has_state) return Fail(h, FC_LBM_ESTATE, "no state: call a reset first");
+    return FC_LBM_OK;
+}
+
+// In-place layout: side values must describe the state after a seed/restore.
+void RefreshSide(fc_lbm* h) {
+    if (h->layout != FC_LBM_LAYOUT_INPLACE) return;
+    const long long work = static_cast<long long>(h->nrec) + 2LL * h->ny * h->nz;
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        using R = typename S::R;
+        hipLaunchKernelGGL((KRefreshSide<Q, S, Lay>), dim3(Blocks(work, 256)), dim3(256), 0, h->stream, h->pops[0],
+                           h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz, h->nearlist, h->nrec,
+                           static_cast<BRec<R>*>(h->recs), static_cast<R*>(h->side_mom));
+    });
+}
+
+void LaunchSurfaceForce(fc_lbm* h) {
+    if (h->nsurf == 0) return;
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        using R = typename S::R;
+        hipLaunchKernelGGL((KSurfaceForce<Q, S, Lay>), dim3(Blocks(h->nsurf, 256)), dim3(256), 0, h->stream,
+                           h->pops[0], h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz, h->flags,
+                           h->bidx, static_cast<const BRec<R>*>(h->recs), h->surf, h->nsurf,
+                           static_cast<R*>(h->sforce));
+    });
+}
+
+void LaunchReduceForces(fc_lbm* h, double* out) {
+    if (h->rbytes == 8)
+        hipLaunchKernelGGL(KReduceForces<double>, dim3(1), dim3(1024), 0, h->stream,
+                           static_cast<const double*>(h->sforce), h->nsurf, out);
+    else
+        hipLaunchKernelGGL(KReduceForces<float>, dim3(1), dim3(1024), 0, h->stream,
+                           static_cast<const float*>(h->sforce), h->nsurf, out);
+}
+
+int FieldChunk(fc_lbm* h, int z0, int zc, float* dout) {
+    const long long cnt = static_cast<long long>(h->nx) * h->ny * zc;
+    LaunchSurfaceForce(h);
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        hipLaunchKernelGGL((KField<Q, S, Lay>), dim3(Blocks(cnt, 256)), dim3(256), 0, h->stream, h->pops[0],
+                           h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz, h->flags, z0, zc, dout);
+    });
+    if (h->nsurf > 0) {
+        if (h->rbytes == 8)
+            hipLaunchKernelGGL(KScatterSurface<double>, dim3(Blocks(h->nsurf, 256)), dim3(256), 0, h->stream, h->surf,
+                               h->nsurf, static_cast<const double*>(h->sforce), h->nx, h->ny, z0, zc, dout);
+        else
+            hipLaunchKernelGGL(KScatterSurface<float>, dim3(Blocks(h->nsurf, 256)), dim3(256), 0, h->stream, h->surf,
+                               h->nsurf, static_cast<const float*>(h->sforce), h->nx, h->ny, z0, zc, dout);
+    }
+    FC_HIP(h, hipGetLastError());
+    return FC_LBM_OK;
+}
+
+int Seed(fc_lbm* h, const float* dfield, double rho0, double u0) {
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        hipLaunchKernelGGL((KSeed<Q, S, Lay>), dim3(Blocks(h->N, 256)), dim3(256), 0, h->stream, h->pops[0],
+                           h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz, h->dsolid, dfield, rho0, u0);
+    });
+    FC_HIP(h, hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
+    RefreshSide(h);
+    FC_HIP(h, hipGetLastError());
+    h->has_state = true;
+    return FC_LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fc_lbm_params_default(fc_lbm_params* p) {
+    if (!p) return;
+    p->tau = 0.8f;
+    p->inlet_velocity = 0.05f;
+    p->reverse_flow = 0;
+    p->free_slip_y = 1;
+    p->free_slip_z = 1;
+    p->smagorinsky_cs = 0.12f;
+}
+
+int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
+    if (!out) return FC_LBM_EINVAL;
+    *out = nullptr;
+    if (!desc) return FC_LBM_EINVAL;
+    fc_lbm* h = new fc_lbm();
+    *out = h;  // returned even on failure so fc_lbm_last_error can explain; destroy it either way
+    if (desc->nx < 4 || desc->ny < 1 || desc->nz < 1)
+        return Fail(h, FC_LBM_EINVAL, "lattice must be at least 4 x 1 x 1");
+    if (desc->model != FC_LBM_D3Q15 && desc->model != FC_LBM_D3Q19) return Fail(h, FC_LBM_EINVAL, "unknown model");
+    if (desc->storage < FC_LBM_STORE_F32 || desc->storage > FC_LBM_STORE_F64)
+        return Fail(h, FC_LBM_EINVAL, "unknown storage");
+    if (desc->layout != FC_LBM_LAYOUT_PINGPONG && desc->layout != FC_LBM_LAYOUT_INPLACE)
+        return Fail(h, FC_LBM_EINVAL, "unknown layout");
+    h->nx = desc->nx;
+    h->ny = desc->ny;
+    h->nz = desc->nz;
+    h->Q = desc->model == FC_LBM_D3Q15 ? 15 : 19;
+    h->storage = desc->storage;
+    h->layout = desc->layout;
+    h->pery = desc->periodic_y ? 1 : 0;
+    h->perz = desc->periodic_z ? 1 : 0;
+    h->N = static_cast<long long>(h->nx) * h->ny * h->nz;
+    h->P = static_cast<long long>(h->nx) * (h->ny + 2) * (h->nz + 2);
+    if (h->P >= (1LL << 31)) return Fail(h, FC_LBM_EINVAL, "lattice too large (padded cells must be < 2^31)");
+    h->tbytes = h->storage == FC_LBM_STORE_F64 ? 8 : (h->storage == FC_LBM_STORE_F32 ? 4 : 2);
+    h->rbytes = h->storage == FC_LBM_STORE_F64 ? 8 : 4;
+    fc_lbm_params_default(&h->params);
+    FC_HIP(h, hipGetDevice(&h->device));
+    if (desc->stream) {
+        h->stream = static_cast<hipStream_t>(desc->stream);
+    } else {
+        FC_HIP(h, hipStreamCreate(&h->stream));
+        h->own_stream = true;
+    }
+    const size_t popbytes = static_cast<size_t>(h->Q) * h->P * h->tbytes;
+    const int copies = h->layout == FC_LBM_LAYOUT_INPLACE ? 1 : 2;
+    for (int i = 0; i < copies; ++i) {
+        if (hipMalloc(&h->pops[i], popbytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return Fail(h, FC_LBM_ENOMEM, "population allocation failed (" + std::to_string(popbytes) + " bytes)");
+        }
+        FC_HIP(h, hipMemsetAsync(h->pops[i], 0, popbytes, h->stream));
+    }
+    if (copies == 1) h->pops[1] = h->pops[0];
+    FC_HIP(h, hipMalloc(&h->side_mom, 16ull * h->ny * h->nz * h->rbytes));
+    FC_HIP(h, hipMemsetAsync(h->side_mom, 0, 16ull * h->ny * h->nz * h->rbytes, h->stream));
+    FC_HIP(h, hipMalloc(&h->dforces, 4 * sizeof(double)));
+    FC_HIP(h, hipMalloc(&h->stat_part, kStatBlocks * 8 * sizeof(double)));
+    FC_HIP(h, hipMalloc(&h->stat_out, 8 * sizeof(double)));
+    FC_HIP(h, hipMalloc(&h->dflag, sizeof(int)));
+    FC_HIP(h, hipMalloc(&h->dprobe, sizeof(int)));
+    FC_HIP(h, hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_destroy(fc_lbm* h) {
+    if (!h) return FC_LBM_EINVAL;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    FreeGeometry(h);
+    if (h->pops[1] != h->pops[0]) DevFree(h->pops[1]);
+    DevFree(h->pops[0]);
+    DevFree(h->side_mom);
+    DevFree(h->dforces);
+    DevFree(h->stat_part);
+    DevFree(h->stat_out);
+    DevFree(h->dflag);
+    DevFree(h->dprobe);
+    DevFree(h->staging);
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FC_LBM_OK;
+}
+
+const char* fc_lbm_last_error(const fc_lbm* h) { return h ? h->err.c_str() : kNoHandle; }
+
+int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf) {
+    if (!h) return FC_LBM_EINVAL;
+    if (!h->pops[0]) return Fail(h, FC_LBM_ESTATE, "handle was not created successfully");
+    if (!solid) return Fail(h, FC_LBM_EINVAL, "solid mask is required");
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    FreeGeometry(h);
+    h->solid_host.assign(solid, solid + h->N);
+    FC_HIP(h, hipMalloc(&h->dsolid, h->N));
+    FC_HIP(h, hipMemcpyAsync(h->dsolid, solid, h->N, hipMemcpyHostToDevice, h->stream));
+    float* dsdf = nullptr;
+    if (sdf) {
+        FC_HIP(h, hipMalloc(&dsdf, h->N * sizeof(float)));
+        FC_HIP(h, hipMemcpyAsync(dsdf, sdf, h->N * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    }
+    FC_HIP(h, hipMalloc(&h->flags, h->P));
+    FC_HIP(h, hipMemsetAsync(h->flags, 0, h->P, h->stream));
+    if (h->Q == 15)
+        hipLaunchKernelGGL(KClassify<15>, dim3(Blocks(h->N, 256)), dim3(256), 0, h->stream, h->dsolid, h->flags, h->nx,
+                           h->ny, h->nz, h->pery, h->perz);
+    else
+        hipLaunchKernelGGL(KClassify<19>, dim3(Blocks(h->N, 256)), dim3(256), 0, h->stream, h->dsolid, h->flags, h->nx,
+                           h->ny, h->nz, h->pery, h->perz);
+    FC_HIP(h, hipGetLastError());
+    int rc = Compact(h, kNearSolid, &h->nearlist, &h->nrec);
+    if (rc) return rc;
+    rc = Compact(h, kSurface, &h->surf, &h->nsurf);
+    if (rc) return rc;
+    FC_HIP(h, hipMalloc(&h->bidx, h->P * sizeof(int)));
+    FC_HIP(h, hipMemsetAsync(h->bidx, 0xFF, h->P * sizeof(int), h->stream));
+    const size_t recbytes = h->rbytes == 8 ? sizeof(BRec<double>) : sizeof(BRec<float>);
+    FC_HIP(h, hipMalloc(&h->recs, recbytes * std::max(h->nrec, 1)));
+    if (h->nrec > 0) {
+        Dispatch(h, [&]<int Q, class S, class Lay>() {
+            using R = typename S::R;
+            hipLaunchKernelGGL((KBuildRecords<Q, R>), dim3(Blocks(h->nrec, 256)), dim3(256), 0, h->stream, h->dsolid,
+                               dsdf, h->nearlist, h->nrec, h->bidx, static_cast<BRec<R>*>(h->recs), h->nx, h->ny,
+                               h->nz, h->pery, h->perz);
+        });
+    }
+    FC_HIP(h, hipMalloc(&h->sforce, 4 * h->rbytes * std::max(h->nsurf, 1)));
+    FC_HIP(h, hipGetLastError());
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    DevFree(dsdf);
+    h->has_geometry = true;
+    h->has_state = false;
+    return FC_LBM_OK;
+}
+
+int fc_lbm_set_params(fc_lbm* h, const fc_lbm_params* p) {
+    if (!h || !p) return FC_LBM_EINVAL;
+    h->params = *p;
+    return FC_LBM_OK;
+}
+
+int fc_lbm_get_params(const fc_lbm* h, fc_lbm_params* p) {
+    if (!h || !p) return FC_LBM_EINVAL;
+    *p = h->params;
+    return FC_LBM_OK;
+}
+
+int fc_lbm_reset_uniform(fc_lbm* h, int seed_mid) {
+    if (!h) return FC_LBM_EINVAL;
+    if (!h->has_geometry) return Fail(h, FC_LBM_ESTATE, "no geometry: call fc_lbm_set_geometry first");
+    h->t = 0;
+    double rho0 = 1.0, u0 = 0.0;
+    if (seed_mid) {
+        if (h->rbytes == 4) {
+            // LbmReference::Reset (LbmReference.cpp:192-202), in float.
+            float u = h->params.inlet_velocity;
+            float mid = 0.5f * u;
+            float ux = h->params.reverse_flow ? -mid : mid;
+            float dev = 0.5f * (u * u - mid * mid);
+            rho0 = 1.0f + 3.0f * dev;
+            u0 = ux;
+        } else {
+            double u = h->params.inlet_velocity;
+            double mid = 0.5 * u;
+            u0 = h->params.reverse_flow ? -mid : mid;
+            rho0 = 1.0 + 3.0 * (0.5 * (u * u - mid * mid));
+        }
+    }
+    return Seed(h, nullptr, rho0, u0);
+}
+
+int fc_lbm_reset_from_field_device(fc_lbm* h, const float* field4_device) {
+    if (!h || !field4_device) return FC_LBM_EINVAL;
+    if (!h->has_geometry) return Fail(h, FC_LBM_ESTATE, "no geometry: call fc_lbm_set_geometry first");
+    h->t = 0;
+    return Seed(h, field4_device, 1.0, 0.0);
+}
+
+int fc_lbm_reset_from_field(fc_lbm* h, const float* field4) {
+    if (!h || !field4) return FC_LBM_EINVAL;
+    if (!h->has_geometry) return Fail(h, FC_LBM_ESTATE, "no geometry: call fc_lbm_set_geometry first");
+    int rc = EnsureStaging(h, static_cast<size_t>(h->N) * 16);
+    if (rc) return rc;
+    FC_HIP(h, hipMemcpyAsync(h->staging, field4, static_cast<size_t>(h->N) * 16, hipMemcpyHostToDevice, h->stream));
+    rc = fc_lbm_reset_from_field_device(h, h->staging);
+    if (rc) return rc;
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
+    if (!h) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    if (z0 < 0) {
+        zcount += z0;
+        z0 = 0;
+    }
+    if (z0 + zcount > h->nz) zcount = h->nz - z0;
+    if (zcount <= 0) return FC_LBM_OK;
+    StepArgs a{};
+    const int par = static_cast<int>(h->t & 1);
+    a.pop_src = h->pops[par];
+    a.pop_dst = h->layout == FC_LBM_LAYOUT_INPLACE ? h->pops[0] : h->pops[par ^ 1];
+    a.flags = h->flags;
+    a.bidx = h->bidx;
+    a.recs = h->recs;
+    a.side_mom = h->side_mom;
+    a.diverged = h->dflag;
+    a.P = h->P;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.nz = h->nz;
+    a.z0 = z0;
+    a.zc = zcount;
+    a.parity = par;
+    a.reverse = h->params.reverse_flow ? 1 : 0;
+    a.fsy = h->params.free_slip_y ? 1 : 0;
+    a.fsz = h->params.free_slip_z ? 1 : 0;
+    a.pery = h->pery;
+    a.perz = h->perz;
+    a.tau = h->params.tau;
+    a.u_in = h->params.inlet_velocity;
+    a.cs = h->params.smagorinsky_cs;
+    a.tau_d = h->params.tau;
+    a.u_in_d = h->params.inlet_velocity;
+    a.cs_d = h->params.smagorinsky_cs;
+    const long long cells = static_cast<long long>(h->nx) * h->ny * zcount;
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        hipLaunchKernelGGL((KStep<Q, S, Lay>), dim3(Blocks(cells, 256)), dim3(256), 0, h->stream, a);
+    });
+    FC_HIP(h, hipGetLastError());
+    return FC_LBM_OK;
+}
+
+int fc_lbm_advance(fc_lbm* h) {
+    if (!h) return FC_LBM_EINVAL;
+    ++h->t;
+    return FC_LBM_OK;
+}
+
+int fc_lbm_step(fc_lbm* h, int n) {
+    if (!h) return FC_LBM_EINVAL;
+    for (int i = 0; i < n; ++i) {
+        int rc = fc_lbm_step_range(h, 0, h->nz);
+        if (rc) return rc;
+        ++h->t;
+    }
+    return FC_LBM_OK;
+}
+
+int fc_lbm_field_device(fc_lbm* h, float* out_device) {
+    if (!h || !out_device) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    return FieldChunk(h, 0, h->nz, out_device);
+}
+
+int fc_lbm_readback_field_range(fc_lbm* h, int z0, int zcount, float* out) {
+    if (!h || !out) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    if (z0 < 0) {
+        zcount += z0;
+        z0 = 0;
+    }
+    if (z0 + zcount > h->nz) zcount = h->nz - z0;
+    if (zcount <= 0) return FC_LBM_OK;
+    const size_t layer = static_cast<size_t>(h->nx) * h->ny * 4;
+    rc = EnsureStaging(h, layer * zcount * sizeof(float));
+    if (rc) return rc;
+    rc = FieldChunk(h, z0, zcount, h->staging);
+    if (rc) return rc;
+    FC_HIP(h, hipMemcpyAsync(out + layer * z0, h->staging, layer * zcount * sizeof(float), hipMemcpyDeviceToHost,
+                             h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_readback_field(fc_lbm* h, float* out) {
+    if (!h) return FC_LBM_EINVAL;
+    return fc_lbm_readback_field_range(h, 0, h->nz, out);
+}
+
+int fc_lbm_forces_device(fc_lbm* h, double* out_device) {
+    if (!h || !out_device) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    LaunchSurfaceForce(h);
+    LaunchReduceForces(h, out_device);
+    FC_HIP(h, hipGetLastError());
+    return FC_LBM_OK;
+}
+
+int fc_lbm_forces(fc_lbm* h, double out[4]) {
+    if (!h || !out) return FC_LBM_EINVAL;
+    int rc = fc_lbm_forces_device(h, h->dforces);
+    if (rc) return rc;
+    FC_HIP(h, hipMemcpyAsync(out, h->dforces, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_readback_stats(fc_lbm* h, fc_lbm_stats* out) {
+    if (!h || !out) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    LaunchSurfaceForce(h);
+    LaunchReduceForces(h, h->dforces);
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        hipLaunchKernelGGL((KStatsPartial<Q, S, Lay>), dim3(kStatBlocks), dim3(256), 0, h->stream, h->pops[0],
+                           h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz, h->flags, h->stat_part);
+    });
+    hipLaunchKernelGGL(KStatsFinal, dim3(1), dim3(64), 0, h->stream, h->stat_part, h->stat_out);
+    FC_HIP(h, hipGetLastError());
+    double f[4], s[8];
+    FC_HIP(h, hipMemcpyAsync(f, h->dforces, sizeof(f), hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipMemcpyAsync(s, h->stat_out, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    const long long n = static_cast<long long>(s[2]);
+    out->fluid_cells = n;
+    out->avg_density = n ? s[0] / static_cast<double>(n) : 0.0;
+    out->avg_speed = n ? s[1] / static_cast<double>(n) : 0.0;
+    out->min_density = n ? s[4] : 0.0;
+    out->max_density = n ? s[5] : 0.0;
+    out->max_speed = s[6];
+    out->total_mass = s[0];
+    out->has_nan = s[3] > 0 ? 1 : 0;
+    out->force_x = f[0];
+    out->force_y = f[1];
+    out->force_z = f[2];
+    out->max_stress = f[3];
+    return FC_LBM_OK;
+}
+
+int fc_lbm_probe_health(fc_lbm* h, int z, int* diverged) {
+    if (!h || !diverged) return FC_LBM_EINVAL;
+    *diverged = 0;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    z = std::clamp(z, 0, h->nz - 1);
+    FC_HIP(h, hipMemsetAsync(h->dprobe, 0, sizeof(int), h->stream));
+    LaunchSurfaceForce(h);
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        hipLaunchKernelGGL((KProbe<Q, S, Lay>), dim3(Blocks(static_cast<long long>(h->nx) * h->ny, 256)), dim3(256), 0,
+                           h->stream, h->pops[0], h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz,
+                           h->flags, z, h->dprobe);
+    });
+    if (h->nsurf > 0) {
+        if (h->rbytes == 8)
+            hipLaunchKernelGGL(KProbeSurface<double>, dim3(Blocks(h->nsurf, 256)), dim3(256), 0, h->stream, h->surf,
+                               h->nsurf, static_cast<const double*>(h->sforce), h->nx, h->ny, z, h->dprobe);
+        else
+            hipLaunchKernelGGL(KProbeSurface<float>, dim3(Blocks(h->nsurf, 256)), dim3(256), 0, h->stream, h->surf,
+                               h->nsurf, static_cast<const float*>(h->sforce), h->nx, h->ny, z, h->dprobe);
+    }
+    FC_HIP(h, hipGetLastError());
+    FC_HIP(h, hipMemcpyAsync(diverged, h->dprobe, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_has_diverged(fc_lbm* h, int* diverged) {
+    if (!h || !diverged) return FC_LBM_EINVAL;
+    *diverged = 0;
+    if (!h->dflag) return Fail(h, FC_LBM_ESTATE, "handle was not created successfully");
+    FC_HIP(h, hipMemcpyAsync(diverged, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_get_populations(fc_lbm* h, void* out) {
+    if (!h || !out) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    const size_t bytes = static_cast<size_t>(h->Q) * h->N * h->rbytes;
+    void* tmp = nullptr;
+    FC_HIP(h, hipMalloc(&tmp, bytes));
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        using R = typename S::R;
+        hipLaunchKernelGGL((KGetPops<Q, S, Lay>), dim3(Blocks(h->N, 256)), dim3(256), 0, h->stream, h->pops[0],
+                           h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz, h->dsolid,
+                           static_cast<R*>(tmp));
+    });
+    hipError_t e = hipMemcpyAsync(out, tmp, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    DevFree(tmp);
+    if (e != hipSuccess) return Fail(h, FC_LBM_EDEVICE, std::string("get_populations: ") + hipGetErrorString(e));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_set_populations(fc_lbm* h, const void* in) {
+    if (!h || !in) return FC_LBM_EINVAL;
+    if (!h->has_geometry) return Fail(h, FC_LBM_ESTATE, "no geometry: call fc_lbm_set_geometry first");
+    const size_t bytes = static_cast<size_t>(h->Q) * h->N * h->rbytes;
+    void* tmp = nullptr;
+    FC_HIP(h, hipMalloc(&tmp, bytes));
+    hipError_t e = hipMemcpyAsync(tmp, in, bytes, hipMemcpyHostToDevice, h->stream);
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        using R = typename S::R;
+        hipLaunchKernelGGL((KSetPops<Q, S, Lay>), dim3(Blocks(h->N, 256)), dim3(256), 0, h->stream, h->pops[0],
+                           h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz, static_cast<const R*>(tmp));
+    });
+    h->has_state = true;
+    RefreshSide(h);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    DevFree(tmp);
+    if (e != hipSuccess) return Fail(h, FC_LBM_EDEVICE, std::string("set_populations: ") + hipGetErrorString(e));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_sync(fc_lbm* h) {
+    if (!h) return FC_LBM_EINVAL;
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_info(const fc_lbm* h, int* nx, int* ny, int* nz, int* q, int* storage, int* layout) {
+    if (!h) return FC_LBM_EINVAL;
+    if (nx) *nx = h->nx;
+    if (ny) *ny = h->ny;
+    if (nz) *nz = h->nz;
+    if (q) *q = h->Q;
+    if (storage) *storage = h->storage;
+    if (layout) *layout = h->layout;
+    return FC_LBM_OK;
+}
+
+long long fc_lbm_step_count(const fc_lbm* h) { return h ? h->t : -1; }
+
+size_t fc_lbm_population_bytes(const fc_lbm* h) {
+    if (!h) return 0;
+    return static_cast<size_t>(h->Q) * h->P * h->tbytes * (h->layout == FC_LBM_LAYOUT_INPLACE ? 1 : 2);
+}
+
+void* fc_lbm_stream(const fc_lbm* h) { return h ? static_cast<void*>(h->stream) : nullptr; }
+
+long long fc_lbm_surface_cells(const fc_lbm* h) { return h ? h->nsurf : -1; }
+
+int fc_lbm_boundary_links(fc_lbm* h, long long max, long long* cell, int* dir, int* kind, float* q,
+                          long long* count) {
+    if (!h || !count) return FC_LBM_EINVAL;
+    if (!h->has_geometry) return Fail(h, FC_LBM_ESTATE, "no geometry: call fc_lbm_set_geometry first");
+    std::vector<int> list(std::max(h->nrec, 1));
+    const size_t rb = h->rbytes == 8 ? sizeof(BRec<double>) : sizeof(BRec<float>);
+    std::vector<unsigned char> recs(rb * std::max(h->nrec, 1));
+    FC_HIP(h, hipMemcpy(list.data(), h->nearlist, sizeof(int) * h->nrec, hipMemcpyDeviceToHost));
+    FC_HIP(h, hipMemcpy(recs.data(), h->recs, rb * h->nrec, hipMemcpyDeviceToHost));
+    long long k = 0;
+    for (int r = 0; r < h->nrec; ++r) {
+        const int n = list[r];
+        const int x = n % h->nx;
+        const int y = (n / h->nx) % (h->ny + 2) - 1;
+        const int z = n / (h->nx * (h->ny + 2)) - 1;
+        const long long c = (static_cast<long long>(z) * h->ny + y) * h->nx + x;
+        const uint32_t* kinds = reinterpret_cast<const uint32_t*>(recs.data() + rb * r);
+        for (int d = 1; d < h->Q; ++d) {
+            const unsigned kd = (kinds[d >> 3] >> ((d & 7) * 4)) & 15u;
+            if (kd == kPull) continue;
+            if (k < max) {
+                if (cell) cell[k] = c;
+                if (dir) dir[k] = d;
+                if (kind) kind[k] = static_cast<int>(kd);
+                if (q) {
+                    if (h->rbytes == 8)
+                        q[k] = static_cast<float>(reinterpret_cast<const BRec<double>*>(recs.data() + rb * r)->q[d]);
+                    else
+                        q[k] = reinterpret_cast<const BRec<float>*>(recs.data() + rb * r)->q[d];
+                }
+            }
+            ++k;
+        }
+    }
+    *count = k;
+    return FC_LBM_OK;
+}
+
+}  // extern "C"

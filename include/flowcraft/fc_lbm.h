@@ -1,0 +1,191 @@
+/* fc_lbm.h — C ABI of the MI355X-native Lattice Boltzmann engine (libfc_lbm.so).
+ *
+ * This is the drop-in boundary for FlowCraft's LbmSolver step
+ * (FlowCraft/LbmSolver.h:59-302).  Every entry point is a plain C function over
+ * host pointers and sizes; device memory is owned by the library.  All calls
+ * return an int status (FC_LBM_OK = 0) unless stated otherwise; on failure
+ * fc_lbm_last_error() describes the cause.  A handle is not thread-safe: the
+ * reference serialises every solver call under one mutex (FlowCraft/main.cpp:
+ * 366-373) and callers of this ABI must do the same.
+ *
+ * Field layout (FlowCraft/LbmSolver.h:190-214, consumers FieldOverlay.cpp:38-46
+ * and SmokeTrail.cpp:96): 4 floats per cell at global index
+ * cell = (z*ny + y)*nx + x; fluid cells hold (rho, ux, uy, uz), solid cells hold
+ * (|F|/(6*sum w), Fx, Fy, Fz) — the momentum-exchange force of the CURRENT
+ * populations (the LbmReference convention, LbmReference.cpp:127-148,475-500).
+ */
+#ifndef FLOWCRAFT_FC_LBM_H
+#define FLOWCRAFT_FC_LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FC_LBM_OK 0
+#define FC_LBM_EINVAL 1    /* bad argument or size mismatch */
+#define FC_LBM_ENOMEM 2    /* device allocation failed */
+#define FC_LBM_EDEVICE 3   /* HIP runtime error */
+#define FC_LBM_ESTATE 4    /* call not valid in the handle's state (e.g. no geometry yet) */
+
+/* LatticeModel (FlowCraft/LatticeModel.h:12): same numbering as the reference enum. */
+#define FC_LBM_D3Q15 0
+#define FC_LBM_D3Q19 1
+
+/* Population storage.  F32 mirrors LbmReference's float populations; F16/BF16 store
+ * g = (f - w_i) * S (S = 4096 for F16, 1 for BF16) and run the collision in FP32;
+ * F64 runs the whole step in double (the GPU twin of the FP64 oracle). */
+#define FC_LBM_STORE_F32 0
+#define FC_LBM_STORE_F16 1
+#define FC_LBM_STORE_BF16 2
+#define FC_LBM_STORE_F64 3
+
+/* Population layout: two SoA copies (pull, ping-pong) or one SoA copy updated in
+ * place (esoteric pull). */
+#define FC_LBM_LAYOUT_PINGPONG 0
+#define FC_LBM_LAYOUT_INPLACE 1
+
+typedef struct fc_lbm fc_lbm;
+
+typedef struct fc_lbm_desc {
+    int nx, ny, nz;     /* lattice size; nx >= 4 */
+    int model;          /* FC_LBM_D3Q15 / FC_LBM_D3Q19 */
+    int storage;        /* FC_LBM_STORE_* */
+    int layout;         /* FC_LBM_LAYOUT_* */
+    int periodic_y;     /* extension: wrap y instead of the free-slip/no-slip wall */
+    int periodic_z;     /* extension: wrap z */
+    void* stream;       /* hipStream_t to launch on; NULL = the library creates its own */
+} fc_lbm_desc;
+
+/* LbmSolver::Params (FlowCraft/LbmSolver.h:61-100); defaults tau 0.8, u 0.05,
+ * reverse 0, free-slip y/z 1, Cs 0.12 (fc_lbm_params_default). */
+typedef struct fc_lbm_params {
+    float tau;
+    float inlet_velocity;
+    int reverse_flow;
+    int free_slip_y;
+    int free_slip_z;
+    float smagorinsky_cs;
+} fc_lbm_params;
+
+/* LbmSolver::Stats (FlowCraft/LbmSolver.h:172-184) plus LbmReference::Stats'
+ * totalMass (FlowCraft/LbmReference.h:112-119).  Sums accumulate in FP64. */
+typedef struct fc_lbm_stats {
+    double avg_density, min_density, max_density;
+    double avg_speed, max_speed;
+    double force_x, force_y, force_z;
+    double max_stress;
+    double total_mass;
+    long long fluid_cells;
+    int has_nan;
+} fc_lbm_stats;
+
+/* Replaces LbmSolver::Init's allocation half (FlowCraft/LbmSolver.cpp:205-366).
+ * The device is the HIP current device at the call. */
+int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out);
+
+/* Replaces LbmSolver::Shutdown (FlowCraft/LbmSolver.cpp:368-387). */
+int fc_lbm_destroy(fc_lbm* h);
+
+/* Replaces LbmSolver::LastError (FlowCraft/LbmSolver.h:110).  Never NULL. */
+const char* fc_lbm_last_error(const fc_lbm* h);
+
+void fc_lbm_params_default(fc_lbm_params* p);
+
+/* Replaces LbmSolver::Init's geometry upload (FlowCraft/LbmSolver.cpp:331-363):
+ * solid is nx*ny*nz bytes (VoxelGrid::solid, 1 = solid), sdf is nx*ny*nz floats
+ * (VoxelGrid::signedDistance) or NULL for plain bounce-back everywhere.  Builds the
+ * per-cell flags, the Bouzidi link table q = clamp(phi_f/(phi_f - phi_s), 0, 1)
+ * (FlowCraft/LbmReference.cpp:104-112) and the solid-surface list. Host pointers. */
+int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf);
+
+/* Replaces LbmSolver::SetParams (FlowCraft/LbmSolver.h:135): state untouched. */
+int fc_lbm_set_params(fc_lbm* h, const fc_lbm_params* p);
+int fc_lbm_get_params(const fc_lbm* h, fc_lbm_params* p);
+
+/* Replaces LbmSolver::Reset's seeding (FlowCraft/LbmSolver.cpp:389-427) with the
+ * current params: quiescent (rho 1, u 0) or mid-speed (u = u_in/2, Bernoulli rho). */
+int fc_lbm_reset_uniform(fc_lbm* h, int seed_mid);
+
+/* Replaces LbmSolver::ResetFromField (FlowCraft/LbmSolver.cpp:429-472): per-cell
+ * equilibrium of field4 (4 floats per cell, solid cells -> (1,0,0,0)).  Host pointer. */
+int fc_lbm_reset_from_field(fc_lbm* h, const float* field4);
+/* Same, from a device pointer (the learned warm start hands its prediction over in HBM). */
+int fc_lbm_reset_from_field_device(fc_lbm* h, const float* field4_device);
+
+/* Replaces LbmSolver::Step (FlowCraft/LbmSolver.cpp:556-562): n substeps, enqueued
+ * on the handle's stream (asynchronous, like the reference's command recording). */
+int fc_lbm_step(fc_lbm* h, int n);
+
+/* Replaces LbmSolver::StepRange (FlowCraft/LbmSolver.cpp:543-554): one substep of
+ * the cells with z in [z0, z0+zcount); call fc_lbm_advance once after the chunks
+ * covering [0, nz). */
+int fc_lbm_step_range(fc_lbm* h, int z0, int zcount);
+
+/* Replaces LbmSolver::AdvanceSubstep (FlowCraft/LbmSolver.h:170). */
+int fc_lbm_advance(fc_lbm* h);
+
+/* Replaces LbmSolver::ReadbackFieldRange (FlowCraft/LbmSolver.cpp:584-651): writes
+ * the layers [z0, z0+zcount) into out (host, sized for the WHOLE field, 4*N floats)
+ * at their global offsets.  Blocking. */
+int fc_lbm_readback_field_range(fc_lbm* h, int z0, int zcount, float* out);
+
+/* Replaces LbmSolver::ReadbackField (FlowCraft/LbmSolver.cpp:722-737): whole field, host. */
+int fc_lbm_readback_field(fc_lbm* h, float* out);
+
+/* The whole field into a device buffer of 4*N floats (no host round trip). */
+int fc_lbm_field_device(fc_lbm* h, float* out_device);
+
+/* Replaces LbmSolver::ReadbackStats (FlowCraft/LbmSolver.cpp:739-788) and
+ * LbmReference::ComputeStats (FlowCraft/LbmReference.cpp:502-585), reduced on the
+ * device in a fixed order (deterministic).  Blocking. */
+int fc_lbm_readback_stats(fc_lbm* h, fc_lbm_stats* out);
+
+/* Momentum-exchange force only (the Cd/Cl path): out[0..2] = sum of Fx, Fy, Fz over
+ * all solid cells in FP64, out[3] = max stress.  Blocking. */
+int fc_lbm_forces(fc_lbm* h, double out[4]);
+
+/* Same sums left in device memory (4 doubles) without synchronising, for loops that
+ * sample the force every few steps. */
+int fc_lbm_forces_device(fc_lbm* h, double* out_device);
+
+/* Replaces LbmSolver::ProbeSliceHealth (FlowCraft/LbmSolver.cpp:653-720): *diverged
+ * = 1 if any cell of layer z (clamped) is non-finite, or a fluid cell has rho <= 0 or
+ * |u|^2 > 4.  Blocking. */
+int fc_lbm_probe_health(fc_lbm* h, int z, int* diverged);
+
+/* LbmReference::HasDiverged (FlowCraft/LbmReference.h:129): sticky flag set by any
+ * step since the last reset.  Blocking. */
+int fc_lbm_has_diverged(fc_lbm* h, int* diverged);
+
+/* Exact state dump/restore (extension; the reference only has the lossy
+ * ResetFromField): f[d*N + cell] as float (F64 storage: as double), host pointers. */
+int fc_lbm_get_populations(fc_lbm* h, void* out);
+int fc_lbm_set_populations(fc_lbm* h, const void* in);
+
+/* Waits for all work queued on the handle's stream. */
+int fc_lbm_sync(fc_lbm* h);
+
+/* Introspection used by bindings, tests and the bench. */
+int fc_lbm_info(const fc_lbm* h, int* nx, int* ny, int* nz, int* q, int* storage, int* layout);
+long long fc_lbm_step_count(const fc_lbm* h);
+size_t fc_lbm_population_bytes(const fc_lbm* h);
+void* fc_lbm_stream(const fc_lbm* h);
+
+/* Boundary link table dump (tests pin q-fractions bit-exactly): up to max links,
+ * each (cell, direction i, kind, q) for a fluid cell whose direction-i source is
+ * solid.  kind: 2 plain bounce-back, 3 Bouzidi q < 1/2, 4 Bouzidi q >= 1/2.
+ * *count receives the total number of links. */
+int fc_lbm_boundary_links(fc_lbm* h, long long max, long long* cell, int* dir, int* kind, float* q,
+                          long long* count);
+
+/* Number of solid cells with a fluid neighbour (the force kernel's work list). */
+long long fc_lbm_surface_cells(const fc_lbm* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLOWCRAFT_FC_LBM_H */

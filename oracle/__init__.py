@@ -1,0 +1,233 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes front-end for the CPU oracle built from ``oracle/lbm_oracle.cpp``
+(``liblbm_oracle.so``) and, where it was built, FlowCraft's own LbmReference
+(``oracle/_ref/libref_lbm.so``).  Only ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s CPU-baseline leg may import this package, and only as the
+checker: the engine in ``flowcraft_amd`` never routes through it.
+
+The restated functions and the reference lines they follow are listed in the
+header of ``lbm_oracle.cpp``; every function there is pinned by tests in
+``tests/test_oracle*.py``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "liblbm_oracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libref_lbm.so")
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_u8_p = ctypes.POINTER(ctypes.c_uint8)
+_c_u16_p = ctypes.POINTER(ctypes.c_uint16)
+
+STORAGE_NATIVE, STORAGE_F16, STORAGE_BF16 = 0, 1, 2
+STATS_FIELDS = ("avgDensity", "minDensity", "maxDensity", "avgSpeed", "maxSpeed", "forceX", "forceY",
+                "forceZ", "maxStress", "totalMass", "hasNaN")
+
+
+def build(reference: bool = False) -> None:
+    """Compile the oracle (and, when asked and the reference is present, oracle/_ref)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liblbm_oracle.so"], check=True)
+    if reference and os.path.isdir("/root/reference/FlowCraft"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int] * 9
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_geometry.argtypes = [ctypes.c_void_p, _c_u8_p, _c_float_p]
+        L.orc_set_params.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_float]
+        L.orc_reset.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_reset_from_field.argtypes = [ctypes.c_void_p, _c_float_p]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_readback_field_f32.argtypes = [ctypes.c_void_p, _c_float_p]
+        L.orc_readback_field_f64.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.orc_stats.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.orc_has_diverged.argtypes = [ctypes.c_void_p]
+        L.orc_has_diverged.restype = ctypes.c_int
+        L.orc_get_pops.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.orc_set_pops.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.orc_f32_to_f16.argtypes = [ctypes.c_float]
+        L.orc_f32_to_f16.restype = ctypes.c_uint16
+        L.orc_f16_to_f32.argtypes = [ctypes.c_uint16]
+        L.orc_f16_to_f32.restype = ctypes.c_float
+        L.orc_f32_to_bf16.argtypes = [ctypes.c_float]
+        L.orc_f32_to_bf16.restype = ctypes.c_uint16
+        L.orc_bf16_to_f32.argtypes = [ctypes.c_uint16]
+        L.orc_bf16_to_f32.restype = ctypes.c_float
+        L.orc_f32_to_f16_array.argtypes = [_c_float_p, _c_u16_p, ctypes.c_size_t]
+        L.orc_f32_to_bf16_array.argtypes = [_c_float_p, _c_u16_p, ctypes.c_size_t]
+        L.orc_f16_scale.restype = ctypes.c_float
+        L.orc_time_steps.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.orc_time_steps.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref/libref_lbm.so is not built (make -C oracle ref)")
+        L = ctypes.CDLL(REF_SO)
+        L.ref_create.restype = ctypes.c_void_p
+        L.ref_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_u8_p, _c_float_p,
+                                 ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_float, ctypes.c_int]
+        L.ref_destroy.argtypes = [ctypes.c_void_p]
+        L.ref_set_params.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_float]
+        L.ref_reset.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ref_reset_from_field.argtypes = [ctypes.c_void_p, _c_float_p]
+        L.ref_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ref_readback_field.argtypes = [ctypes.c_void_p, _c_float_p]
+        L.ref_stats.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.ref_has_diverged.argtypes = [ctypes.c_void_p]
+        L.ref_has_diverged.restype = ctypes.c_int
+        L.ref_time_steps.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.ref_time_steps.restype = ctypes.c_double
+        _ref = L
+    return _ref
+
+
+class OracleLbm:
+    """The restated LbmReference.  bits=32 (parity twin) or 64 (FP64 oracle)."""
+
+    def __init__(self, solid: np.ndarray, sdf: np.ndarray | None, q: int = 19, bits: int = 32,
+                 storage: int = STORAGE_NATIVE, periodic_y: bool = False, periodic_z: bool = False,
+                 nthreads: int = 0):
+        L = lib()
+        nz, ny, nx = solid.shape
+        self.shape = (nz, ny, nx)
+        self.q = q
+        self.bits = bits
+        self.h = L.orc_create(nx, ny, nz, q, bits, storage, int(periodic_y), int(periodic_z), nthreads)
+        if not self.h:
+            raise ValueError("orc_create rejected the configuration")
+        self._solid = np.ascontiguousarray(solid, dtype=np.uint8)
+        self._sdf = None if sdf is None else np.ascontiguousarray(sdf, dtype=np.float32)
+        L.orc_set_geometry(self.h, _ptr(self._solid, _c_u8_p),
+                           None if self._sdf is None else _ptr(self._sdf, _c_float_p))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_params(self, tau=0.8, u=0.05, reverse=False, free_slip_y=True, free_slip_z=True, cs=0.12):
+        lib().orc_set_params(self.h, tau, u, int(reverse), int(free_slip_y), int(free_slip_z), cs)
+
+    def reset(self, seed_mid=False):
+        lib().orc_reset(self.h, int(seed_mid))
+
+    def reset_from_field(self, field: np.ndarray):
+        f = np.ascontiguousarray(field, dtype=np.float32)
+        lib().orc_reset_from_field(self.h, _ptr(f, _c_float_p))
+
+    def step(self, n=1):
+        lib().orc_step(self.h, n)
+
+    def field(self) -> np.ndarray:
+        n = int(np.prod(self.shape))
+        if self.bits == 64:
+            out = np.empty(n * 4, np.float64)
+            lib().orc_readback_field_f64(self.h, _ptr(out, _c_double_p))
+        else:
+            out = np.empty(n * 4, np.float32)
+            lib().orc_readback_field_f32(self.h, _ptr(out, _c_float_p))
+        return out
+
+    def stats(self) -> dict:
+        out = np.zeros(11, np.float64)
+        lib().orc_stats(self.h, _ptr(out, _c_double_p))
+        return dict(zip(STATS_FIELDS, out.tolist()))
+
+    def diverged(self) -> bool:
+        return bool(lib().orc_has_diverged(self.h))
+
+    def pops(self) -> np.ndarray:
+        out = np.empty(self.q * int(np.prod(self.shape)), np.float64)
+        lib().orc_get_pops(self.h, _ptr(out, _c_double_p))
+        return out.reshape(self.q, *self.shape)
+
+    def set_pops(self, pops: np.ndarray):
+        p = np.ascontiguousarray(pops, dtype=np.float64)
+        lib().orc_set_pops(self.h, _ptr(p, _c_double_p))
+
+    def time_steps(self, warmup: int, steps: int) -> float:
+        return lib().orc_time_steps(self.h, warmup, steps)
+
+
+class RefLbm:
+    """FlowCraft's own LbmReference, compiled into oracle/_ref (this container only)."""
+
+    def __init__(self, solid: np.ndarray, sdf: np.ndarray | None, q: int = 19, tau=0.8, u=0.05,
+                 reverse=False, free_slip_y=True, free_slip_z=True, cs=0.12, seed_mid=False):
+        L = ref_lib()
+        nz, ny, nx = solid.shape
+        self.shape = (nz, ny, nx)
+        self._solid = np.ascontiguousarray(solid, dtype=np.uint8)
+        self._sdf = None if sdf is None else np.ascontiguousarray(sdf, dtype=np.float32)
+        self.h = L.ref_create(nx, ny, nz, q, _ptr(self._solid, _c_u8_p),
+                              None if self._sdf is None else _ptr(self._sdf, _c_float_p),
+                              tau, u, int(reverse), int(free_slip_y), int(free_slip_z), cs, int(seed_mid))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref_lib().ref_destroy(self.h)
+            self.h = None
+
+    def set_params(self, tau=0.8, u=0.05, reverse=False, free_slip_y=True, free_slip_z=True, cs=0.12):
+        ref_lib().ref_set_params(self.h, tau, u, int(reverse), int(free_slip_y), int(free_slip_z), cs)
+
+    def reset(self, seed_mid=False):
+        ref_lib().ref_reset(self.h, int(seed_mid))
+
+    def reset_from_field(self, field):
+        f = np.ascontiguousarray(field, dtype=np.float32)
+        ref_lib().ref_reset_from_field(self.h, _ptr(f, _c_float_p))
+
+    def step(self, n=1):
+        ref_lib().ref_step(self.h, n)
+
+    def field(self) -> np.ndarray:
+        out = np.empty(int(np.prod(self.shape)) * 4, np.float32)
+        ref_lib().ref_readback_field(self.h, _ptr(out, _c_float_p))
+        return out
+
+    def stats(self) -> dict:
+        out = np.zeros(11, np.float64)
+        ref_lib().ref_stats(self.h, _ptr(out, _c_double_p))
+        return dict(zip(STATS_FIELDS, out.tolist()))
+
+    def diverged(self) -> bool:
+        return bool(ref_lib().ref_has_diverged(self.h))
+
+    def time_steps(self, warmup: int, steps: int) -> float:
+        return ref_lib().ref_time_steps(self.h, warmup, steps)

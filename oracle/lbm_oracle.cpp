@@ -1,0 +1,713 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// Clean-room CPU restatement of FlowCraft's LbmReference (the CPU twin of the
+// GPU LbmSolver step).  It exists to CHECK the HIP engine in flowcraft_amd/:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// it, and never as the thing measured or shipped.
+//
+// Templated on the arithmetic type:
+//   Real = float  : parity twin of LbmReference (same operations in the same
+//                   order, compiled with -ffp-contract=off so no FMA contraction)
+//   Real = double : the FP64 oracle every GPU result states its error against.
+// For Real=float the population store can also be emulated in the engine's
+// reduced-precision formats (FP16 / BF16 offsets from the rest weights), so the
+// GPU's reduced-storage kernels have an exact CPU twin as well.
+//
+// Followed, function by function (paths relative to the FlowCraft repo):
+//   lattice tables ............ FlowCraft/LatticeModel.h:20-64
+//   Equilibrium ............... FlowCraft/LbmReference.cpp:9-13
+//   ParallelForZ .............. FlowCraft/LbmReference.cpp:24-39
+//   MacroscopicFrom ........... FlowCraft/LbmReference.cpp:43-61
+//   InterpolatedBounceBack .... FlowCraft/LbmReference.cpp:100-122
+//   SolidCellForce ............ FlowCraft/LbmReference.cpp:127-148
+//   Init (mirror search) ...... FlowCraft/LbmReference.cpp:150-190
+//   Reset / ResetFromField .... FlowCraft/LbmReference.cpp:192-239
+//   Step ...................... FlowCraft/LbmReference.cpp:241-469
+//   ReadbackField ............. FlowCraft/LbmReference.cpp:475-500
+//   ComputeStats .............. FlowCraft/LbmReference.cpp:502-585
+// Extensions with no reference counterpart (pinned only by the FP64 build and
+// the invariants in tests/): periodic y/z wrap, FP16/BF16 offset storage.
+//
+// Exposed through a small extern "C" API (orc_*) for ctypes.
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <type_traits>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- lattices
+struct Dir {
+    int ex, ey, ez;
+    float w;  // the reference keeps weights as float literals (1/3, 1/18, ...)
+    double wd;  // exact-as-double weight for the FP64 oracle
+    int opp;
+};
+
+// LatticeModel.h:20-43 (D3Q19) and :45-64 (D3Q15): same order, same pairing.
+const Dir kD3Q19[19] = {
+    {0, 0, 0, 1.0f / 3.0f, 1.0 / 3.0, 0},    {1, 0, 0, 1.0f / 18.0f, 1.0 / 18.0, 2},
+    {-1, 0, 0, 1.0f / 18.0f, 1.0 / 18.0, 1}, {0, 1, 0, 1.0f / 18.0f, 1.0 / 18.0, 4},
+    {0, -1, 0, 1.0f / 18.0f, 1.0 / 18.0, 3}, {0, 0, 1, 1.0f / 18.0f, 1.0 / 18.0, 6},
+    {0, 0, -1, 1.0f / 18.0f, 1.0 / 18.0, 5}, {1, 1, 0, 1.0f / 36.0f, 1.0 / 36.0, 8},
+    {-1, -1, 0, 1.0f / 36.0f, 1.0 / 36.0, 7}, {1, -1, 0, 1.0f / 36.0f, 1.0 / 36.0, 10},
+    {-1, 1, 0, 1.0f / 36.0f, 1.0 / 36.0, 9}, {1, 0, 1, 1.0f / 36.0f, 1.0 / 36.0, 12},
+    {-1, 0, -1, 1.0f / 36.0f, 1.0 / 36.0, 11}, {1, 0, -1, 1.0f / 36.0f, 1.0 / 36.0, 14},
+    {-1, 0, 1, 1.0f / 36.0f, 1.0 / 36.0, 13}, {0, 1, 1, 1.0f / 36.0f, 1.0 / 36.0, 16},
+    {0, -1, -1, 1.0f / 36.0f, 1.0 / 36.0, 15}, {0, 1, -1, 1.0f / 36.0f, 1.0 / 36.0, 18},
+    {0, -1, 1, 1.0f / 36.0f, 1.0 / 36.0, 17},
+};
+const Dir kD3Q15[15] = {
+    {0, 0, 0, 2.0f / 9.0f, 2.0 / 9.0, 0},     {1, 0, 0, 1.0f / 9.0f, 1.0 / 9.0, 2},
+    {-1, 0, 0, 1.0f / 9.0f, 1.0 / 9.0, 1},    {0, 1, 0, 1.0f / 9.0f, 1.0 / 9.0, 4},
+    {0, -1, 0, 1.0f / 9.0f, 1.0 / 9.0, 3},    {0, 0, 1, 1.0f / 9.0f, 1.0 / 9.0, 6},
+    {0, 0, -1, 1.0f / 9.0f, 1.0 / 9.0, 5},    {1, 1, 1, 1.0f / 72.0f, 1.0 / 72.0, 8},
+    {-1, -1, -1, 1.0f / 72.0f, 1.0 / 72.0, 7}, {1, 1, -1, 1.0f / 72.0f, 1.0 / 72.0, 10},
+    {-1, -1, 1, 1.0f / 72.0f, 1.0 / 72.0, 9}, {1, -1, 1, 1.0f / 72.0f, 1.0 / 72.0, 12},
+    {-1, 1, -1, 1.0f / 72.0f, 1.0 / 72.0, 11}, {-1, 1, 1, 1.0f / 72.0f, 1.0 / 72.0, 14},
+    {1, -1, -1, 1.0f / 72.0f, 1.0 / 72.0, 13},
+};
+
+// ------------------------------------------------------ storage emulation
+// IEEE binary32 -> binary16, round-to-nearest-even, subnormals kept, overflow
+// to infinity (the behaviour of a hardware RNE convert; written here from the
+// IEEE 754 definition and checked in tests against numpy.float16).
+uint16_t F32ToF16(float v) {
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    uint32_t sign = (b >> 16) & 0x8000u;
+    uint32_t absb = b & 0x7FFFFFFFu;
+    if (absb > 0x7F800000u) return static_cast<uint16_t>(sign | 0x7E00u | ((absb >> 13) & 0x3FFu));  // NaN
+    if (absb >= 0x477FF000u) return static_cast<uint16_t>(sign | 0x7C00u);  // rounds to >= 65520 -> inf
+    int e = static_cast<int>(absb >> 23) - 127;
+    uint32_t m = (absb & 0x7FFFFFu) | 0x800000u;
+    if (e >= -14) {
+        // normal half: keep 10 mantissa bits
+        uint32_t hm = (absb & 0x7FFFFFu) >> 13;
+        uint32_t rest = absb & 0x1FFFu;
+        uint32_t h = (static_cast<uint32_t>(e + 15) << 10) | hm;
+        if (rest > 0x1000u || (rest == 0x1000u && (h & 1u))) h += 1;  // carry may bump the exponent: still correct
+        return static_cast<uint16_t>(sign | h);
+    }
+    if (e < -25) return static_cast<uint16_t>(sign);  // below half the smallest subnormal -> 0
+    // subnormal half: value = m * 2^(e-23); half subnormal unit = 2^-24
+    int shift = -e - 1;  // number of bits to drop from the 24-bit significand to land on 2^-24 units
+    uint32_t hm = m >> shift;
+    uint32_t rest = m & ((1u << shift) - 1u);
+    uint32_t halfway = 1u << (shift - 1);
+    if (rest > halfway || (rest == halfway && (hm & 1u))) hm += 1;
+    return static_cast<uint16_t>(sign | hm);
+}
+
+float F16ToF32(uint16_t h) {
+    uint32_t sign = static_cast<uint32_t>(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1Fu;
+    uint32_t m = h & 0x3FFu;
+    uint32_t b;
+    if (e == 0) {
+        if (m == 0) {
+            b = sign;
+        } else {  // subnormal: m * 2^-24, exact in binary32
+            float f = static_cast<float>(m) * 5.9604644775390625e-08f;
+            std::memcpy(&b, &f, 4);
+            b |= sign;
+        }
+    } else if (e == 31) {
+        b = sign | 0x7F800000u | (m << 13);
+    } else {
+        b = sign | ((e - 15 + 127) << 23) | (m << 13);
+    }
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// binary32 -> bfloat16, round-to-nearest-even (NaN kept quiet).
+uint16_t F32ToBF16(float v) {
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    if ((b & 0x7FFFFFFFu) > 0x7F800000u) return static_cast<uint16_t>((b >> 16) | 0x40u);
+    uint32_t lsb = (b >> 16) & 1u;
+    b += 0x7FFFu + lsb;
+    return static_cast<uint16_t>(b >> 16);
+}
+
+float BF16ToF32(uint16_t h) {
+    uint32_t b = static_cast<uint32_t>(h) << 16;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// Storage formats of the engine: 0 = native Real, 1 = FP16 offset, 2 = BF16 offset.
+// Offset formats hold g = (f - w) * S and decode f = w + g * (1/S).
+constexpr float kF16Scale = 4096.0f;
+constexpr float kF16InvScale = 1.0f / 4096.0f;
+
+// A decimal constant of the reference as the float literal (Real=float, so the
+// value is the one the reference's `...f` literal rounds to) or as a double.
+template <typename R>
+constexpr R K(float f, double d) {
+    return std::is_same<R, float>::value ? static_cast<R>(f) : static_cast<R>(d);
+}
+
+template <typename Real>
+struct Oracle {
+    int nx = 0, ny = 0, nz = 0, Q = 0;
+    size_t N = 0;
+    int storage = 0;
+    int nthreads = 1;
+    bool periodicY = false, periodicZ = false;
+    // Params (LbmReference.h:41-52)
+    float tau = 0.8f, inletVelocity = 0.05f, cs = 0.12f;
+    bool reverseFlow = false, freeSlipY = true, freeSlipZ = true;
+
+    std::vector<int> ex, ey, ez, opp, mirrorY, mirrorZ;
+    std::vector<Real> w;
+    std::vector<float> wf;
+    std::vector<uint8_t> solid;
+    std::vector<float> sdf;  // empty = no SDF
+    std::vector<Real> f[2];
+    int cur = 0;
+    std::atomic<bool> diverged{false};
+
+    size_t Cell(int x, int y, int z) const { return (static_cast<size_t>(z) * ny + y) * nx + x; }
+    size_t Pop(int d, size_t c) const { return static_cast<size_t>(d) * N + c; }
+
+    // Bounds test with the periodic extension: y/z wrap in place when periodic.
+    bool Resolve(int& x, int& y, int& z) const {
+        if (x < 0 || x >= nx) return false;
+        if (y < 0 || y >= ny) {
+            if (!periodicY) return false;
+            y = (y + ny) % ny;
+        }
+        if (z < 0 || z >= nz) {
+            if (!periodicZ) return false;
+            z = (z + nz) % nz;
+        }
+        return true;
+    }
+
+    void Init(int nx_, int ny_, int nz_, int q) {
+        nx = nx_;
+        ny = ny_;
+        nz = nz_;
+        N = static_cast<size_t>(nx) * ny * nz;
+        const Dir* t = q == 15 ? kD3Q15 : kD3Q19;
+        Q = q == 15 ? 15 : 19;
+        ex.resize(Q);
+        ey.resize(Q);
+        ez.resize(Q);
+        opp.resize(Q);
+        w.resize(Q);
+        wf.resize(Q);
+        for (int i = 0; i < Q; ++i) {
+            ex[i] = t[i].ex;
+            ey[i] = t[i].ey;
+            ez[i] = t[i].ez;
+            opp[i] = t[i].opp;
+            wf[i] = t[i].w;
+            w[i] = sizeof(Real) == 4 ? static_cast<Real>(t[i].w) : static_cast<Real>(t[i].wd);
+        }
+        // Mirror tables found by search (LbmReference.cpp:172-183).
+        mirrorY.assign(Q, 0);
+        mirrorZ.assign(Q, 0);
+        for (int i = 0; i < Q; ++i)
+            for (int j = 0; j < Q; ++j) {
+                if (ex[j] == ex[i] && ey[j] == -ey[i] && ez[j] == ez[i]) mirrorY[i] = j;
+                if (ex[j] == ex[i] && ey[j] == ey[i] && ez[j] == -ez[i]) mirrorZ[i] = j;
+            }
+        solid.assign(N, 0);
+        sdf.clear();
+        f[0].assign(static_cast<size_t>(Q) * N, Real(0));
+        f[1].assign(static_cast<size_t>(Q) * N, Real(0));
+        cur = 0;
+    }
+
+    template <typename Body>
+    void ParallelForZ(Body body) const {
+        int nt = std::max(1, std::min(nthreads, nz));
+        int per = (nz + nt - 1) / nt;
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) {
+            int z0 = t * per, z1 = std::min(z0 + per, nz);
+            if (z0 >= z1) break;
+            th.emplace_back(body, z0, z1, t);
+        }
+        for (auto& x : th) x.join();
+    }
+
+    // Quantise a post-collision value through the emulated storage format.
+    Real Store(Real v, int d) const {
+        if (storage == 0) return v;
+        float vf = static_cast<float>(v);
+        if (storage == 1) {
+            float g = (vf - wf[d]) * kF16Scale;
+            return static_cast<Real>(wf[d] + F16ToF32(F32ToF16(g)) * kF16InvScale);
+        }
+        float g = vf - wf[d];
+        return static_cast<Real>(wf[d] + BF16ToF32(F32ToBF16(g)));
+    }
+
+    // LbmReference.cpp:9-13, same association: w*rho*(((1+3eu)+(4.5eu)eu)-1.5u2)
+    Real Feq(int i, Real rho, Real ux, Real uy, Real uz) const {
+        Real eu = static_cast<Real>(ex[i]) * ux + static_cast<Real>(ey[i]) * uy + static_cast<Real>(ez[i]) * uz;
+        Real u2 = ux * ux + uy * uy + uz * uz;
+        return w[i] * rho * (Real(1) + Real(3) * eu + Real(4.5) * eu * eu - Real(1.5) * u2);
+    }
+
+    // LbmReference.cpp:43-61
+    void Moments(const std::vector<Real>& fv, size_t c, Real& rho, Real& ux, Real& uy, Real& uz) const {
+        rho = ux = uy = uz = Real(0);
+        for (int i = 0; i < Q; ++i) {
+            Real fi = fv[Pop(i, c)];
+            rho += fi;
+            ux += fi * static_cast<Real>(ex[i]);
+            uy += fi * static_cast<Real>(ey[i]);
+            uz += fi * static_cast<Real>(ez[i]);
+        }
+        if (rho > K<Real>(1e-6f, 1e-6)) {
+            ux /= rho;
+            uy /= rho;
+            uz /= rho;
+        }
+    }
+
+    static Real Clamp01(Real v) { return v < Real(0) ? Real(0) : (Real(1) < v ? Real(1) : v); }
+
+    // LbmReference.cpp:100-122.  (x,y,z) fluid, (sx,sy,sz) its solid direction-i source.
+    Real Bouzidi(const std::vector<Real>& fv, int x, int y, int z, size_t c, int i, int sx, int sy, int sz) const {
+        int o = opp[i];
+        Real plain = fv[Pop(o, c)];
+        if (sdf.empty()) return plain;
+        float phiF = sdf[c];
+        float phiS = sdf[Cell(sx, sy, sz)];
+        if (!std::isfinite(phiF) || !std::isfinite(phiS)) return plain;
+        Real denom = static_cast<Real>(phiF) - static_cast<Real>(phiS);
+        if (!(denom > K<Real>(1e-4f, 1e-4))) return plain;
+        Real q = Clamp01(static_cast<Real>(phiF) / denom);
+        if (q < Real(0.5)) {
+            int fx = x + ex[i], fy = y + ey[i], fz = z + ez[i];
+            if (!Resolve(fx, fy, fz) || solid[Cell(fx, fy, fz)]) return plain;
+            return Real(2) * q * plain + (Real(1) - Real(2) * q) * fv[Pop(o, Cell(fx, fy, fz))];
+        }
+        Real inv2q = Real(1) / (Real(2) * q);
+        return inv2q * plain + (Real(1) - inv2q) * fv[Pop(i, c)];
+    }
+
+    // LbmReference.cpp:127-148 (force of the CURRENT state).
+    void SolidForce(const std::vector<Real>& fv, int x, int y, int z, Real& stress, Real& fx, Real& fy,
+                    Real& fz) const {
+        fx = fy = fz = Real(0);
+        Real wsum = Real(0);
+        for (int i = 1; i < Q; ++i) {
+            int nxf = x + ex[i], nyf = y + ey[i], nzf = z + ez[i];
+            if (!Resolve(nxf, nyf, nzf) || solid[Cell(nxf, nyf, nzf)]) continue;
+            size_t fc = Cell(nxf, nyf, nzf);
+            Real plain = fv[Pop(opp[i], fc)];
+            Real fb = Bouzidi(fv, nxf, nyf, nzf, fc, i, x, y, z);
+            Real dp = (plain + fb) - Real(2) * w[i];
+            fx -= static_cast<Real>(ex[i]) * dp;
+            fy -= static_cast<Real>(ey[i]) * dp;
+            fz -= static_cast<Real>(ez[i]) * dp;
+            wsum += w[i];
+        }
+        Real mag = std::sqrt(fx * fx + fy * fy + fz * fz);
+        stress = wsum > Real(0) ? mag / (Real(6) * wsum) : Real(0);
+    }
+
+    void FillUniform(Real rho, Real ux, Real uy, Real uz) {
+        std::vector<Real> fe(Q);
+        for (int i = 0; i < Q; ++i) fe[i] = Store(Feq(i, rho, ux, uy, uz), i);
+        for (int i = 0; i < Q; ++i) std::fill(f[0].begin() + Pop(i, 0), f[0].begin() + Pop(i, 0) + N, fe[i]);
+        cur = 0;
+        diverged = false;
+    }
+
+    // LbmReference.cpp:192-217
+    void Reset(bool mid) {
+        Real rho = Real(1), ux = Real(0);
+        if (mid) {
+            Real u = static_cast<Real>(inletVelocity);
+            Real midSpeed = Real(0.5) * u;
+            ux = reverseFlow ? -midSpeed : midSpeed;
+            Real dev = Real(0.5) * (u * u - midSpeed * midSpeed);
+            rho = Real(1) + Real(3) * dev;
+        }
+        FillUniform(rho, ux, Real(0), Real(0));
+    }
+
+    // LbmReference.cpp:219-239
+    void ResetFromField(const float* field) {
+        diverged = false;
+        ParallelForZ([&](int z0, int z1, int) {
+            for (size_t c = static_cast<size_t>(z0) * nx * ny; c < static_cast<size_t>(z1) * nx * ny; ++c) {
+                Real rho = solid[c] ? Real(1) : static_cast<Real>(field[c * 4 + 0]);
+                Real ux = solid[c] ? Real(0) : static_cast<Real>(field[c * 4 + 1]);
+                Real uy = solid[c] ? Real(0) : static_cast<Real>(field[c * 4 + 2]);
+                Real uz = solid[c] ? Real(0) : static_cast<Real>(field[c * 4 + 3]);
+                for (int i = 0; i < Q; ++i) f[0][Pop(i, c)] = Store(Feq(i, rho, ux, uy, uz), i);
+            }
+        });
+        cur = 0;
+    }
+
+    // LbmReference.cpp:241-469
+    void Step() {
+        const std::vector<Real>& fin = f[cur];
+        std::vector<Real>& fout = f[1 - cur];
+        const int inletX = reverseFlow ? nx - 1 : 0;
+        const int outletX = reverseFlow ? 0 : nx - 1;
+        const Real inletUx = static_cast<Real>(reverseFlow ? -inletVelocity : inletVelocity);
+        const Real rtau = static_cast<Real>(tau);
+        const Real rcs = static_cast<Real>(cs);
+        const Real kCs2 = Real(1) / Real(3);
+        const Real kSmag = Real(18) * K<Real>(1.41421356f, 1.41421356);
+        ParallelForZ([&](int z0, int z1, int) {
+            bool localDiv = false;
+            Real fpre[19];
+            for (int z = z0; z < z1; ++z)
+                for (int y = 0; y < ny; ++y)
+                    for (int x = 0; x < nx; ++x) {
+                        size_t c = Cell(x, y, z);
+                        if (solid[c]) {
+                            for (int i = 0; i < Q; ++i) fout[Pop(i, c)] = w[i];
+                            continue;
+                        }
+                        if (x == inletX) {
+                            for (int i = 0; i < Q; ++i)
+                                fout[Pop(i, c)] = Store(Feq(i, Real(1), inletUx, Real(0), Real(0)), i);
+                            continue;
+                        }
+                        if (x == outletX) {
+                            int nbx = reverseFlow ? x + 1 : x - 1;
+                            Real nr, nu, nv, nw;
+                            Moments(fin, Cell(nbx, y, z), nr, nu, nv, nw);
+                            Real rho = nr > K<Real>(1e-6f, 1e-6) ? Real(1) + K<Real>(0.999f, 0.999) * (nr - Real(1))
+                                                                 : Real(1);
+                            for (int i = 0; i < Q; ++i) fout[Pop(i, c)] = Store(Feq(i, rho, nu, nv, nw), i);
+                            continue;
+                        }
+                        for (int i = 0; i < Q; ++i) {
+                            int sx = x - ex[i], sy = y - ey[i], sz = z - ez[i];
+                            int rx = sx, ry = sy, rz = sz;
+                            bool inb = Resolve(rx, ry, rz);
+                            if (inb && !solid[Cell(rx, ry, rz)]) {
+                                fpre[i] = fin[Pop(i, Cell(rx, ry, rz))];
+                            } else if (inb) {
+                                fpre[i] = Bouzidi(fin, x, y, z, c, i, rx, ry, rz);
+                            } else {
+                                bool outY = sy < 0 || sy >= ny;
+                                bool outZ = sz < 0 || sz >= nz;
+                                bool ok = (!outY || freeSlipY) && (!outZ || freeSlipZ);
+                                if ((outY || outZ) && ok) {
+                                    int d = i;
+                                    if (outY) d = mirrorY[d];
+                                    if (outZ) d = mirrorZ[d];
+                                    fpre[i] = fin[Pop(d, c)];
+                                } else {
+                                    fpre[i] = fin[Pop(opp[i], c)];
+                                }
+                            }
+                        }
+                        Real rho = 0, ux = 0, uy = 0, uz = 0;
+                        for (int i = 0; i < Q; ++i) {
+                            rho += fpre[i];
+                            ux += fpre[i] * static_cast<Real>(ex[i]);
+                            uy += fpre[i] * static_cast<Real>(ey[i]);
+                            uz += fpre[i] * static_cast<Real>(ez[i]);
+                        }
+                        if (rho > K<Real>(1e-6f, 1e-6)) {
+                            ux /= rho;
+                            uy /= rho;
+                            uz /= rho;
+                        }
+                        if (!std::isfinite(rho) || !std::isfinite(ux) || !std::isfinite(uy) || !std::isfinite(uz) ||
+                            rho <= Real(0) || (ux * ux + uy * uy + uz * uz) > Real(4))
+                            localDiv = true;
+                        Real pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+                        for (int i = 0; i < Q; ++i) {
+                            Real feq = Feq(i, rho, ux, uy, uz);
+                            Real fneq = fpre[i] - feq;
+                            fpre[i] = feq;
+                            Real fx = static_cast<Real>(ex[i]), fy = static_cast<Real>(ey[i]),
+                                 fz = static_cast<Real>(ez[i]);
+                            pxx += fx * fx * fneq;
+                            pyy += fy * fy * fneq;
+                            pzz += fz * fz * fneq;
+                            pxy += fx * fy * fneq;
+                            pxz += fx * fz * fneq;
+                            pyz += fy * fz * fneq;
+                        }
+                        Real piMag =
+                            std::sqrt(pxx * pxx + pyy * pyy + pzz * pzz + Real(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
+                        Real rmax = rho < K<Real>(1e-6f, 1e-6) ? K<Real>(1e-6f, 1e-6) : rho;  // std::max(rho, 1e-6f)
+                        Real tauEff = Real(0.5) * (rtau + std::sqrt(rtau * rtau + kSmag * rcs * rcs * piMag / rmax));
+                        Real omega = Real(1) / tauEff;
+                        for (int i = 0; i < Q; ++i) {
+                            Real fx = static_cast<Real>(ex[i]), fy = static_cast<Real>(ey[i]),
+                                 fz = static_cast<Real>(ez[i]);
+                            Real qxx = fx * fx - kCs2, qyy = fy * fy - kCs2, qzz = fz * fz - kCs2;
+                            Real qxy = fx * fy, qxz = fx * fz, qyz = fy * fz;
+                            Real reg = (w[i] * Real(4.5)) * (qxx * pxx + qyy * pyy + qzz * pzz + Real(2) * qxy * pxy +
+                                                             Real(2) * qxz * pxz + Real(2) * qyz * pyz);
+                            fout[Pop(i, c)] = Store(fpre[i] + (Real(1) - omega) * reg, i);
+                        }
+                    }
+            if (localDiv) diverged = true;
+        });
+        cur = 1 - cur;
+    }
+
+    // LbmReference.cpp:475-500
+    template <typename Out>
+    void ReadbackField(Out* out) const {
+        const std::vector<Real>& fc = f[cur];
+        ParallelForZ([&](int z0, int z1, int) {
+            for (int z = z0; z < z1; ++z)
+                for (int y = 0; y < ny; ++y)
+                    for (int x = 0; x < nx; ++x) {
+                        size_t c = Cell(x, y, z);
+                        Real a, b, d, e;
+                        if (solid[c])
+                            SolidForce(fc, x, y, z, a, b, d, e);
+                        else
+                            Moments(fc, c, a, b, d, e);
+                        out[c * 4 + 0] = static_cast<Out>(a);
+                        out[c * 4 + 1] = static_cast<Out>(b);
+                        out[c * 4 + 2] = static_cast<Out>(d);
+                        out[c * 4 + 3] = static_cast<Out>(e);
+                    }
+        });
+    }
+
+    // LbmReference.cpp:502-585.  out[11]: avgDensity, minDensity, maxDensity, avgSpeed,
+    // maxSpeed, forceX, forceY, forceZ, maxStress, totalMass, hasNaN.
+    // Partial sums are kept per z-layer and merged in layer order, so the result does
+    // not depend on the thread count.
+    void Stats(double* out) const {
+        struct Part {
+            Real minD = Real(1e30), maxD = Real(-1e30), maxS = 0, maxStress = 0;
+            double dsum = 0, ssum = 0, mass = 0, fx = 0, fy = 0, fz = 0;
+            size_t n = 0;
+            bool nan = false;
+        };
+        std::vector<Part> parts(nz);
+        const std::vector<Real>& fc = f[cur];
+        ParallelForZ([&](int z0, int z1, int) {
+            for (int z = z0; z < z1; ++z) {
+                Part& p = parts[z];
+                for (int y = 0; y < ny; ++y)
+                    for (int x = 0; x < nx; ++x) {
+                        size_t c = Cell(x, y, z);
+                        if (solid[c]) {
+                            Real st, fx, fy, fz;
+                            SolidForce(fc, x, y, z, st, fx, fy, fz);
+                            if (std::isfinite(st) && std::isfinite(fx) && std::isfinite(fy) && std::isfinite(fz)) {
+                                p.fx += static_cast<double>(fx);
+                                p.fy += static_cast<double>(fy);
+                                p.fz += static_cast<double>(fz);
+                                p.maxStress = std::max(p.maxStress, st);
+                            }
+                            continue;
+                        }
+                        Real rho, ux, uy, uz;
+                        Moments(fc, c, rho, ux, uy, uz);
+                        if (!std::isfinite(rho) || !std::isfinite(ux) || !std::isfinite(uy) || !std::isfinite(uz)) {
+                            p.nan = true;
+                            continue;
+                        }
+                        Real sp = std::sqrt(ux * ux + uy * uy + uz * uz);
+                        p.minD = std::min(p.minD, rho);
+                        p.maxD = std::max(p.maxD, rho);
+                        p.maxS = std::max(p.maxS, sp);
+                        p.dsum += static_cast<double>(rho);
+                        p.ssum += static_cast<double>(sp);
+                        p.mass += static_cast<double>(rho);
+                        ++p.n;
+                    }
+            }
+        });
+        double minD = 1e30, maxD = -1e30, maxS = 0, maxStress = 0, dsum = 0, ssum = 0, mass = 0, fx = 0, fy = 0,
+               fz = 0;
+        size_t n = 0;
+        bool nan = false;
+        for (const Part& p : parts) {
+            nan = nan || p.nan;
+            maxStress = std::max(maxStress, static_cast<double>(p.maxStress));
+            fx += p.fx;
+            fy += p.fy;
+            fz += p.fz;
+            if (p.n == 0) continue;
+            minD = std::min(minD, static_cast<double>(p.minD));
+            maxD = std::max(maxD, static_cast<double>(p.maxD));
+            maxS = std::max(maxS, static_cast<double>(p.maxS));
+            dsum += p.dsum;
+            ssum += p.ssum;
+            mass += p.mass;
+            n += p.n;
+        }
+        out[0] = n ? dsum / static_cast<double>(n) : 0.0;
+        out[1] = n ? minD : 0.0;
+        out[2] = n ? maxD : 0.0;
+        out[3] = n ? ssum / static_cast<double>(n) : 0.0;
+        out[4] = maxS;
+        out[5] = fx;
+        out[6] = fy;
+        out[7] = fz;
+        out[8] = maxStress;
+        out[9] = mass;
+        out[10] = nan ? 1.0 : 0.0;
+    }
+};
+
+struct Handle {
+    int bits = 32;
+    Oracle<float>* f32 = nullptr;
+    Oracle<double>* f64 = nullptr;
+};
+
+template <typename Fn>
+void Dispatch(Handle* h, Fn fn) {
+    if (h->bits == 64)
+        fn(*h->f64);
+    else
+        fn(*h->f32);
+}
+
+}  // namespace
+
+extern "C" {
+
+// real_bits: 32 or 64.  storage: 0 native, 1 FP16 offset, 2 BF16 offset (32-bit only).
+void* orc_create(int nx, int ny, int nz, int q, int real_bits, int storage, int periodic_y, int periodic_z,
+                 int nthreads) {
+    if (nx < 3 || ny < 1 || nz < 1 || (q != 15 && q != 19)) return nullptr;
+    if (real_bits == 64 && storage != 0) return nullptr;
+    Handle* h = new Handle();
+    h->bits = real_bits == 64 ? 64 : 32;
+    if (h->bits == 64)
+        h->f64 = new Oracle<double>();
+    else
+        h->f32 = new Oracle<float>();
+    int nt = nthreads > 0 ? nthreads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+    Dispatch(h, [&](auto& o) {
+        o.Init(nx, ny, nz, q);
+        o.storage = storage;
+        o.periodicY = periodic_y != 0;
+        o.periodicZ = periodic_z != 0;
+        o.nthreads = nt;
+    });
+    return h;
+}
+
+void orc_destroy(void* p) {
+    Handle* h = static_cast<Handle*>(p);
+    if (!h) return;
+    delete h->f32;
+    delete h->f64;
+    delete h;
+}
+
+void orc_set_geometry(void* p, const uint8_t* solid, const float* sdf) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) {
+        o.solid.assign(solid, solid + o.N);
+        if (sdf)
+            o.sdf.assign(sdf, sdf + o.N);
+        else
+            o.sdf.clear();
+    });
+}
+
+void orc_set_params(void* p, float tau, float u, int reverse, int fsy, int fsz, float cs) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) {
+        o.tau = tau;
+        o.inletVelocity = u;
+        o.reverseFlow = reverse != 0;
+        o.freeSlipY = fsy != 0;
+        o.freeSlipZ = fsz != 0;
+        o.cs = cs;
+    });
+}
+
+void orc_reset(void* p, int mid) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) { o.Reset(mid != 0); });
+}
+
+void orc_reset_from_field(void* p, const float* field) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) { o.ResetFromField(field); });
+}
+
+void orc_step(void* p, int n) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) {
+        for (int i = 0; i < n; ++i) o.Step();
+    });
+}
+
+void orc_readback_field_f32(void* p, float* out) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) { o.ReadbackField(out); });
+}
+
+void orc_readback_field_f64(void* p, double* out) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) { o.ReadbackField(out); });
+}
+
+void orc_stats(void* p, double* out) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) { o.Stats(out); });
+}
+
+int orc_has_diverged(void* p) {
+    int r = 0;
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) { r = o.diverged ? 1 : 0; });
+    return r;
+}
+
+// Current post-collision populations f[d*N + cell] as double.
+void orc_get_pops(void* p, double* out) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) {
+        const auto& fc = o.f[o.cur];
+        for (size_t k = 0; k < fc.size(); ++k) out[k] = static_cast<double>(fc[k]);
+    });
+}
+
+// Overwrite the current populations (f[d*N + cell]); values pass through the storage format.
+void orc_set_pops(void* p, const double* in) {
+    Dispatch(static_cast<Handle*>(p), [&](auto& o) {
+        auto& fc = o.f[o.cur];
+        for (int d = 0; d < o.Q; ++d)
+            for (size_t c = 0; c < o.N; ++c) {
+                using R = typename std::remove_reference<decltype(fc[0])>::type;
+                fc[o.Pop(d, c)] = o.Store(static_cast<R>(in[o.Pop(d, c)]), d);
+            }
+    });
+}
+
+// Storage-format helpers, exported so tests can pin them against numpy.
+uint16_t orc_f32_to_f16(float v) { return F32ToF16(v); }
+float orc_f16_to_f32(uint16_t h) { return F16ToF32(h); }
+uint16_t orc_f32_to_bf16(float v) { return F32ToBF16(v); }
+float orc_bf16_to_f32(uint16_t h) { return BF16ToF32(h); }
+void orc_f32_to_f16_array(const float* in, uint16_t* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = F32ToF16(in[i]);
+}
+void orc_f32_to_bf16_array(const float* in, uint16_t* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = F32ToBF16(in[i]);
+}
+float orc_f16_scale() { return kF16Scale; }
+
+// Times `steps` oracle steps after `warmup` untimed ones; returns seconds.
+double orc_time_steps(void* p, int warmup, int steps) {
+    orc_step(p, warmup);
+    auto t0 = std::chrono::steady_clock::now();
+    orc_step(p, steps);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"
