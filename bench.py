@@ -1,0 +1,178 @@
+"""Benchmark of the LbmSolver step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--layout inplace|pingpong]
+
+Workload (the 512x256x256 configuration the metric is quoted on): D3Q19
+regularized BGK + Smagorinsky (Cs = 0.1), NACA 0012 wing of chord 128 cells,
+leading edge at x = 128, angle of attack 4 deg, extruded over all of z; inlet
+u = 0.08, tau = 0.5005, free-slip y/z walls; uniform rest start.  Synthetic,
+deterministic geometry.  A "step" is one fused stream-collide pass over all
+33.5M cells; MLUPS counts every cell (solid included), the convention of
+FlowCraft/main.cpp:3781-3783.
+
+`value` is the FP32-storage throughput; FP16 and BF16 offset storage, and D3Q15,
+are reported under `variants`.  Timing: W untimed steps, then K steps bracketed
+by HIP events recorded on the solver's own stream, after a device synchronise on
+both sides.  The roofline uses the algorithmic 2*Q*s bytes per cell update.
+The CPU baseline times FlowCraft's LbmReference::Step (oracle/_ref) — or the
+oracle's restatement where that build is absent — on the host cores, over a
+bounded number of steps of the same lattice.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "D3Q19 MLUPS & % HBM peak; wall time to converged Cd, learned vs plain"
+NX, NY, NZ = 512, 256, 256
+
+
+def wing_grid():
+    from flowcraft_amd import geometry
+    return geometry.naca_wing_grid(NX, NY, NZ, m=0.0, p=0.0, t=0.12, chord=128.0, le_x=128.0, le_y=NY / 2.0,
+                                   aoa_deg=4.0)
+
+
+def time_variant(torch, grid, model, storage, layout, steps, warmup):
+    from flowcraft_amd.lbm import LbmSolver, Params
+    s = LbmSolver()
+    if not s.Init(None, grid, model, storage=storage, layout=layout):
+        raise RuntimeError(s.LastError())
+    s.Reset(Params(tau=0.5005, inletVelocity=0.08, smagorinskyCs=0.1), False)
+    stream = torch.cuda.ExternalStream(s.Stream())
+    s.Step(None, warmup)
+    s.Sync()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    s.Step(None, steps)
+    e1.record(stream)
+    s.Sync()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    st = s.ReadbackStats()
+    out = {"ms_per_step": ms / steps, "mlups": grid.CellCount() * steps / (ms * 1e-3) / 1e6,
+           "finite": not st.hasNaN, "population_bytes": s.PopulationBytes()}
+    s.Shutdown()
+    return out
+
+
+def cpu_baseline(grid, budget_s=15.0, max_steps=20):
+    """FlowCraft's LbmReference (oracle/_ref) if built, else the oracle restatement,
+    on all host threads, over a bounded number of steps of the same lattice."""
+    import oracle
+    p = dict(tau=0.5005, u=0.08, cs=0.1)
+    cores = os.cpu_count() or 1
+    if oracle.ref_available():
+        sim = oracle.RefLbm(grid.solid, grid.signedDistance, q=19, tau=p["tau"], u=p["u"], cs=p["cs"])
+        kind = "reference"
+    else:
+        sim = oracle.OracleLbm(grid.solid, grid.signedDistance, q=19, bits=32, nthreads=cores)
+        sim.set_params(p["tau"], p["u"], False, True, True, p["cs"])
+        sim.reset(False)
+        kind = "port"
+    t1 = sim.time_steps(0, 1)
+    n = int(max(1, min(max_steps, budget_s / max(t1, 1e-3))))
+    t = sim.time_steps(0, n)
+    mlups = grid.CellCount() * n / t / 1e6
+    del sim
+    return {"value": round(mlups, 2), "unit": "MLUPS", "cores": cores, "kind": kind,
+            "sample": f"{n} steps (after 1 untimed) of the full {NX}x{NY}x{NZ} D3Q19 FP32 lattice, all host threads"}
+
+
+def load_traffic(name):
+    """Per-launch HBM bytes of the step kernel from the committed rocprofv3 --pmc
+    summaries (profiles/), if they exist for this workload."""
+    p = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--layout", default="pingpong", choices=["inplace", "pingpong"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the FP32 D3Q19 step kernel for --steps steps (for rocprofv3 runs); "
+                         "prints the HIP-event average step time")
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
+                    help="storage of the --profile run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this engine runs the lattice on a single MI355X "
+              f"(--gpus {args.gpus} is not supported).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from flowcraft_amd.lbm import LatticeModel, Layout, Storage
+
+    torch.cuda.init()
+    layout = Layout.INPLACE if args.layout == "inplace" else Layout.PINGPONG
+    t0 = time.time()
+    grid = wing_grid()
+    print(f"# geometry {time.time() - t0:.1f}s, solid cells {int(grid.solid.sum())}", file=sys.stderr, flush=True)
+
+    if args.profile:
+        st = {"f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16}[args.storage]
+        r = time_variant(torch, grid, LatticeModel.D3Q19, st, layout, args.steps, args.warmup)
+        b = 4 if st == Storage.F32 else 2
+        print(json.dumps({"profile": f"D3Q19 {args.storage} {layout.name.lower()}", "steps": args.steps,
+                          "ms_per_step": r["ms_per_step"], "mlups": r["mlups"],
+                          "algorithmic_bytes_per_launch": 2 * 19 * b * grid.CellCount(),
+                          "achieved_GBs": r["mlups"] * 1e6 * 2 * 19 * b / 1e9}), flush=True)
+        return
+    main_r = time_variant(torch, grid, LatticeModel.D3Q19, Storage.F32, layout, args.steps, args.warmup)
+    print(f"# D3Q19 F32 {layout.name}: {main_r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    variants = {}
+    if not args.no_variants:
+        for model, storage in ((LatticeModel.D3Q19, Storage.F16), (LatticeModel.D3Q19, Storage.BF16),
+                               (LatticeModel.D3Q15, Storage.F32), (LatticeModel.D3Q15, Storage.F16)):
+            r = time_variant(torch, grid, model, storage, layout, args.steps, args.warmup)
+            q = 19 if model == LatticeModel.D3Q19 else 15
+            b = 4 if storage == Storage.F32 else 2
+            gbs = r["mlups"] * 1e6 * 2 * q * b / 1e9
+            variants[f"{model.name}_{storage.name}"] = {
+                "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+                "bytes_per_cell": 2 * q * b, "achieved_GBs": round(gbs, 1),
+                "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+            print(f"# {model.name} {storage.name}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        variants["fp16_over_fp32"] = round(variants["D3Q19_F16"]["mlups"] / main_r["mlups"], 3)
+
+    bytes_per_cell = 2 * 19 * 4
+    achieved = main_r["mlups"] * 1e6 * bytes_per_cell / 1e9
+    traffic = load_traffic(f"lbm_step_d3q19_f32_{layout.name.lower()}_512x256x256")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
+                "algorithmic_bytes_per_launch": bytes_per_cell * grid.CellCount(),
+                "kernel": f"fc::KStep<19, StoreF32, {layout.name.title().replace('place', 'Place')}>"}
+    cpu = None if args.no_cpu else cpu_baseline(grid)
+    line = {
+        "metric": METRIC, "value": round(main_r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(main_r["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"D3Q19 Smagorinsky Cs=0.1 regularized BGK, FP32 storage, {NX}x{NY}x{NZ} "
+                               f"(33.5M cells), NACA 0012 chord 128 AoA 4 deg, u=0.08, tau=0.5005, "
+                               f"free-slip y/z, {layout.name.lower()} layout",
+                   "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "finite": main_r["finite"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

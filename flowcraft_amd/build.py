@@ -26,7 +26,28 @@ COMMON = ["--offload-arch=" + ARCH, "-std=c++20", "-O3", "-fPIC", "-shared", "-f
 
 TARGETS = {
     "libfc_lbm.so": ["fc_lbm.hip"],
+    "libfc_nn.so": ["nn_kernels.hip"],
 }
+EXECUTABLES = {
+    "hbm_probe": ["hbm_probe.hip"],
+}
+
+
+def build_dropin_test(verbose: bool = False) -> str:
+    """Compiles tests/cpp/dropin_test.cpp against include/flowcraft/LbmSolver.h, linking
+    libfc_lbm.so and the oracle (a test binary; it needs a GPU to run)."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "dropin_test")
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include", "flowcraft"), "-o", out, src,
+           "-L", LIB_DIR, "-lfc_lbm", "-L", os.path.join(ROOT, "oracle"), "-llbm_oracle",
+           "-Wl,-rpath," + LIB_DIR + ":" + os.path.join(ROOT, "oracle"), "-Wl,--disable-new-dtags"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed for the drop-in test")
+    return out
 
 
 def build(verbose: bool = False) -> list[str]:
@@ -43,6 +64,17 @@ def build(verbose: bool = False) -> list[str]:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"hipcc failed for {lib}")
         os.replace(tmp, out)
+        built.append(out)
+    for exe, srcs in EXECUTABLES.items():
+        out = os.path.join(LIB_DIR, exe)
+        cmd = [HIPCC] + [c for c in COMMON if c not in ("-shared", "-fPIC")] + ["-o", out] + \
+              [os.path.join(CSRC, s) for s in srcs]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed for {exe}")
         built.append(out)
     return built
 

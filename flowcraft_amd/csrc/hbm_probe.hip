@@ -1,0 +1,127 @@
+// hbm_probe: measures HBM copy bandwidth on the current device, and serves as the
+// known-size reference kernel for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE.
+//
+//   hbm_probe [GiB per buffer (default 2)] [repeats (default 20)]
+//
+// Kernels (each reads one buffer and writes another of the same size):
+//   copy16    16 bytes per lane (float4)
+//   copy4      4 bytes per lane
+//   copy2      2 bytes per lane
+//   pull19_4   the D3Q19 pull pattern without arithmetic: 19 SoA planes of 4-byte
+//              values, each lane reads plane d at x - e_d and writes plane d at x
+//   pull19_2   same with 2-byte values
+// Prints one line per kernel: bytes moved per launch and GB/s (HIP events).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#define CHECK(x)                                                                          \
+    do {                                                                                  \
+        hipError_t e = (x);                                                               \
+        if (e != hipSuccess) {                                                            \
+            std::fprintf(stderr, "%s failed: %s\n", #x, hipGetErrorString(e));           \
+            std::exit(1);                                                                 \
+        }                                                                                 \
+    } while (0)
+
+template <class T>
+__global__ void KCopy(const T* __restrict__ a, T* __restrict__ b, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__constant__ int kEx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+__constant__ int kEy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+__constant__ int kEz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+
+// Interior cells of an nx*ny*nz block with a one-cell shell in y and z (so every source
+// index stays inside the plane); x sources wrap inside the row.
+template <class T>
+__global__ void KPull19(const T* __restrict__ a, T* __restrict__ b, int nx, int ny, int nz, long long P) {
+    const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lin >= static_cast<long long>(nx) * ny * nz) return;
+    const int x = static_cast<int>(lin % nx);
+    const int y = static_cast<int>((lin / nx) % ny);
+    const int z = static_cast<int>(lin / (static_cast<long long>(nx) * ny));
+    const int sy = nx, sz = nx * (ny + 2);
+    const int self = (z + 1) * sz + (y + 1) * sy + x;
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        int xs = x - kEx[d];
+        xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+        const int src = (z + 1 - kEz[d]) * sz + (y + 1 - kEy[d]) * sy + xs;
+        acc += static_cast<float>(a[d * P + src]);
+        b[d * P + self] = static_cast<T>(acc);
+    }
+}
+
+template <class F>
+float TimeMs(F launch, int reps) {
+    hipEvent_t e0, e1;
+    CHECK(hipEventCreate(&e0));
+    CHECK(hipEventCreate(&e1));
+    launch();
+    CHECK(hipDeviceSynchronize());
+    CHECK(hipEventRecord(e0));
+    for (int r = 0; r < reps; ++r) launch();
+    CHECK(hipEventRecord(e1));
+    CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    CHECK(hipEventElapsedTime(&ms, e0, e1));
+    CHECK(hipEventDestroy(e0));
+    CHECK(hipEventDestroy(e1));
+    return ms / reps;
+}
+
+int main(int argc, char** argv) {
+    const double gib = argc > 1 ? std::atof(argv[1]) : 2.0;
+    const int reps = argc > 2 ? std::atoi(argv[2]) : 20;
+    const size_t bytes = static_cast<size_t>(gib * (1ull << 30)) / 256 * 256;
+    void *a, *b;
+    CHECK(hipMalloc(&a, bytes));
+    CHECK(hipMalloc(&b, bytes));
+    CHECK(hipMemset(a, 0, bytes));
+    CHECK(hipMemset(b, 0, bytes));
+    auto report = [&](const char* name, double moved, float ms) {
+        std::printf("%-9s bytes/launch %.0f  avg %.4f ms  %.1f GB/s\n", name, moved, ms, moved / (ms * 1e-3) / 1e9);
+    };
+    {
+        const long long n = bytes / 16;
+        float ms = TimeMs([&] { hipLaunchKernelGGL(KCopy<float4>, dim3((n + 255) / 256), dim3(256), 0, 0,
+                                                   (const float4*)a, (float4*)b, n); }, reps);
+        report("copy16", 2.0 * bytes, ms);
+    }
+    {
+        const long long n = bytes / 4;
+        float ms = TimeMs([&] { hipLaunchKernelGGL(KCopy<float>, dim3((n + 255) / 256), dim3(256), 0, 0,
+                                                   (const float*)a, (float*)b, n); }, reps);
+        report("copy4", 2.0 * bytes, ms);
+    }
+    {
+        const long long n = bytes / 2;
+        float ms = TimeMs([&] { hipLaunchKernelGGL(KCopy<unsigned short>, dim3((n + 255) / 256), dim3(256), 0, 0,
+                                                   (const unsigned short*)a, (unsigned short*)b, n); }, reps);
+        report("copy2", 2.0 * bytes, ms);
+    }
+    // pull pattern on a 512 x 256 x nz block sized to the buffer
+    for (int s : {4, 2}) {
+        const int nx = 512, ny = 256;
+        const long long plane_pad = static_cast<long long>(nx) * (ny + 2);
+        const int nz = static_cast<int>(bytes / s / 19 / plane_pad) - 2;
+        const long long P = plane_pad * (nz + 2);
+        const long long cells = static_cast<long long>(nx) * ny * nz;
+        float ms;
+        if (s == 4)
+            ms = TimeMs([&] { hipLaunchKernelGGL(KPull19<float>, dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const float*)a, (float*)b, nx, ny, nz, P); }, reps);
+        else
+            ms = TimeMs([&] { hipLaunchKernelGGL(KPull19<_Float16>, dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const _Float16*)a, (_Float16*)b, nx, ny, nz, P); }, reps);
+        report(s == 4 ? "pull19_4" : "pull19_2", 2.0 * 19 * s * cells, ms);
+    }
+    CHECK(hipFree(a));
+    CHECK(hipFree(b));
+    return 0;
+}

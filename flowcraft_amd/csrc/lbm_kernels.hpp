@@ -32,6 +32,20 @@
 
 namespace fc {
 
+// Population stores stream through the cache hierarchy: a step writes every
+// population once and nothing reads it again within the step.
+#ifndef FC_LBM_NT_STORES
+#define FC_LBM_NT_STORES 0
+#endif
+template <class T>
+__device__ __forceinline__ void PutPop(T* p, T v) {
+#if FC_LBM_NT_STORES
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // -------------------------------------------------------------- constants
 template <class R>
 __host__ __device__ constexpr R K(float f, double d) {
@@ -474,7 +488,7 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     // ---------------------------------------------------------------- store
     if constexpr (!IP) {
 #pragma unroll
-        for (int d = 0; d < Q; ++d) dst[d * P + self] = S::Store(fo[d], d);
+        for (int d = 0; d < Q; ++d) PutPop(&dst[d * P + self], S::Store(fo[d], d));
     } else {
         const bool interior = x > 0 && x < a.nx - 1 && y > 0 && y < a.ny - 1 && z > 0 && z < a.nz - 1;
 #pragma unroll
@@ -487,7 +501,7 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
             }
             int pl, c;
             StoreSlot(d, par, self, anc, pl, c);
-            dst[pl * P + c] = S::Store(fo[d], d);
+            PutPop(&dst[pl * P + c], S::Store(fo[d], d));
         }
         if (rec != nullptr) {
             uint32_t kinds[3] = {rec->kind[0], rec->kind[1], rec->kind[2]};

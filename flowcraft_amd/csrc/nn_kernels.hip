@@ -1,0 +1,628 @@
+// libfc_nn.so — hand-written BF16 MFMA kernels for the warm-start surrogate
+// (a 3D convolutional encoder-decoder), forward and backward, plus Adam.
+//
+// Tensors are channels-last bf16: [voxel][channel] with voxel = ((n*D + z)*H + y)*W + x.
+// Convolution weights are [cout][27][cin] bf16 (k = tap*cin + ci contiguous per output
+// channel), taps ordered tz-major: tap = (kz*3 + ky)*3 + kx; padding 1.
+//
+// Convolution = implicit GEMM on v_mfma_f32_16x16x32_bf16 (gfx950):
+//   Out[v][n] = sum_k A[v][k] * B[k][n],  A = gathered input, B[k][n] = Wk[n][k].
+// Lane layout of the instruction (checked on the device): lane l holds A row l%16,
+// k = 8*(l/16)..+7; B column l%16, same k; result rows 4*(l/16)+r, column l%16.
+// Every 8-run of k stays inside one tap because channel counts are multiples of 8, so
+// a lane's A fragment is one 16-byte load (or zeros outside the volume).
+//
+// Gather modes:
+//   kFwd   : input voxel i = o*s + t - 1 per axis (stride s), zero outside
+//   kUpcat : stride 1; channels [0,c1) from a half-resolution tensor at i/2 (nearest
+//            upsample), channels [c1,cin) from a full-resolution tensor at i
+//   kAdj   : adjoint of a stride-s kFwd conv (data gradient): o = (i + 1 - t)/s when
+//            divisible and inside the forward output; weights pre-transposed
+//            to [cin][27][cout]
+// All reductions are in a fixed order (split-K partials summed in split order), so
+// results are deterministic.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstring>
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16;
+
+namespace {
+
+constexpr int kFwd = 0, kUpcat = 1, kAdj = 2;
+constexpr float kLeak = 0.05f;
+
+__device__ __forceinline__ float Bf2F(u16 h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+__device__ __forceinline__ u16 F2Bf(float f) {
+    uint32_t b = __float_as_uint(f);
+    if ((b & 0x7FFFFFFFu) > 0x7F800000u) return static_cast<u16>((b >> 16) | 0x40u);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return static_cast<u16>(b >> 16);
+}
+
+struct ConvArgs {
+    const u16* in;    // kFwd/kAdj: input; kUpcat: full-resolution part
+    const u16* in2;   // kUpcat: half-resolution part
+    const u16* w;     // [n_out][27][k_in]
+    const float* bias;  // [n_out] or null
+    u16* out;         // [V_out][n_out] bf16, or
+    float* outf;      // [V_out][n_out] fp32 (when non-null)
+    int nb;
+    int di, hi, wi;   // gather-source spatial dims (kUpcat: the full-resolution dims)
+    int dd, hd, wd;   // destination spatial dims
+    int kin;          // channels gathered per tap
+    int c1;           // kUpcat: channels taken from in2
+    int nout;         // output channels
+    int stride;
+    int act;          // 1 = leaky ReLU on the output
+};
+
+// Gathers the 8 values A[v][k0..k0+7] for destination voxel (n, z, y, x).
+template <int MODE>
+__device__ __forceinline__ bf16x8 GatherA(const ConvArgs& a, int n, int z, int y, int x, int k0, bool vok) {
+    bf16x8 r;
+    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+    const int K = 27 * a.kin;
+    if (vok && k0 < K) {
+        const int t = k0 / a.kin;
+        const int ci = k0 - t * a.kin;
+        const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+        int iz, iy, ix;
+        bool ok;
+        if (MODE == kAdj) {
+            const int s = a.stride;
+            const int pz = z + 1 - kz, py = y + 1 - ky, px = x + 1 - kx;
+            ok = pz >= 0 && py >= 0 && px >= 0 && (pz % s) == 0 && (py % s) == 0 && (px % s) == 0;
+            iz = pz / s;
+            iy = py / s;
+            ix = px / s;
+            ok = ok && iz < a.di && iy < a.hi && ix < a.wi;
+        } else {
+            const int s = MODE == kUpcat ? 1 : a.stride;
+            iz = z * s + kz - 1;
+            iy = y * s + ky - 1;
+            ix = x * s + kx - 1;
+            ok = iz >= 0 && iy >= 0 && ix >= 0 && iz < a.di && iy < a.hi && ix < a.wi;
+        }
+        if (ok) {
+            if (MODE == kUpcat && ci < a.c1) {
+                const int h2 = a.hi >> 1, w2 = a.wi >> 1, d2 = a.di >> 1;
+                const long long vi = ((static_cast<long long>(n) * d2 + (iz >> 1)) * h2 + (iy >> 1)) * w2 + (ix >> 1);
+                raw = *reinterpret_cast<const uint4*>(a.in2 + vi * a.c1 + ci);
+            } else {
+                const int cs = MODE == kUpcat ? a.kin - a.c1 : a.kin;
+                const int cc = MODE == kUpcat ? ci - a.c1 : ci;
+                const long long vi = ((static_cast<long long>(n) * a.di + iz) * a.hi + iy) * a.wi + ix;
+                raw = *reinterpret_cast<const uint4*>(a.in + vi * cs + cc);
+            }
+        }
+    }
+    __builtin_memcpy(&r, &raw, 16);
+    return r;
+}
+
+// One wave computes (16*TM) destination voxels x (16*TN) output channels; a block is
+// 4 waves along the voxel axis.
+template <int MODE, int TM, int TN>
+__global__ __launch_bounds__(256) void KConv(const ConvArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long long V = static_cast<long long>(a.nb) * a.dd * a.hd * a.wd;
+    const long long vbase = (static_cast<long long>(blockIdx.x) * 4 + wave) * (16 * TM);
+    const int nbase = blockIdx.y * (16 * TN);
+    const int K = 27 * a.kin;
+    const int nchunks = (K + 31) / 32;
+    const int row = lane & 15, kq = (lane >> 4) * 8;
+
+    int vn[TM], vz[TM], vy[TM], vx[TM];
+    bool vok[TM];
+#pragma unroll
+    for (int m = 0; m < TM; ++m) {
+        long long v = vbase + m * 16 + row;
+        vok[m] = v < V;
+        if (!vok[m]) v = 0;
+        vx[m] = static_cast<int>(v % a.wd);
+        long long r = v / a.wd;
+        vy[m] = static_cast<int>(r % a.hd);
+        r /= a.hd;
+        vz[m] = static_cast<int>(r % a.dd);
+        vn[m] = static_cast<int>(r / a.dd);
+    }
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int m = 0; m < TM; ++m)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int kc = 0; kc < nchunks; ++kc) {
+        const int k0 = kc * 32 + kq;
+        bf16x8 bfr[TN];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+            const int n = nbase + j * 16 + row;
+            if (k0 < K && n < a.nout) raw = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(n) * K + k0);
+            __builtin_memcpy(&bfr[j], &raw, 16);
+        }
+#pragma unroll
+        for (int m = 0; m < TM; ++m) {
+            const bf16x8 afr = GatherA<MODE>(a, vn[m], vz[m], vy[m], vx[m], k0, vok[m]);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+        }
+    }
+    // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+#pragma unroll
+    for (int m = 0; m < TM; ++m)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int n = nbase + j * 16 + (lane & 15);
+            if (n >= a.nout) continue;
+            const float b = a.bias ? a.bias[n] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long v = vbase + m * 16 + (lane >> 4) * 4 + r;
+                if (v >= V) continue;
+                float y = acc[m][j][r] + b;
+                if (a.act == 1) y = y > 0.f ? y : kLeak * y;
+                if (a.outf)
+                    a.outf[v * a.nout + n] = y;
+                else
+                    a.out[v * a.nout + n] = F2Bf(y);
+            }
+        }
+}
+
+// Weight gradient of a kFwd/kUpcat conv: dW[co][k] = sum_v dZ[v][co] * A[v][k], split over
+// voxel ranges (blockIdx.z) into fp32 partials part[split][co][K].  Block tile 64 co x 64 k,
+// 32 voxels per step, staged transposed in LDS so fragments are single 16-byte reads.
+struct WgradArgs {
+    ConvArgs g;       // gather description of the forward conv (MODE, dims, kin, c1, stride)
+    const u16* dz;    // [V][cout]
+    int cout;
+    long long vper;   // voxels per split
+    float* part;      // [splits][cout][27*kin]
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void KWgrad(const WgradArgs w) {
+    constexpr int VS = 32, LD = 40;  // voxels per step, padded LDS row (bf16 elements)
+    __shared__ __attribute__((aligned(16))) u16 sA[64 * LD];  // [co][v]
+    __shared__ __attribute__((aligned(16))) u16 sB[64 * LD];  // [k][v]
+    const ConvArgs& a = w.g;
+    const int K = 27 * a.kin;
+    const int co0 = blockIdx.y * 64, k0b = blockIdx.x * 64;
+    const long long V = static_cast<long long>(a.nb) * a.dd * a.hd * a.wd;
+    const long long vbeg = static_cast<long long>(blockIdx.z) * w.vper;
+    const long long vend = vbeg + w.vper < V ? vbeg + w.vper : V;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;  // wave tile 32 co x 32 k
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (long long v0 = vbeg; v0 < vend; v0 += VS) {
+        // stage dZ[v0..v0+32][co0..co0+64] transposed: 2048 values, 8 per thread
+        for (int e = threadIdx.x; e < VS * 64; e += 256) {
+            const int vv = e >> 6, cc = e & 63;
+            const long long v = v0 + vv;
+            u16 val = 0;
+            if (v < vend && co0 + cc < w.cout) val = w.dz[v * w.cout + co0 + cc];
+            sA[cc * LD + vv] = val;
+        }
+        // stage A[v][k0b..k0b+64] (gathered input) transposed: 8-runs per thread
+        for (int e = threadIdx.x; e < VS * 8; e += 256) {
+            const int vv = e >> 3, kk = (e & 7) * 8;
+            const long long v = v0 + vv;
+            bf16x8 g8;
+            const bool vok = v < vend;
+            long long vc = vok ? v : 0;
+            const int x = static_cast<int>(vc % a.wd);
+            long long r = vc / a.wd;
+            const int y = static_cast<int>(r % a.hd);
+            r /= a.hd;
+            const int z = static_cast<int>(r % a.dd);
+            const int n = static_cast<int>(r / a.dd);
+            g8 = GatherA<MODE>(a, n, z, y, x, k0b + kk, vok);
+            u16 tmp[8];
+            __builtin_memcpy(tmp, &g8, 16);
+            for (int q = 0; q < 8; ++q) sB[(kk + q) * LD + vv] = tmp[q];
+        }
+        __syncthreads();
+        const int row = lane & 15, kq = (lane >> 4) * 8;
+        bf16x8 af[2], bf[2];
+        for (int i = 0; i < 2; ++i) __builtin_memcpy(&af[i], &sA[(wm + i * 16 + row) * LD + kq], 16);
+        for (int j = 0; j < 2; ++j) __builtin_memcpy(&bf[j], &sB[(wn + j * 16 + row) * LD + kq], 16);
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    float* part = w.part + static_cast<long long>(blockIdx.z) * w.cout * K;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            const int k = k0b + wn + j * 16 + (lane & 15);
+            for (int r = 0; r < 4; ++r) {
+                const int co = co0 + wm + i * 16 + (lane >> 4) * 4 + r;
+                if (co < w.cout && k < K) part[static_cast<long long>(co) * K + k] = acc[i][j][r];
+            }
+        }
+}
+
+// out[i] = sum_s part[s][i] (fixed order), optionally += into out.
+__global__ void KReduceSplits(const float* __restrict__ part, int splits, long long n, float* __restrict__ out,
+                              int accumulate) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = accumulate ? out[i] : 0.f;
+    for (int k = 0; k < splits; ++k) s += part[static_cast<long long>(k) * n + i];
+    out[i] = s;
+}
+
+// Per-channel sums over voxels of a [V][C] bf16 tensor: part[block][c] (fixed grid).
+constexpr int kColBlocks = 256;
+__global__ __launch_bounds__(256) void KColSumPartial(const u16* __restrict__ x, long long V, int C,
+                                                      float* __restrict__ part) {
+    // thread handles channel c = tid % C over voxels strided by (256 / C) per block
+    const int lanes_per_c = 256 / C;
+    const int c = threadIdx.x % C, sub = threadIdx.x / C;
+    float s = 0.f;
+    if (sub < lanes_per_c)
+        for (long long v = static_cast<long long>(blockIdx.x) * lanes_per_c + sub; v < V;
+             v += static_cast<long long>(kColBlocks) * lanes_per_c)
+            s += Bf2F(x[v * C + c]);
+    __shared__ float red[256];
+    red[threadIdx.x] = sub < lanes_per_c ? s : 0.f;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.f;
+        for (int k = 0; k < lanes_per_c; ++k) t += red[k * C + threadIdx.x];
+        part[blockIdx.x * C + threadIdx.x] = t;
+    }
+}
+
+// dZ = dY * leaky'(Y) (Y is the activated output; its sign is the pre-activation's).
+__global__ void KLeakyBackward(const u16* __restrict__ y, const u16* __restrict__ dy, u16* __restrict__ dz,
+                               long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = Bf2F(dy[i]);
+    dz[i] = F2Bf(Bf2F(y[i]) > 0.f ? g : kLeak * g);
+}
+
+// Wt[ci][t][co] = W[co][t][ci]
+__global__ void KTransposeW(const u16* __restrict__ w, u16* __restrict__ wt, int cout, int cin) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long n = static_cast<long long>(cout) * 27 * cin;
+    if (i >= n) return;
+    const int ci = static_cast<int>(i % cin);
+    const int t = static_cast<int>((i / cin) % 27);
+    const int co = static_cast<int>(i / (27LL * cin));
+    wt[(static_cast<long long>(ci) * 27 + t) * cout + co] = w[i];
+}
+
+// Split the data gradient of a kUpcat conv input ([V][c1+c2] at full resolution):
+// channels >= c1 are added into dskip ([V][c2]); channels < c1 are summed over the 8
+// children of each coarse voxel into dcoarse ([V/8][c1]).
+__global__ void KUpcatSplit(const u16* __restrict__ dcat, int c1, int c2, int nb, int d, int h, int w,
+                            u16* __restrict__ dskip, u16* __restrict__ dcoarse) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int C = c1 + c2;
+    const int d2 = d >> 1, h2 = h >> 1, w2 = w >> 1;
+    const long long Vc = static_cast<long long>(nb) * d2 * h2 * w2;
+    const long long V = static_cast<long long>(nb) * d * h * w;
+    if (i < V * c2) {
+        const long long v = i / c2;
+        const int c = static_cast<int>(i % c2);
+        dskip[i] = F2Bf(Bf2F(dskip[i]) + Bf2F(dcat[v * C + c1 + c]));
+    }
+    if (i < Vc * c1) {
+        const long long vc = i / c1;
+        const int c = static_cast<int>(i % c1);
+        const int x2 = static_cast<int>(vc % w2);
+        long long r = vc / w2;
+        const int y2 = static_cast<int>(r % h2);
+        r /= h2;
+        const int z2 = static_cast<int>(r % d2);
+        const int n = static_cast<int>(r / d2);
+        float s = 0.f;
+        for (int q = 0; q < 8; ++q) {
+            const int z = 2 * z2 + (q >> 2), y = 2 * y2 + ((q >> 1) & 1), x = 2 * x2 + (q & 1);
+            const long long v = ((static_cast<long long>(n) * d + z) * h + y) * w + x;
+            s += Bf2F(dcat[v * C + c]);
+        }
+        dcoarse[i] = F2Bf(s);
+    }
+}
+
+// Output head (1x1 conv, cin -> 4, fp32) fused with the masked MSE loss:
+//   pred = Wh h + bh;  loss = sum_v m_v * sum_c (pred_c - tgt_c)^2 / norm
+// writes dh (bf16, d loss / d h), per-block partials of dWh [4][cin], dbh [4] and loss.
+constexpr int kHeadBlocks = 512;
+template <int cin>
+__global__ __launch_bounds__(256) void KHeadLoss(const u16* __restrict__ h, const float* __restrict__ wh,
+                                                 const float* __restrict__ bh, const float* __restrict__ tgt,
+                                                 const float* __restrict__ wmask, long long V, float inv_norm,
+                                                 u16* __restrict__ dh, float* __restrict__ part, float* pred_out) {
+    // part[block][4*cin + 4 + 1]
+    constexpr int P = 4 * cin + 5;
+    float gw[4 * cin];
+    float gb[4] = {0, 0, 0, 0};
+    float ls = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4 * cin; ++k) gw[k] = 0.f;
+    for (long long v = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; v < V;
+         v += static_cast<long long>(kHeadBlocks) * 256) {
+        float hv[cin];
+#pragma unroll
+        for (int c = 0; c < cin; ++c) hv[c] = Bf2F(h[v * cin + c]);
+        float g[4];
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            float p = bh[o];
+#pragma unroll
+            for (int c = 0; c < cin; ++c) p += wh[o * cin + c] * hv[c];
+            if (pred_out) pred_out[v * 4 + o] = p;
+            const float e = p - (tgt ? tgt[v * 4 + o] : 0.f);
+            const float m = wmask ? wmask[v] : 1.f;
+            ls += m * e * e * inv_norm;
+            g[o] = 2.f * m * e * inv_norm;
+            gb[o] += g[o];
+#pragma unroll
+            for (int c = 0; c < cin; ++c) gw[o * cin + c] += g[o] * hv[c];
+        }
+        if (dh)
+#pragma unroll
+            for (int c = 0; c < cin; ++c) {
+                float s = 0.f;
+#pragma unroll
+                for (int o = 0; o < 4; ++o) s += g[o] * wh[o * cin + c];
+                dh[v * cin + c] = F2Bf(s);
+            }
+    }
+    __shared__ float red[256];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        float val = k < 4 * cin ? gw[k] : (k < 4 * cin + 4 ? gb[k - 4 * cin] : ls);
+        red[threadIdx.x] = val;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[static_cast<long long>(blockIdx.x) * P + k] = red[0];
+        __syncthreads();
+    }
+}
+
+// Adam (Kingma & Ba 2015) on fp32 master weights, refreshing the bf16 copy.
+__global__ void KAdam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                      u16* __restrict__ pbf, long long n, float lr, float b1, float b2, float eps, float bc1,
+                      float bc2, float gscale) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] * gscale;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float upd = lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    const float pi = p[i] - upd;
+    p[i] = pi;
+    if (pbf) pbf[i] = F2Bf(pi);
+}
+
+__global__ void KF32ToBf16(const float* __restrict__ x, u16* __restrict__ y, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = F2Bf(x[i]);
+}
+
+__global__ void KBf16ToF32(const u16* __restrict__ x, float* __restrict__ y, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = Bf2F(x[i]);
+}
+
+// Surrogate input features from a voxel grid: [V][8] bf16 =
+//   (solid, clip(sdf,+-16)/16, u_in/0.1, (x+.5)/W, (y+.5)/H, (z+.5)/D, 0, 0)
+__global__ void KFeatures(const uint8_t* __restrict__ solid, const float* __restrict__ sdf, float u_in, int nb, int d,
+                          int h, int w, u16* __restrict__ out) {
+    const long long v = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long V = static_cast<long long>(nb) * d * h * w;
+    if (v >= V) return;
+    const int x = static_cast<int>(v % w);
+    const int y = static_cast<int>((v / w) % h);
+    const int z = static_cast<int>((v / (static_cast<long long>(w) * h)) % d);
+    float s = sdf[v];
+    s = s != s ? 0.f : (s > 16.f ? 16.f : (s < -16.f ? -16.f : s));
+    const float f[8] = {solid[v] ? 1.f : 0.f, s / 16.f, u_in / 0.1f, (x + 0.5f) / w, (y + 0.5f) / h,
+                        (z + 0.5f) / d, 0.f, 0.f};
+    for (int c = 0; c < 8; ++c) out[v * 8 + c] = F2Bf(f[c]);
+}
+
+// Training targets from an LBM field (4 floats per cell): ((rho-1)*3/u^2, u/u_in);
+// solid cells get 0 and weight 0.
+__global__ void KTargets(const float* __restrict__ field, const uint8_t* __restrict__ solid, float u_in, long long V,
+                         float* __restrict__ tgt, float* __restrict__ wmask) {
+    const long long v = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const bool s = solid[v] != 0;
+    const float inv = 1.f / u_in;
+    tgt[v * 4 + 0] = s ? 0.f : (field[v * 4 + 0] - 1.f) * 3.f * inv * inv;
+    tgt[v * 4 + 1] = s ? 0.f : field[v * 4 + 1] * inv;
+    tgt[v * 4 + 2] = s ? 0.f : field[v * 4 + 2] * inv;
+    tgt[v * 4 + 3] = s ? 0.f : field[v * 4 + 3] * inv;
+    wmask[v] = s ? 0.f : 1.f;
+}
+
+// Inverse of KTargets: a surrogate prediction -> LBM seed field (rho, ux, uy, uz).
+__global__ void KPredToField(const float* __restrict__ pred, float u_in, long long V, float* __restrict__ field) {
+    const long long v = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    field[v * 4 + 0] = 1.f + pred[v * 4 + 0] * u_in * u_in / 3.f;
+    field[v * 4 + 1] = pred[v * 4 + 1] * u_in;
+    field[v * 4 + 2] = pred[v * 4 + 2] * u_in;
+    field[v * 4 + 3] = pred[v * 4 + 3] * u_in;
+}
+
+unsigned Blk(long long n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
+
+template <int MODE>
+void LaunchConv(const ConvArgs& a, hipStream_t s) {
+    const long long V = static_cast<long long>(a.nb) * a.dd * a.hd * a.wd;
+    if (a.nout <= 16) {
+        hipLaunchKernelGGL((KConv<MODE, 4, 1>), dim3(Blk(V, 256), 1), dim3(256), 0, s, a);
+    } else if (a.nout <= 32) {
+        hipLaunchKernelGGL((KConv<MODE, 2, 2>), dim3(Blk(V, 128), 1), dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((KConv<MODE, 2, 4>), dim3(Blk(V, 128), Blk(a.nout, 64)), dim3(256), 0, s, a);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Convolution (mode 0 forward, 1 upsample-concat forward, 2 adjoint/data-gradient).
+// Shapes: see ConvArgs.  Returns 0 or a HIP error code.
+int fc_nn_conv(int mode, const void* in, const void* in2, const void* w, const float* bias, void* out, float* outf,
+               int nb, int di, int hi, int wi, int dd, int hd, int wd, int kin, int c1, int nout, int stride, int act,
+               void* stream) {
+    if (kin % 8 != 0 || (mode == kUpcat && (c1 % 8 != 0 || (kin - c1) % 8 != 0))) return -1;
+    ConvArgs a{static_cast<const u16*>(in), static_cast<const u16*>(in2), static_cast<const u16*>(w), bias,
+               static_cast<u16*>(out), outf, nb, di, hi, wi, dd, hd, wd, kin, c1, nout, stride, act};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (mode == kFwd)
+        LaunchConv<kFwd>(a, s);
+    else if (mode == kUpcat)
+        LaunchConv<kUpcat>(a, s);
+    else
+        LaunchConv<kAdj>(a, s);
+    return static_cast<int>(hipGetLastError());
+}
+
+// Weight gradient (fp32, [cout][27*kin]) of a mode-0/1 conv into dw (overwrite, or add
+// when accumulate); ws must hold splits*cout*27*kin floats.
+int fc_nn_wgrad(int mode, const void* in, const void* in2, const void* dz, float* dw, float* ws, int splits,
+                int accumulate, int nb, int di, int hi, int wi, int dd, int hd, int wd, int kin, int c1, int cout,
+                int stride, void* stream) {
+    if (kin % 8 != 0) return -1;
+    WgradArgs w{};
+    w.g = ConvArgs{static_cast<const u16*>(in), static_cast<const u16*>(in2), nullptr, nullptr, nullptr, nullptr,
+                   nb, di, hi, wi, dd, hd, wd, kin, c1, cout, stride, 0};
+    w.dz = static_cast<const u16*>(dz);
+    w.cout = cout;
+    const long long V = static_cast<long long>(nb) * dd * hd * wd;
+    long long vper = (V + splits - 1) / splits;
+    vper = (vper + 31) / 32 * 32;
+    w.vper = vper;
+    w.part = ws;
+    const int K = 27 * kin;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dim3 grid(Blk(K, 64), Blk(cout, 64), splits);
+    if (mode == kUpcat)
+        hipLaunchKernelGGL(KWgrad<kUpcat>, grid, dim3(256), 0, s, w);
+    else
+        hipLaunchKernelGGL(KWgrad<kFwd>, grid, dim3(256), 0, s, w);
+    const long long n = static_cast<long long>(cout) * K;
+    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(n, 256)), dim3(256), 0, s, ws, splits, n, dw, accumulate);
+    return static_cast<int>(hipGetLastError());
+}
+
+// Per-channel sums (bias gradient) of a [V][C] bf16 tensor into out[C] (fp32); ws holds
+// 256*C floats.  C must divide 256.
+int fc_nn_colsum(const void* x, long long V, int C, float* out, float* ws, int accumulate, void* stream) {
+    if (C <= 0 || 256 % C != 0) return -1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(KColSumPartial, dim3(kColBlocks), dim3(256), 0, s, static_cast<const u16*>(x), V, C, ws);
+    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(C, 256)), dim3(256), 0, s, ws, kColBlocks, static_cast<long long>(C),
+                       out, accumulate);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_leaky_backward(const void* y, const void* dy, void* dz, long long n, void* stream) {
+    hipLaunchKernelGGL(KLeakyBackward, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const u16*>(y), static_cast<const u16*>(dy), static_cast<u16*>(dz), n);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_transpose_w(const void* w, void* wt, int cout, int cin, void* stream) {
+    const long long n = static_cast<long long>(cout) * 27 * cin;
+    hipLaunchKernelGGL(KTransposeW, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const u16*>(w), static_cast<u16*>(wt), cout, cin);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_upcat_split(const void* dcat, int c1, int c2, int nb, int d, int h, int w, void* dskip, void* dcoarse,
+                      void* stream) {
+    const long long V = static_cast<long long>(nb) * d * h * w;
+    const long long n = V * c2 > V / 8 * c1 ? V * c2 : V / 8 * c1;
+    hipLaunchKernelGGL(KUpcatSplit, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const u16*>(dcat), c1, c2, nb, d, h, w, static_cast<u16*>(dskip),
+                       static_cast<u16*>(dcoarse));
+    return static_cast<int>(hipGetLastError());
+}
+
+// Head + loss.  Results: dwh[4*cin], dbh[4], loss (fp32, device) via ws (kHeadBlocks*(4cin+5)).
+int fc_nn_head_loss(const void* h, int cin, const float* wh, const float* bh, const float* tgt, const float* wmask,
+                    long long V, float inv_norm, void* dh, float* ws, float* out_grads_and_loss, float* pred_out,
+                    void* stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (cin == 16)
+        hipLaunchKernelGGL(KHeadLoss<16>, dim3(kHeadBlocks), dim3(256), 0, s, static_cast<const u16*>(h), wh, bh, tgt,
+                           wmask, V, inv_norm, static_cast<u16*>(dh), ws, pred_out);
+    else if (cin == 8)
+        hipLaunchKernelGGL(KHeadLoss<8>, dim3(kHeadBlocks), dim3(256), 0, s, static_cast<const u16*>(h), wh, bh, tgt,
+                           wmask, V, inv_norm, static_cast<u16*>(dh), ws, pred_out);
+    else
+        return -1;
+    const long long P = 4LL * cin + 5;
+    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(P, 256)), dim3(256), 0, s, ws, kHeadBlocks, P, out_grads_and_loss, 0);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_adam(float* p, const float* g, float* m, float* v, void* pbf, long long n, float lr, float b1, float b2,
+               float eps, int step, float gscale, void* stream) {
+    const float bc1 = 1.f - powf(b1, static_cast<float>(step));
+    const float bc2 = 1.f - powf(b2, static_cast<float>(step));
+    hipLaunchKernelGGL(KAdam, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), p, g, m, v,
+                       static_cast<u16*>(pbf), n, lr, b1, b2, eps, bc1, bc2, gscale);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_f32_to_bf16(const float* x, void* y, long long n, void* stream) {
+    hipLaunchKernelGGL(KF32ToBf16, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), x,
+                       static_cast<u16*>(y), n);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_bf16_to_f32(const void* x, float* y, long long n, void* stream) {
+    hipLaunchKernelGGL(KBf16ToF32, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const u16*>(x), y, n);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_features(const uint8_t* solid, const float* sdf, float u_in, int nb, int d, int h, int w, void* out,
+                   void* stream) {
+    const long long V = static_cast<long long>(nb) * d * h * w;
+    hipLaunchKernelGGL(KFeatures, dim3(Blk(V, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), solid, sdf, u_in,
+                       nb, d, h, w, static_cast<u16*>(out));
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_targets(const float* field, const uint8_t* solid, float u_in, long long V, float* tgt, float* wmask,
+                  void* stream) {
+    hipLaunchKernelGGL(KTargets, dim3(Blk(V, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), field, solid, u_in,
+                       V, tgt, wmask);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_pred_to_field(const float* pred, float u_in, long long V, float* field, void* stream) {
+    hipLaunchKernelGGL(KPredToField, dim3(Blk(V, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), pred, u_in, V,
+                       field);
+    return static_cast<int>(hipGetLastError());
+}
+
+}  // extern "C"

@@ -1,0 +1,228 @@
+// Drop-in replacement for FlowCraft's LbmSolver class (FlowCraft/LbmSolver.h:59-302),
+// header-only over the C ABI of libfc_lbm.so (fc_lbm.h).
+//
+// Same public methods, argument meaning and error behaviour as the reference:
+// Init() returns false and sets LastError(); every other method is a silent no-op
+// on an invalid solver; ResetFromField() ignores a field of the wrong size; Step()/
+// StepRange() only enqueue work (on the solver's HIP stream — the reference records
+// into the caller's command buffer), readbacks block.  The SDL handle parameters
+// are accepted and ignored, so reference call sites compile unchanged.
+//
+// LatticeModel and VoxelGrid come from the including application's own headers
+// (FlowCraft/LatticeModel.h, FlowCraft/VoxelGrid.h); include/flowcraft/ carries
+// minimal definitions with the same names and members for standalone builds.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "LatticeModel.h"
+#include "VoxelGrid.h"
+#include "fc_lbm.h"
+
+struct SDL_GPUDevice;
+struct SDL_GPUCommandBuffer;
+
+class LbmSolver {
+public:
+    // LbmSolver::Params (FlowCraft/LbmSolver.h:61-100), same fields and defaults.
+    struct Params {
+        float tau = 0.8f;
+        float inletVelocity = 0.05f;
+        bool reverseFlow = false;
+        bool freeSlipY = true;
+        bool freeSlipZ = true;
+        float smagorinskyCs = 0.12f;
+    };
+
+    // LbmSolver::Stats (FlowCraft/LbmSolver.h:172-184).  centerOfPressure/liftBaseZ
+    // are FieldOverlay's, not the solver's: left zero here as in the reference's
+    // ReadbackStats (FlowCraft/LbmSolver.cpp:739-788).
+    struct Stats {
+        float avgDensity = 0.0f, minDensity = 0.0f, maxDensity = 0.0f;
+        float avgSpeed = 0.0f, maxSpeed = 0.0f;
+        float forceX = 0.0f, forceY = 0.0f, forceZ = 0.0f;
+        float maxStress = 0.0f;
+        Vec3 centerOfPressure{0.0f, 0.0f, 0.0f};
+        float liftBaseZ = 0.0f;
+        bool hasNaN = false;
+        bool valid = false;
+        double totalMass = 0.0;  // extension (LbmReference::Stats::totalMass)
+    };
+
+    // Engine options with no reference counterpart (population storage and layout,
+    // periodic y/z); Init() without them gives the FP32 in-place engine.
+    struct EngineOptions {
+        int storage = FC_LBM_STORE_F32;
+        int layout = FC_LBM_LAYOUT_INPLACE;
+        bool periodicY = false;
+        bool periodicZ = false;
+        void* stream = nullptr;  // hipStream_t; nullptr = the engine's own
+    };
+
+    LbmSolver() = default;
+    LbmSolver(const LbmSolver&) = delete;
+    LbmSolver& operator=(const LbmSolver&) = delete;
+    ~LbmSolver() { Shutdown(); }
+
+    bool Init(SDL_GPUDevice* /*device*/, const VoxelGrid& grid, LatticeModel model) {
+        return Init(nullptr, grid, model, EngineOptions{});
+    }
+
+    bool Init(SDL_GPUDevice* /*device*/, const VoxelGrid& grid, LatticeModel model, const EngineOptions& opt) {
+        Shutdown();
+        if (grid.nx <= 0 || grid.ny <= 0 || grid.nz <= 0 || grid.solid.size() != grid.CellCount()) {
+            lastError_ = "empty or inconsistent voxel grid";
+            return false;
+        }
+        fc_lbm_desc d{};
+        d.nx = grid.nx;
+        d.ny = grid.ny;
+        d.nz = grid.nz;
+        d.model = model == LatticeModel::D3Q15 ? FC_LBM_D3Q15 : FC_LBM_D3Q19;
+        d.storage = opt.storage;
+        d.layout = opt.layout;
+        d.periodic_y = opt.periodicY ? 1 : 0;
+        d.periodic_z = opt.periodicZ ? 1 : 0;
+        d.stream = opt.stream;
+        fc_lbm* h = nullptr;
+        if (fc_lbm_create(&d, &h) != FC_LBM_OK) {
+            lastError_ = fc_lbm_last_error(h);
+            if (h) fc_lbm_destroy(h);
+            return false;
+        }
+        const float* sdf = grid.signedDistance.size() == grid.CellCount() ? grid.signedDistance.data() : nullptr;
+        if (fc_lbm_set_geometry(h, grid.solid.data(), sdf) != FC_LBM_OK) {
+            lastError_ = fc_lbm_last_error(h);
+            fc_lbm_destroy(h);
+            return false;
+        }
+        h_ = h;
+        nx_ = grid.nx;
+        ny_ = grid.ny;
+        nz_ = grid.nz;
+        model_ = model;
+        solidCpu_ = grid.solid;
+        lastError_.clear();
+        return true;
+    }
+
+    void Shutdown() {
+        if (h_) fc_lbm_destroy(h_);
+        h_ = nullptr;
+    }
+
+    const std::string& LastError() const { return lastError_; }
+
+    void Reset(const Params& params, bool seedAtMidSpeed = false) {
+        params_ = params;
+        if (!h_) return;
+        PushParams();
+        Check(fc_lbm_reset_uniform(h_, seedAtMidSpeed ? 1 : 0));
+    }
+
+    void ResetFromField(const std::vector<float>& field) {
+        if (!h_ || field.size() != CellCount() * 4) return;
+        Check(fc_lbm_reset_from_field(h_, field.data()));
+    }
+
+    void SetParams(const Params& params) {
+        params_ = params;
+        if (h_) PushParams();
+    }
+
+    void Step(SDL_GPUCommandBuffer* /*cmd*/, int numSubsteps) {
+        if (h_) Check(fc_lbm_step(h_, numSubsteps));
+    }
+
+    void StepRange(SDL_GPUCommandBuffer* /*cmd*/, int zStart, int zCount) {
+        if (h_) Check(fc_lbm_step_range(h_, zStart, zCount));
+    }
+
+    void AdvanceSubstep() {
+        if (h_) Check(fc_lbm_advance(h_));
+    }
+
+    Stats ReadbackStats() {
+        Stats s;
+        if (!h_) return s;
+        fc_lbm_stats r{};
+        if (!Check(fc_lbm_readback_stats(h_, &r))) return s;
+        s.avgDensity = static_cast<float>(r.avg_density);
+        s.minDensity = static_cast<float>(r.min_density);
+        s.maxDensity = static_cast<float>(r.max_density);
+        s.avgSpeed = static_cast<float>(r.avg_speed);
+        s.maxSpeed = static_cast<float>(r.max_speed);
+        s.forceX = static_cast<float>(r.force_x);
+        s.forceY = static_cast<float>(r.force_y);
+        s.forceZ = static_cast<float>(r.force_z);
+        s.maxStress = static_cast<float>(r.max_stress);
+        s.hasNaN = r.has_nan != 0;
+        s.totalMass = r.total_mass;
+        s.valid = true;
+        return s;
+    }
+
+    std::vector<float> ReadbackField() {
+        std::vector<float> field;
+        if (!h_) return field;
+        field.resize(CellCount() * 4);
+        Check(fc_lbm_readback_field(h_, field.data()));
+        return field;
+    }
+
+    void ReadbackFieldRange(int zStart, int zCount, std::vector<float>& outField) {
+        if (!h_ || outField.size() != CellCount() * 4) return;
+        Check(fc_lbm_readback_field_range(h_, zStart, zCount, outField.data()));
+    }
+
+    bool ProbeSliceHealth(int zLayer) {
+        if (!h_ || nz_ <= 0) return false;
+        int diverged = 0;
+        if (!Check(fc_lbm_probe_health(h_, zLayer, &diverged))) return false;
+        return diverged != 0;
+    }
+
+    const std::vector<uint8_t>& SolidMask() const { return solidCpu_; }
+    bool IsValid() const { return h_ != nullptr; }
+    LatticeModel Model() const { return model_; }
+    int Nx() const { return nx_; }
+    int Ny() const { return ny_; }
+    int Nz() const { return nz_; }
+
+    // Extensions.
+    fc_lbm* Handle() const { return h_; }
+    void Sync() {
+        if (h_) Check(fc_lbm_sync(h_));
+    }
+
+private:
+    size_t CellCount() const { return static_cast<size_t>(nx_) * ny_ * nz_; }
+
+    void PushParams() {
+        fc_lbm_params p{};
+        p.tau = params_.tau;
+        p.inlet_velocity = params_.inletVelocity;
+        p.reverse_flow = params_.reverseFlow ? 1 : 0;
+        p.free_slip_y = params_.freeSlipY ? 1 : 0;
+        p.free_slip_z = params_.freeSlipZ ? 1 : 0;
+        p.smagorinsky_cs = params_.smagorinskyCs;
+        Check(fc_lbm_set_params(h_, &p));
+    }
+
+    // Device errors have no reference counterpart (SDL_GPU calls there return
+    // nothing); they are kept in LastError() for the caller to inspect.
+    bool Check(int rc) {
+        if (rc == FC_LBM_OK) return true;
+        lastError_ = fc_lbm_last_error(h_);
+        return false;
+    }
+
+    fc_lbm* h_ = nullptr;
+    int nx_ = 0, ny_ = 0, nz_ = 0;
+    LatticeModel model_ = LatticeModel::D3Q19;
+    Params params_;
+    std::vector<uint8_t> solidCpu_;
+    std::string lastError_;
+};

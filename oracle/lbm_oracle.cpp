@@ -403,8 +403,10 @@ struct Oracle {
                             } else if (inb) {
                                 fpre[i] = Bouzidi(fin, x, y, z, c, i, rx, ry, rz);
                             } else {
-                                bool outY = sy < 0 || sy >= ny;
-                                bool outZ = sz < 0 || sz >= nz;
+                                // Only a non-periodic axis is a wall (with the periodic extension
+                                // a crossing of a periodic axis has already been wrapped).
+                                bool outY = !periodicY && (sy < 0 || sy >= ny);
+                                bool outZ = !periodicZ && (sz < 0 || sz >= nz);
                                 bool ok = (!outY || freeSlipY) && (!outZ || freeSlipZ);
                                 if ((outY || outZ) && ok) {
                                     int d = i;
