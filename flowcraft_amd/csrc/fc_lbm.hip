@@ -453,9 +453,10 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     a.tau_d = h->params.tau;
     a.u_in_d = h->params.inlet_velocity;
     a.cs_d = h->params.smagorinsky_cs;
-    const long long cells = static_cast<long long>(h->nx) * h->ny * zcount;
+    const long long tiles = static_cast<long long>((h->nx + kTileX - 1) / kTileX) * ((h->ny + kTileY - 1) / kTileY) *
+                            zcount;
     Dispatch(h, [&]<int Q, class S, class Lay>() {
-        hipLaunchKernelGGL((KStep<Q, S, Lay>), dim3(Blocks(cells, 256)), dim3(256), 0, h->stream, a);
+        hipLaunchKernelGGL((KStep<Q, S, Lay>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, h->stream, a);
     });
     FC_HIP(h, hipGetLastError());
     return FC_LBM_OK;

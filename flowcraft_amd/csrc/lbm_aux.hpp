@@ -210,9 +210,11 @@ __global__ void KSeed(void* pops0, void* pops1, long long P, long long t, int nx
         uz = static_cast<R>(field4[c * 4 + 3]);
     }
     const int self = g.Idx(x, y, z);
+    R feq[Q];
+    FeqAll<Q, R>(rho, ux, uy, uz, feq);
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-        const R fe = Feq<Q, R>(d, rho, ux, uy, uz);
+        const R fe = feq[d];
         if constexpr (!Lay::kInPlace) {
             T* f = static_cast<T*>((t & 1) ? pops1 : pops0);
             f[d * P + self] = S::Store(fe, d);

@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lattice.hpp"
 
 namespace fc {
@@ -35,7 +37,7 @@ namespace fc {
 // Population stores stream through the cache hierarchy: a step writes every
 // population once and nothing reads it again within the step.
 #ifndef FC_LBM_NT_STORES
-#define FC_LBM_NT_STORES 0
+#define FC_LBM_NT_STORES 1
 #endif
 template <class T>
 __device__ __forceinline__ void PutPop(T* p, T v) {
@@ -199,7 +201,7 @@ __device__ __forceinline__ void StoreSlot(int d, int par, int self, int anc, int
 }
 
 template <int Q>
-__device__ __forceinline__ int PairAnchor(int d) {
+__host__ __device__ constexpr int PairAnchor(int d) {
     return d == 0 ? 0 : ((d & 1) ? d : d - 1);
 }
 
@@ -222,28 +224,187 @@ __device__ __forceinline__ typename S::R CurrentPop(const void* pops0, const voi
     }
 }
 
-template <int Q, class R>
-__device__ __forceinline__ R Feq(int d, R rho, R ux, R uy, R uz) {
-    R eu = static_cast<R>(Lat<Q>::ex[d]) * ux + static_cast<R>(Lat<Q>::ey[d]) * uy + static_cast<R>(Lat<Q>::ez[d]) * uz;
-    R u2 = ux * ux + uy * uy + uz * uz;
+// ------------------------------------------------- compile-time lattice sums
+// The reference evaluates sums like ex*ux + ey*uy + ez*uz with ex in {-1,0,1}.  Terms
+// with a zero coefficient add an exact +-0, and multiplying by +-1 is exact, so the
+// sums below skip those terms and keep the remaining ones in the reference's order:
+// for finite operands the result is identical, except possibly the sign of an exact
+// zero.  This removes most of the collision's arithmetic without changing rounding.
+template <int B, int E, class F>
+__device__ __forceinline__ void StaticFor(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        StaticFor<B + 1, E>(f);
+    }
+}
+
+template <int c, class R>
+__device__ __forceinline__ void AccC(R& a, R v) {
+    if constexpr (c == 1)
+        a += v;
+    else if constexpr (c == -1)
+        a -= v;
+}
+
+template <int c, class R>
+__device__ __forceinline__ R MulC(R v) {
+    if constexpr (c == 1)
+        return v;
+    else
+        return -v;
+}
+
+// c1*u + c2*v + c3*w, left to right, zero terms skipped.
+template <int c1, int c2, int c3, class R>
+__device__ __forceinline__ R Dot3(R u, R v, R w) {
+    if constexpr (c1 != 0) {
+        R r = MulC<c1>(u);
+        if constexpr (c2 != 0) r = r + MulC<c2>(v);
+        if constexpr (c3 != 0) r = r + MulC<c3>(w);
+        return r;
+    } else if constexpr (c2 != 0) {
+        R r = MulC<c2>(v);
+        if constexpr (c3 != 0) r = r + MulC<c3>(w);
+        return r;
+    } else if constexpr (c3 != 0) {
+        return MulC<c3>(w);
+    } else {
+        return R(0);
+    }
+}
+
+// Equilibrium of direction d (LbmReference.cpp:9-13): w*rho*(((1+3eu)+(4.5eu)eu)-1.5u2),
+// u2 = ux*ux + uy*uy + uz*uz computed once by the caller.
+template <int Q, int d, class R>
+__device__ __forceinline__ R FeqD(R rho, R ux, R uy, R uz, R u2) {
+    const R eu = Dot3<Lat<Q>::ex[d], Lat<Q>::ey[d], Lat<Q>::ez[d]>(ux, uy, uz);
     return W<Q, R>(d) * rho * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * u2);
 }
 
+template <int Q, class R>
+__device__ __forceinline__ void FeqAll(R rho, R ux, R uy, R uz, R (&out)[Q]) {
+    const R u2 = ux * ux + uy * uy + uz * uz;
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        out[d] = FeqD<Q, d, R>(rho, ux, uy, uz, u2);
+    });
+}
+
+// Moments (rho, rho*u) of a population set in index order (LbmReference.cpp:43-61, before
+// the division by rho).
+template <int Q, class R>
+__device__ __forceinline__ void MomentSums(const R (&f)[Q], R& rho, R& jx, R& jy, R& jz) {
+    rho = R(0);
+    jx = R(0);
+    jy = R(0);
+    jz = R(0);
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        rho += f[d];
+        AccC<Lat<Q>::ex[d]>(jx, f[d]);
+        AccC<Lat<Q>::ey[d]>(jy, f[d]);
+        AccC<Lat<Q>::ez[d]>(jz, f[d]);
+    });
+}
+
+// Regularized collision with Smagorinsky eddy viscosity (LbmReference.cpp:385-453):
+// f holds the gathered populations on entry; fo receives the post-collision ones.
+// Returns true when the cell diverged (non-finite, rho <= 0 or |u|^2 > 4).
+template <int Q, class R>
+__device__ __forceinline__ bool Collide(const R (&f)[Q], R (&fo)[Q], R tau, R cs) {
+    R rho, ux, uy, uz;
+    MomentSums<Q, R>(f, rho, ux, uy, uz);
+    if (rho > K<R>(1e-6f, 1e-6)) {
+        ux /= rho;
+        uy /= rho;
+        uz /= rho;
+    }
+    const bool bad = !__builtin_isfinite(rho) || !__builtin_isfinite(ux) || !__builtin_isfinite(uy) ||
+                     !__builtin_isfinite(uz) || rho <= R(0) || (ux * ux + uy * uy + uz * uz) > R(4);
+    R feq[Q];
+    FeqAll<Q, R>(rho, ux, uy, uz, feq);
+    R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        constexpr int cx = Lat<Q>::ex[d], cy = Lat<Q>::ey[d], cz = Lat<Q>::ez[d];
+        const R fneq = f[d] - feq[d];
+        if constexpr (cx != 0) pxx += fneq;
+        if constexpr (cy != 0) pyy += fneq;
+        if constexpr (cz != 0) pzz += fneq;
+        AccC<cx * cy>(pxy, fneq);
+        AccC<cx * cz>(pxz, fneq);
+        AccC<cy * cz>(pyz, fneq);
+    });
+    const R piMag = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const R rmax = rho < K<R>(1e-6f, 1e-6) ? K<R>(1e-6f, 1e-6) : rho;
+    const R kSmag = R(18) * K<R>(1.41421356f, 1.41421356);
+    const R tauEff = R(0.5) * (tau + sqrt(tau * tau + kSmag * cs * cs * piMag / rmax));
+    const R omega = R(1) / tauEff;
+    const R keep = R(1) - omega;
+    constexpr R kCs2 = R(1) / R(3);
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        constexpr int cx = Lat<Q>::ex[d], cy = Lat<Q>::ey[d], cz = Lat<Q>::ez[d];
+        constexpr R qxx = R(cx * cx) - kCs2, qyy = R(cy * cy) - kCs2, qzz = R(cz * cz) - kCs2;
+        R sum = qxx * pxx;
+        sum = sum + qyy * pyy;
+        sum = sum + qzz * pzz;
+        if constexpr (cx * cy != 0) sum = sum + R(2 * cx * cy) * pxy;
+        if constexpr (cx * cz != 0) sum = sum + R(2 * cx * cz) * pxz;
+        if constexpr (cy * cz != 0) sum = sum + R(2 * cy * cz) * pyz;
+        const R reg = (W<Q, R>(d) * R(4.5)) * sum;
+        fo[d] = feq[d] + keep * reg;
+    });
+    return bad;
+}
+
 // ----------------------------------------------------------- step kernel
+// Work decomposition: one workgroup = a 64 (x) by 4 (y) tile of one z layer, 256
+// threads, no integer division per thread.  Workgroups are dealt to the 8 XCDs
+// round-robin by hardware id; the id is remapped so that each XCD processes one
+// contiguous range of tiles, keeping the cache lines shared by neighbouring tiles
+// (x-shifted pulls, y/z neighbour rows) inside one XCD's L2.
+constexpr int kTileX = 64, kTileY = 4;
+
+__device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
+    const unsigned q = n >> 3, r = n & 7u, xcd = hw & 7u, slot = hw >> 3;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+template <int Q, class S>
+__device__ __forceinline__ void WriteMoments(typename S::R* side_mom, const typename S::R (&fo)[Q], int par,
+                                             int col, int plane_yz, int zy) {
+    using R = typename S::R;
+    R v[Q];
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        v[d] = S::Load(S::Store(fo[d], d), d);
+    });
+    R m0, m1, m2, m3;
+    MomentSums<Q, R>(v, m0, m1, m2, m3);
+    R* m = side_mom + ((static_cast<long long>(par) * 2 + col) * plane_yz + zy) * 4;
+    m[0] = m0;
+    m[1] = m1;
+    m[2] = m2;
+    m[3] = m3;
+}
+
 template <int Q, class S, class Lay>
 __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     using T = typename S::T;
     using R = typename S::R;
     constexpr bool IP = Lay::kInPlace;
 
-    const unsigned plane = static_cast<unsigned>(a.nx) * static_cast<unsigned>(a.ny);
-    const unsigned lin = blockIdx.x * 256u + threadIdx.x;
-    if (lin >= static_cast<unsigned>(a.zc) * plane) return;
-    const unsigned zr = lin / plane;
-    const unsigned rem = lin - zr * plane;
-    const int y = static_cast<int>(rem / static_cast<unsigned>(a.nx));
-    const int x = static_cast<int>(rem) - y * a.nx;
-    const int z = a.z0 + static_cast<int>(zr);
+    const unsigned tile = XcdContiguous(blockIdx.x, gridDim.x);
+    const unsigned tiles_x = static_cast<unsigned>((a.nx + kTileX - 1) / kTileX);
+    const unsigned tiles_y = static_cast<unsigned>((a.ny + kTileY - 1) / kTileY);
+    const unsigned tx = tile % tiles_x;
+    const unsigned rest = tile / tiles_x;
+    const unsigned ty = rest % tiles_y;
+    const int x = static_cast<int>(tx * kTileX + (threadIdx.x & (kTileX - 1)));
+    const int y = static_cast<int>(ty * kTileY + (threadIdx.x / kTileX));
+    const int z = a.z0 + static_cast<int>(rest / tiles_y);
+    if (x >= a.nx || y >= a.ny) return;
 
     const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2)};
     const int self = g.Idx(x, y, z);
@@ -263,28 +424,17 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     if (fl & kSolid) {
         // Solid cell: populations w_i (LbmReference.cpp:264-267).  In place, solid
         // cells are inactive; only the outlet-column moments they feed are written.
-#pragma unroll
-        for (int d = 0; d < Q; ++d) fo[d] = W<Q, R>(d);
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            fo[d] = W<Q, R>(d);
+        });
         if constexpr (!IP) {
-#pragma unroll
-            for (int d = 0; d < Q; ++d) dst[d * P + self] = S::Store(fo[d], d);
+            StaticFor<0, Q>([&](auto I) {
+                constexpr int d = I;
+                PutPop(&dst[d * P + self], S::Store(fo[d], d));
+            });
         }
-        if (mom_writer) {
-            R m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-#pragma unroll
-            for (int d = 0; d < Q; ++d) {
-                R v = S::Load(S::Store(fo[d], d), d);
-                m0 += v;
-                m1 += v * static_cast<R>(Lat<Q>::ex[d]);
-                m2 += v * static_cast<R>(Lat<Q>::ey[d]);
-                m3 += v * static_cast<R>(Lat<Q>::ez[d]);
-            }
-            R* m = side_mom + ((static_cast<long long>(par) * 2 + mom_col) * plane_yz + z * a.ny + y) * 4;
-            m[0] = m0;
-            m[1] = m1;
-            m[2] = m2;
-            m[3] = m3;
-        }
+        if (mom_writer) WriteMoments<Q, S>(side_mom, fo, par, mom_col, plane_yz, z * a.ny + y);
         return;
     }
 
@@ -298,24 +448,20 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
 
     if (x == inletX) {
         // Equilibrium inlet (LbmReference.cpp:269-273).
-        const R ux = a.reverse ? -uin : uin;
-#pragma unroll
-        for (int d = 0; d < Q; ++d) fo[d] = Feq<Q, R>(d, R(1), ux, R(0), R(0));
+        FeqAll<Q, R>(R(1), a.reverse ? -uin : uin, R(0), R(0), fo);
     } else if (x == outletX) {
         // Zero-gradient outlet from the neighbour's previous post-collision moments,
         // density relaxed toward 1 (LbmReference.cpp:275-327).
         const int nbx = a.reverse ? x + 1 : x - 1;
-        R nr = 0, nu = 0, nv = 0, nw = 0;
+        R nr, nu, nv, nw;
         if constexpr (!IP) {
             const int nb = self + (nbx - x);
-#pragma unroll
-            for (int d = 0; d < Q; ++d) {
-                R v = S::Load(src[d * P + nb], d);
-                nr += v;
-                nu += v * static_cast<R>(Lat<Q>::ex[d]);
-                nv += v * static_cast<R>(Lat<Q>::ey[d]);
-                nw += v * static_cast<R>(Lat<Q>::ez[d]);
-            }
+            R v[Q];
+            StaticFor<0, Q>([&](auto I) {
+                constexpr int d = I;
+                v[d] = S::Load(src[d * P + nb], d);
+            });
+            MomentSums<Q, R>(v, nr, nu, nv, nw);
         } else {
             const int col = nbx == 1 ? 0 : 1;
             const R* m = side_mom + ((static_cast<long long>(par ^ 1) * 2 + col) * plane_yz + z * a.ny + y) * 4;
@@ -330,27 +476,26 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
             nw /= nr;
         }
         const R rho = nr > K<R>(1e-6f, 1e-6) ? R(1) + K<R>(0.999f, 0.999) * (nr - R(1)) : R(1);
-#pragma unroll
-        for (int d = 0; d < Q; ++d) fo[d] = Feq<Q, R>(d, rho, nu, nv, nw);
+        FeqAll<Q, R>(rho, nu, nv, nw, fo);
     } else {
         // ---------------------------------------------------------- gather
         R f[Q];
         const bool edge = (y == 0) | (y == a.ny - 1) | (z == 0) | (z == a.nz - 1);
         if (!(edge | near)) {
             // Bulk fast path: every source is an in-domain fluid cell, no wrap.
-#pragma unroll
-            for (int d = 0; d < Q; ++d) {
+            StaticFor<0, Q>([&](auto I) {
+                constexpr int d = I;
                 if constexpr (!IP) {
                     const int off = Lat<Q>::ex[d] + Lat<Q>::ey[d] * g.sy + Lat<Q>::ez[d] * g.sz;
                     f[d] = S::Load(src[d * P + (self - off)], d);
                 } else {
-                    const int pa = PairAnchor<Q>(d);
+                    constexpr int pa = PairAnchor<Q>(d);
                     const int anc = self + Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
                     int pl, c;
                     LoadSlot(d, par, self, anc, pl, c);
                     f[d] = S::Load(dst[pl * P + c], d);
                 }
-            }
+            });
         } else {
             uint32_t kinds[3] = {0u, 0u, 0u};
             if (near) {
@@ -361,8 +506,8 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
             }
             // Phase 1: ordinary streaming for every direction whose source is an
             // in-domain fluid cell (periodic wrap included).
-#pragma unroll
-            for (int d = 0; d < Q; ++d) {
+            StaticFor<0, Q>([&](auto I) {
+                constexpr int d = I;
                 const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
                 const bool outY = !a.pery && (sy_ < 0 || sy_ >= a.ny);
                 const bool outZ = !a.perz && (sz_ < 0 || sz_ >= a.nz);
@@ -370,17 +515,17 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                 f[d] = R(0);
                 if (d == 0 || (!outY && !outZ && kd == kPull)) {
                     if constexpr (!IP) {
-                        const int s = g.Nbr(x, y, z, -Lat<Q>::ex[d], -Lat<Q>::ey[d], -Lat<Q>::ez[d]);
-                        f[d] = S::Load(src[d * P + s], d);
+                        const int sidx = g.Nbr(x, y, z, -Lat<Q>::ex[d], -Lat<Q>::ey[d], -Lat<Q>::ez[d]);
+                        f[d] = S::Load(src[d * P + sidx], d);
                     } else {
-                        const int pa = PairAnchor<Q>(d);
+                        constexpr int pa = PairAnchor<Q>(d);
                         const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
                         int pl, c;
                         LoadSlot(d, par, self, anc, pl, c);
                         f[d] = S::Load(dst[pl * P + c], d);
                     }
                 }
-            }
+            });
             // Phase 2: walls and solid sources, from the cell's own previous
             // outgoing populations (LbmReference.cpp:100-122, 336-381).
             auto own_prev = [&](int d) -> R {
@@ -394,8 +539,8 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                     return S::Load(dst[pl * P + c], d);
                 }
             };
-#pragma unroll
-            for (int d = 1; d < Q; ++d) {
+            StaticFor<1, Q>([&](auto I) {
+                constexpr int d = I;
                 const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
                 const bool outY = !a.pery && (sy_ < 0 || sy_ >= a.ny);
                 const bool outZ = !a.perz && (sz_ < 0 || sz_ >= a.nz);
@@ -411,10 +556,10 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                     } else {
                         f[d] = own_prev(Opp(d));
                     }
-                    continue;
+                    return;
                 }
                 const unsigned kd = KindOf(kinds, d);
-                if (kd == kPull) continue;
+                if (kd == kPull) return;
                 const R plain = own_prev(Opp(d));
                 if (kd == kPlainBB) {
                     f[d] = plain;
@@ -431,100 +576,39 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                         fi = rec->side[d];
                     f[d] = inv2q * plain + (R(1) - inv2q) * fi;
                 }
-            }
+            });
         }
-
         // ------------------------------------------------------- collide
-        R rho = 0, ux = 0, uy = 0, uz = 0;
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            rho += f[d];
-            ux += f[d] * static_cast<R>(Lat<Q>::ex[d]);
-            uy += f[d] * static_cast<R>(Lat<Q>::ey[d]);
-            uz += f[d] * static_cast<R>(Lat<Q>::ez[d]);
-        }
-        if (rho > K<R>(1e-6f, 1e-6)) {
-            ux /= rho;
-            uy /= rho;
-            uz /= rho;
-        }
-        if (!__builtin_isfinite(rho) || !__builtin_isfinite(ux) || !__builtin_isfinite(uy) ||
-            !__builtin_isfinite(uz) || rho <= R(0) || (ux * ux + uy * uy + uz * uz) > R(4))
-            atomicOr(a.diverged, 1);
-
-        const R kCs2 = R(1) / R(3);
-        R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            const R fe = Feq<Q, R>(d, rho, ux, uy, uz);
-            const R fneq = f[d] - fe;
-            f[d] = fe;
-            const R cx = static_cast<R>(Lat<Q>::ex[d]), cy = static_cast<R>(Lat<Q>::ey[d]),
-                    cz = static_cast<R>(Lat<Q>::ez[d]);
-            pxx += cx * cx * fneq;
-            pyy += cy * cy * fneq;
-            pzz += cz * cz * fneq;
-            pxy += cx * cy * fneq;
-            pxz += cx * cz * fneq;
-            pyz += cy * cz * fneq;
-        }
-        const R piMag = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
-        const R rmax = rho < K<R>(1e-6f, 1e-6) ? K<R>(1e-6f, 1e-6) : rho;
-        const R kSmag = R(18) * K<R>(1.41421356f, 1.41421356);
-        const R tauEff = R(0.5) * (tau + sqrt(tau * tau + kSmag * cs * cs * piMag / rmax));
-        const R omega = R(1) / tauEff;
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            const R cx = static_cast<R>(Lat<Q>::ex[d]), cy = static_cast<R>(Lat<Q>::ey[d]),
-                    cz = static_cast<R>(Lat<Q>::ez[d]);
-            const R qxx = cx * cx - kCs2, qyy = cy * cy - kCs2, qzz = cz * cz - kCs2;
-            const R qxy = cx * cy, qxz = cx * cz, qyz = cy * cz;
-            const R reg = (W<Q, R>(d) * R(4.5)) *
-                          (qxx * pxx + qyy * pyy + qzz * pzz + R(2) * qxy * pxy + R(2) * qxz * pxz + R(2) * qyz * pyz);
-            fo[d] = f[d] + (R(1) - omega) * reg;
-        }
+        if (Collide<Q, R>(f, fo, tau, cs)) atomicOr(a.diverged, 1);
     }
 
     // ---------------------------------------------------------------- store
     if constexpr (!IP) {
-#pragma unroll
-        for (int d = 0; d < Q; ++d) PutPop(&dst[d * P + self], S::Store(fo[d], d));
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            PutPop(&dst[d * P + self], S::Store(fo[d], d));
+        });
     } else {
         const bool interior = x > 0 && x < a.nx - 1 && y > 0 && y < a.ny - 1 && z > 0 && z < a.nz - 1;
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            const int pa = PairAnchor<Q>(d);
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            constexpr int pa = PairAnchor<Q>(d);
             int anc = self;
-            if (d != 0) {
+            if constexpr (d != 0)
                 anc = interior ? self + Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz
                                : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
-            }
             int pl, c;
             StoreSlot(d, par, self, anc, pl, c);
             PutPop(&dst[pl * P + c], S::Store(fo[d], d));
-        }
+        });
         if (rec != nullptr) {
-            uint32_t kinds[3] = {rec->kind[0], rec->kind[1], rec->kind[2]};
-#pragma unroll
-            for (int d = 1; d < Q; ++d)
+            const uint32_t kinds[3] = {rec->kind[0], rec->kind[1], rec->kind[2]};
+            StaticFor<1, Q>([&](auto I) {
+                constexpr int d = I;
                 if (KindOf(kinds, d) == kBouzidiHi) rec->side[d] = S::Load(S::Store(fo[d], d), d);
+            });
         }
-        if (mom_writer) {
-            R m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-#pragma unroll
-            for (int d = 0; d < Q; ++d) {
-                R v = S::Load(S::Store(fo[d], d), d);
-                m0 += v;
-                m1 += v * static_cast<R>(Lat<Q>::ex[d]);
-                m2 += v * static_cast<R>(Lat<Q>::ey[d]);
-                m3 += v * static_cast<R>(Lat<Q>::ez[d]);
-            }
-            R* m = side_mom + ((static_cast<long long>(par) * 2 + mom_col) * plane_yz + z * a.ny + y) * 4;
-            m[0] = m0;
-            m[1] = m1;
-            m[2] = m2;
-            m[3] = m3;
-        }
+        if (mom_writer) WriteMoments<Q, S>(side_mom, fo, par, mom_col, plane_yz, z * a.ny + y);
     }
 }
 

@@ -1,0 +1,215 @@
+"""GPU checks of the surrogate's MFMA kernels (libfc_nn.so) against the FP64 NumPy
+oracle (oracle/nn_oracle.py), on bf16-representable inputs.
+
+Bars: convolution outputs in fp32 within 1e-4 of the FP64 result relative to the
+output scale (fp32 accumulation of exact bf16 products); bf16 outputs within 2^-7
+relative; whole-network parameter gradients (bf16 activations and activation
+gradients) within 5% relative norm of the FP64 gradients."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as no
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu-case] {request.node.nodeid}", flush=True)
+    yield
+
+
+def torch_mod():
+    import torch
+    return torch
+
+
+def bf(x):
+    """Round to bf16 and back (float64 result), like the device tensors."""
+    t = torch_mod()
+    return t.tensor(np.asarray(x, np.float32)).to(t.bfloat16).to(t.float32).numpy().astype(np.float64)
+
+
+def dev_bf16(x):
+    t = torch_mod()
+    return t.tensor(np.asarray(x, np.float32).reshape(-1)).to(t.bfloat16).cuda()
+
+
+def dev_f32(x):
+    t = torch_mod()
+    return t.tensor(np.asarray(x, np.float32).reshape(-1)).cuda()
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def stream():
+    return ctypes.c_void_p(torch_mod().cuda.current_stream().cuda_stream)
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("stride", [1, 2])
+@pytest.mark.parametrize("cin,cout", [(8, 16), (16, 32), (32, 64), (64, 128)])
+def test_conv_forward_and_adjoint(stride, cin, cout):
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    rng = np.random.default_rng(cin + cout + stride)
+    n, D, H, W = 2, 8, 6, 10
+    x = bf(rng.normal(size=(n, D, H, W, cin)))
+    w = bf(rng.normal(size=(cout, 27, cin)) / np.sqrt(27 * cin))
+    b = rng.normal(size=cout).astype(np.float32).astype(np.float64)
+    Do, Ho, Wo = no.out_dims(D, stride), no.out_dims(H, stride), no.out_dims(W, stride)
+    ref = no.conv_fwd(x, w, b, stride)
+    yd = t.empty(n * Do * Ho * Wo * cout, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_conv(0, ptr(dev_bf16(x)), None, ptr(dev_bf16(w)), ptr(dev_f32(b)), None, ptr(yd), n, D, H, W, Do,
+                      Ho, Wo, cin, 0, cout, stride, 0, stream())
+    assert rc == 0
+    assert rel(yd.cpu().numpy().reshape(ref.shape), ref) < 1e-4
+    # leaky + bf16 output path
+    yb = t.empty(n * Do * Ho * Wo * cout, dtype=t.bfloat16, device="cuda")
+    rc = L.fc_nn_conv(0, ptr(dev_bf16(x)), None, ptr(dev_bf16(w)), ptr(dev_f32(b)), ptr(yb), None, n, D, H, W, Do,
+                      Ho, Wo, cin, 0, cout, stride, 1, stream())
+    assert rc == 0
+    assert rel(yb.float().cpu().numpy().reshape(ref.shape), no.leaky(ref)) < 2 ** -7
+    # adjoint (data gradient) with pre-transposed weights
+    dy = bf(rng.normal(size=(n, Do, Ho, Wo, cout)))
+    wt = t.empty(cout * 27 * cin, dtype=t.bfloat16, device="cuda")
+    assert L.fc_nn_transpose_w(ptr(dev_bf16(w)), ptr(wt), cout, cin, stream()) == 0
+    dxd = t.empty(n * D * H * W * cin, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_conv(2, ptr(dev_bf16(dy)), None, ptr(wt), None, None, ptr(dxd), n, Do, Ho, Wo, D, H, W, cout, 0,
+                      cin, stride, 0, stream())
+    assert rc == 0
+    assert rel(dxd.cpu().numpy().reshape(x.shape), no.conv_adj(dy, w, x.shape, stride)) < 1e-4
+    # weight gradient (split-K, fp32)
+    gw = t.empty(cout * 27 * cin, dtype=t.float32, device="cuda")
+    splits = 3
+    ws = t.empty(splits * cout * 27 * cin, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_wgrad(0, ptr(dev_bf16(x)), None, ptr(dev_bf16(dy)), ptr(gw), ptr(ws), splits, 0, n, D, H, W, Do, Ho,
+                       Wo, cin, 0, cout, stride, stream())
+    assert rc == 0
+    assert rel(gw.cpu().numpy().reshape(cout, 27, cin), no.conv_wgrad(x, dy, stride)) < 1e-4
+
+
+@pytest.mark.parametrize("c1,c2,cout", [(32, 16, 16), (64, 32, 32), (128, 64, 64)])
+def test_upcat_conv_and_split(c1, c2, cout):
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    rng = np.random.default_rng(c1)
+    n, D, H, W = 1, 8, 4, 6
+    xc = bf(rng.normal(size=(n, D // 2, H // 2, W // 2, c1)))
+    xs = bf(rng.normal(size=(n, D, H, W, c2)))
+    kin = c1 + c2
+    w = bf(rng.normal(size=(cout, 27, kin)) / np.sqrt(27 * kin))
+    b = np.zeros(cout)
+    cat = np.concatenate([no.up2(xc), xs], axis=-1)
+    ref = no.conv_fwd(cat, w, b, 1)
+    yd = t.empty(n * D * H * W * cout, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_conv(1, ptr(dev_bf16(xs)), ptr(dev_bf16(xc)), ptr(dev_bf16(w)), ptr(dev_f32(b)), None, ptr(yd), n, D,
+                      H, W, D, H, W, kin, c1, cout, 1, 0, stream())
+    assert rc == 0
+    assert rel(yd.cpu().numpy().reshape(ref.shape), ref) < 1e-4
+    dy = bf(rng.normal(size=(n, D, H, W, cout)))
+    gw = t.empty(cout * 27 * kin, dtype=t.float32, device="cuda")
+    ws = t.empty(2 * cout * 27 * kin, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_wgrad(1, ptr(dev_bf16(xs)), ptr(dev_bf16(xc)), ptr(dev_bf16(dy)), ptr(gw), ptr(ws), 2, 0, n, D, H, W,
+                       D, H, W, kin, c1, cout, 1, stream())
+    assert rc == 0
+    assert rel(gw.cpu().numpy().reshape(cout, 27, kin), no.conv_wgrad(cat, dy, 1)) < 1e-4
+    # split of a concatenated gradient: skip part added, coarse part summed over children
+    dcat = bf(rng.normal(size=(n, D, H, W, kin)))
+    dskip = dev_bf16(np.full((n, D, H, W, c2), 0.5))
+    dco = t.empty(n * D * H * W // 8 * c1, dtype=t.bfloat16, device="cuda")
+    assert L.fc_nn_upcat_split(ptr(dev_bf16(dcat)), c1, c2, n, D, H, W, ptr(dskip), ptr(dco), stream()) == 0
+    assert rel(dskip.float().cpu().numpy().reshape(n, D, H, W, c2), dcat[..., c1:] + 0.5) < 2 ** -7
+    assert rel(dco.float().cpu().numpy().reshape(n, D // 2, H // 2, W // 2, c1), no.down2_sum(dcat[..., :c1])) < 2 ** -6
+
+
+def test_head_loss_colsum_adam():
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    rng = np.random.default_rng(5)
+    V, C = 5000, 16
+    h = bf(rng.normal(size=(V, C)))
+    wh = rng.normal(size=(4, C)).astype(np.float32).astype(np.float64)
+    bh = rng.normal(size=4).astype(np.float32).astype(np.float64)
+    tgt = rng.normal(size=(V, 4)).astype(np.float32).astype(np.float64)
+    wm = (rng.random(V) > 0.3).astype(np.float64)
+    pred = h @ wh.T + bh
+    g = 2 * wm[:, None] * (pred - tgt) / (4.0 * V)
+    loss = (wm[:, None] * (pred - tgt) ** 2).sum() / (4.0 * V)
+    dh = t.empty(V * C, dtype=t.bfloat16, device="cuda")
+    ws = t.empty(512 * (4 * C + 5), dtype=t.float32, device="cuda")
+    out = t.empty(4 * C + 5, dtype=t.float32, device="cuda")
+    pr = t.empty(V * 4, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_head_loss(ptr(dev_bf16(h)), C, ptr(dev_f32(wh)), ptr(dev_f32(bh)), ptr(dev_f32(tgt)), ptr(dev_f32(wm)),
+                           V, 1.0 / (4.0 * V), ptr(dh), ptr(ws), ptr(out), ptr(pr), stream())
+    assert rc == 0
+    o = out.cpu().numpy().astype(np.float64)
+    assert rel(pr.cpu().numpy().reshape(V, 4), pred) < 1e-5
+    assert rel(o[:4 * C].reshape(4, C), g.T @ h) < 1e-4
+    assert rel(o[4 * C:4 * C + 4], g.sum(0)) < 1e-4
+    assert abs(o[-1] - loss) < 1e-5 * loss
+    assert rel(dh.float().cpu().numpy().reshape(V, C), g @ wh) < 2 ** -7
+    # bias-gradient column sums
+    cs = t.empty(C, dtype=t.float32, device="cuda")
+    cws = t.empty(256 * C, dtype=t.float32, device="cuda")
+    assert L.fc_nn_colsum(ptr(dev_bf16(h)), V, C, ptr(cs), ptr(cws), 0, stream()) == 0
+    assert rel(cs.cpu().numpy(), h.sum(0)) < 1e-5
+    # Adam, two steps, against the textbook update (Kingma & Ba 2015)
+    p0 = rng.normal(size=1000).astype(np.float32)
+    g1 = rng.normal(size=1000).astype(np.float32)
+    p, m, v = dev_f32(p0), t.zeros(1000, device="cuda"), t.zeros(1000, device="cuda")
+    pb = t.empty(1000, dtype=t.bfloat16, device="cuda")
+    pe, me, ve = p0.astype(np.float64), np.zeros(1000), np.zeros(1000)
+    for k in (1, 2):
+        assert L.fc_nn_adam(ptr(p), ptr(dev_f32(g1)), ptr(m), ptr(v), ptr(pb), 1000, 1e-3, 0.9, 0.999, 1e-8, k, 1.0,
+                            stream()) == 0
+        me = 0.9 * me + 0.1 * g1
+        ve = 0.999 * ve + 0.001 * g1.astype(np.float64) ** 2
+        pe = pe - 1e-3 * (me / (1 - 0.9 ** k)) / (np.sqrt(ve / (1 - 0.999 ** k)) + 1e-8)
+    assert np.abs(p.cpu().numpy() - pe).max() < 1e-6
+    assert np.array_equal(pb.cpu(), p.cpu().to(t.bfloat16))
+
+
+def test_network_gradients_match_fp64_oracle():
+    """Production layer list at a small resolution (nb=2, 16x8x8): every parameter
+    gradient of the GPU backward within 5% relative norm of the FP64 oracle's, and the
+    loss within 1%."""
+    from flowcraft_amd.surrogate import LAYERS, HEAD_IN, Surrogate
+    t = torch_mod()
+    net = Surrogate(seed=3)
+    nb, dims = 2, (16, 8, 8)
+    rng = np.random.default_rng(7)
+    x = bf(rng.normal(size=(nb, *dims, 8)))
+    tgt = rng.normal(size=(nb, *dims, 4)).astype(np.float32).astype(np.float64)
+    wm = (rng.random((nb, *dims)) > 0.2).astype(np.float64)
+    sd = net.state_dict()
+    params = {name: (bf(sd[f"{name}.w"]), sd[f"{name}.b"].astype(np.float64)) for name, *_ in LAYERS}
+    acts, pred = no.forward(params, x, LAYERS, sd["head.w"].astype(np.float64), sd["head.b"].astype(np.float64))
+    ref_loss = no.loss_fn(pred, tgt, wm)
+    ref = no.backward(params, acts, pred, tgt, wm, LAYERS, sd["head.w"].astype(np.float64))
+    xa = net.forward(dev_bf16(x), nb, dims)
+    loss, dh = net.head(xa, nb, dims, tgt=dev_f32(tgt), wmask=dev_f32(wm), want_grad=True)
+    net.backward(xa, dh, nb, dims)
+    t.cuda.synchronize()
+    assert abs(float(loss.cpu()[0]) - ref_loss) < 1e-2 * ref_loss
+    for name, *_ in LAYERS:
+        p = net.params[name]
+        for kind, got in (("w", p["gw"]), ("b", p["gb"])):
+            r = ref[f"{name}.{kind}"]
+            e = np.linalg.norm(got.cpu().numpy().reshape(r.shape) - r) / max(np.linalg.norm(r), 1e-30)
+            print(name, kind, f"{e:.3e}")
+            assert e < 5e-2, (name, kind, e)
+    hg = net.head_gw.cpu().numpy()
+    assert np.linalg.norm(hg[:4 * HEAD_IN].reshape(4, HEAD_IN) - ref["head.w"]) < 1e-2 * np.linalg.norm(ref["head.w"])
