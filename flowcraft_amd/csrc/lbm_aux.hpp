@@ -211,20 +211,23 @@ __global__ void KSeed(void* pops0, void* pops1, long long P, long long t, int nx
     }
     const int self = g.Idx(x, y, z);
     R feq[Q];
-    FeqAll<Q, R>(rho, ux, uy, uz, feq);
+    if constexpr (S::kOffset)
+        EqOffset<Q, S>(rho - 1.0f, ux, uy, uz, feq);
+    else
+        FeqAll<Q, R>(rho, ux, uy, uz, feq);
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         const R fe = feq[d];
         if constexpr (!Lay::kInPlace) {
             T* f = static_cast<T*>((t & 1) ? pops1 : pops0);
-            f[d * P + self] = S::Store(fe, d);
+            f[d * P + self] = S::Enc(fe);
         } else {
             T* f = static_cast<T*>(pops0);
             const int pa = PairAnchor<Q>(d);
             const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
             int pl, cc;
             StoreSlot(d, static_cast<int>((t - 1) & 1), self, anc, pl, cc);
-            f[pl * P + cc] = S::Store(fe, d);
+            f[pl * P + cc] = S::Enc(fe);
         }
     }
 }
@@ -293,7 +296,7 @@ __global__ void KRefreshSide(const void* pops0, const void* pops1, long long P, 
         const int z = n / (nx * (ny + 2)) - 1;
         BRec<R>& r = recs[k];
         for (int d = 1; d < Q; ++d)
-            if (KindOf(r.kind, d) == kBouzidiHi) r.side[d] = CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
+            if (KindOf(r.kind, d) == kBouzidiHi) r.side[d] = CurrentRaw<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
     } else if (k < count + ncol) {
         const int j = k - count;
         const int col = j / (ny * nz);
@@ -302,7 +305,7 @@ __global__ void KRefreshSide(const void* pops0, const void* pops1, long long P, 
         const int x = col == 0 ? 1 : nx - 2;
         R m0 = 0, m1 = 0, m2 = 0, m3 = 0;
         for (int d = 0; d < Q; ++d) {
-            R v = CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
+            R v = CurrentRaw<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
             m0 += v;
             m1 += v * static_cast<R>(Lat<Q>::ex[d]);
             m2 += v * static_cast<R>(Lat<Q>::ey[d]);

@@ -60,38 +60,51 @@ __host__ __device__ constexpr R W(int d) {
 }
 
 // -------------------------------------------------------- storage formats
+// Raw/Enc move between the stored element and the step's working value: the
+// population f itself for FP32/FP64, the scaled offset g = (f - w) * S for the
+// offset formats, whose step runs entirely on g (see CollideOffset).  Load/Store
+// convert to and from f for readbacks, seeding and state dump/restore.
 // FP32: the reference's float populations.
 template <int Q>
 struct StoreF32 {
     using T = float;
     using R = float;
+    static constexpr bool kOffset = false;
     static __device__ __forceinline__ R Load(T v, int) { return v; }
     static __device__ __forceinline__ T Store(R f, int) { return f; }
+    static __device__ __forceinline__ R Raw(T v) { return v; }
+    static __device__ __forceinline__ T Enc(R v) { return v; }
 };
 
-// FP16 offset: g = (f - w) * 4096 as IEEE binary16 (RNE); collision in FP32.
+// FP16 offset: g = (f - w) * 4096 as IEEE binary16 (RNE); arithmetic in FP32.
 template <int Q>
 struct StoreF16 {
     using T = _Float16;
     using R = float;
+    static constexpr bool kOffset = true;
+    static constexpr float kScale = 4096.0f;
     static __device__ __forceinline__ R Load(T v, int d) { return Lat<Q>::w[d] + static_cast<float>(v) * (1.0f / 4096.0f); }
     static __device__ __forceinline__ T Store(R f, int d) { return static_cast<T>((f - Lat<Q>::w[d]) * 4096.0f); }
+    static __device__ __forceinline__ R Raw(T v) { return static_cast<float>(v); }
+    static __device__ __forceinline__ T Enc(R g) { return static_cast<T>(g); }
 };
 
-// BF16 offset: g = f - w as bfloat16 (RNE, NaN kept quiet); collision in FP32.
+// BF16 offset: g = f - w as bfloat16 (RNE, NaN kept quiet); arithmetic in FP32.
 template <int Q>
 struct StoreBF16 {
     using T = uint16_t;
     using R = float;
-    static __device__ __forceinline__ R Load(T v, int d) {
-        return Lat<Q>::w[d] + __uint_as_float(static_cast<uint32_t>(v) << 16);
-    }
-    static __device__ __forceinline__ T Store(R f, int d) {
-        uint32_t b = __float_as_uint(f - Lat<Q>::w[d]);
+    static constexpr bool kOffset = true;
+    static constexpr float kScale = 1.0f;
+    static __device__ __forceinline__ R Raw(T v) { return __uint_as_float(static_cast<uint32_t>(v) << 16); }
+    static __device__ __forceinline__ T Enc(R g) {
+        uint32_t b = __float_as_uint(g);
         if ((b & 0x7FFFFFFFu) > 0x7F800000u) return static_cast<T>((b >> 16) | 0x40u);
         b += 0x7FFFu + ((b >> 16) & 1u);
         return static_cast<T>(b >> 16);
     }
+    static __device__ __forceinline__ R Load(T v, int d) { return Lat<Q>::w[d] + Raw(v); }
+    static __device__ __forceinline__ T Store(R f, int d) { return Enc(f - Lat<Q>::w[d]); }
 };
 
 // FP64: the whole step in double (GPU twin of the FP64 oracle).
@@ -99,8 +112,11 @@ template <int Q>
 struct StoreF64 {
     using T = double;
     using R = double;
+    static constexpr bool kOffset = false;
     static __device__ __forceinline__ R Load(T v, int) { return v; }
     static __device__ __forceinline__ T Store(R f, int) { return f; }
+    static __device__ __forceinline__ R Raw(T v) { return v; }
+    static __device__ __forceinline__ T Enc(R v) { return v; }
 };
 
 // ------------------------------------------------------ boundary records
@@ -221,6 +237,25 @@ __device__ __forceinline__ typename S::R CurrentPop(const void* pops0, const voi
         int pl, c;
         StoreSlot(d, static_cast<int>((t - 1) & 1), self, anc, pl, c);
         return S::Load(f[pl * P + c], d);
+    }
+}
+
+// The same slot as the step's working value (S::Raw): f, or the offset g.
+template <int Q, class S, class Lay>
+__device__ __forceinline__ typename S::R CurrentRaw(const void* pops0, const void* pops1, long long P, long long t,
+                                                   const Geo& g, int x, int y, int z, int d) {
+    using T = typename S::T;
+    const int self = g.Idx(x, y, z);
+    if constexpr (!Lay::kInPlace) {
+        const T* f = static_cast<const T*>((t & 1) ? pops1 : pops0);
+        return S::Raw(f[d * P + self]);
+    } else {
+        const T* f = static_cast<const T*>(pops0);
+        const int a = PairAnchor<Q>(d);
+        const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[a], Lat<Q>::ey[a], Lat<Q>::ez[a]);
+        int pl, c;
+        StoreSlot(d, static_cast<int>((t - 1) & 1), self, anc, pl, c);
+        return S::Raw(f[pl * P + c]);
     }
 }
 
@@ -358,6 +393,106 @@ __device__ __forceinline__ bool Collide(const R (&f)[Q], R (&fo)[Q], R tau, R cs
     return bad;
 }
 
+// --------------------------------------------------- offset-space collision
+// The same regularized BGK + Smagorinsky model for the offset storage formats,
+// written on g_i = (f_i - w_i) * S in moment space: with rho = 1 + drho,
+//   drho = sum g / S,  j = sum e g / S,  Pi_neq = sum e e g / S - j u - (drho/3) I,
+//   T = j u + (1 - 1/tau_eff) Pi_neq,
+//   g_i' = (w_i S) (drho + 3 e.j + 4.5 Q_i : T),  Q_i = e e - I/3.
+// Identical to the reference's population-space form in exact arithmetic (the
+// lattice moments of f^eq are exact to second order for D3Q19 and D3Q15); it works
+// on small offsets instead of O(w) populations and needs about half the arithmetic.
+// The CPU oracle emulates exactly these operations for the offset formats.
+template <int Q, class S>
+__device__ __forceinline__ void ReconstructOffset(float drho, float jx, float jy, float jz, float txx, float tyy,
+                                                  float tzz, float txy, float txz, float tyz, float (&go)[Q]) {
+    constexpr float c3 = 1.0f / 3.0f;
+    const float tr3 = (txx + tyy + tzz) * c3;
+    const float j3x = 3.0f * jx, j3y = 3.0f * jy, j3z = 3.0f * jz;
+    const float t2xy = 2.0f * txy, t2xz = 2.0f * txz, t2yz = 2.0f * tyz;
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        constexpr int cx = Lat<Q>::ex[d], cy = Lat<Q>::ey[d], cz = Lat<Q>::ez[d];
+        constexpr float ws = Lat<Q>::w[d] * S::kScale;
+        float qt;
+        if constexpr (cx == 0 && cy == 0 && cz == 0) {
+            qt = -tr3;
+        } else {
+            float diag;
+            if constexpr (cx != 0) {
+                diag = txx;
+                if constexpr (cy != 0) diag = diag + tyy;
+                if constexpr (cz != 0) diag = diag + tzz;
+            } else if constexpr (cy != 0) {
+                diag = tyy;
+                if constexpr (cz != 0) diag = diag + tzz;
+            } else {
+                diag = tzz;
+            }
+            qt = diag - tr3;
+        }
+        AccC<cx * cy>(qt, t2xy);
+        AccC<cx * cz>(qt, t2xz);
+        AccC<cy * cz>(qt, t2yz);
+        float base;
+        if constexpr (cx == 0 && cy == 0 && cz == 0)
+            base = drho;
+        else
+            base = drho + Dot3<cx, cy, cz>(j3x, j3y, j3z);
+        go[d] = ws * (base + 4.5f * qt);
+    });
+}
+
+// Equilibrium offsets of (1 + drho, u).
+template <int Q, class S>
+__device__ __forceinline__ void EqOffset(float drho, float ux, float uy, float uz, float (&go)[Q]) {
+    const float rho = 1.0f + drho;
+    const float jx = rho * ux, jy = rho * uy, jz = rho * uz;
+    ReconstructOffset<Q, S>(drho, jx, jy, jz, jx * ux, jy * uy, jz * uz, jx * uy, jx * uz, jy * uz, go);
+}
+
+template <int Q, class S>
+__device__ __forceinline__ bool CollideOffset(const float (&g)[Q], float (&go)[Q], float tau, float cs) {
+    constexpr float invS = 1.0f / S::kScale;
+    float sg = 0, Jx = 0, Jy = 0, Jz = 0, gxx = 0, gyy = 0, gzz = 0, gxy = 0, gxz = 0, gyz = 0;
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        constexpr int cx = Lat<Q>::ex[d], cy = Lat<Q>::ey[d], cz = Lat<Q>::ez[d];
+        sg += g[d];
+        AccC<cx>(Jx, g[d]);
+        AccC<cy>(Jy, g[d]);
+        AccC<cz>(Jz, g[d]);
+        if constexpr (cx != 0) gxx += g[d];
+        if constexpr (cy != 0) gyy += g[d];
+        if constexpr (cz != 0) gzz += g[d];
+        AccC<cx * cy>(gxy, g[d]);
+        AccC<cx * cz>(gxz, g[d]);
+        AccC<cy * cz>(gyz, g[d]);
+    });
+    const float drho = sg * invS;
+    const float jx = Jx * invS, jy = Jy * invS, jz = Jz * invS;
+    const float rho = 1.0f + drho;
+    const float r = rho > 1e-6f ? 1.0f / rho : 1.0f;
+    const float ux = jx * r, uy = jy * r, uz = jz * r;
+    const bool bad = !__builtin_isfinite(rho) || !__builtin_isfinite(ux) || !__builtin_isfinite(uy) ||
+                     !__builtin_isfinite(uz) || rho <= 0.0f || (ux * ux + uy * uy + uz * uz) > 4.0f;
+    constexpr float c3 = 1.0f / 3.0f;
+    const float pxx = (gxx * invS - jx * ux) - drho * c3;
+    const float pyy = (gyy * invS - jy * uy) - drho * c3;
+    const float pzz = (gzz * invS - jz * uz) - drho * c3;
+    const float pxy = gxy * invS - jx * uy;
+    const float pxz = gxz * invS - jx * uz;
+    const float pyz = gyz * invS - jy * uz;
+    const float piMag = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const float rmax = rho < 1e-6f ? 1e-6f : rho;
+    const float kSmag = 18.0f * 1.41421356f;
+    const float tauEff = 0.5f * (tau + sqrtf(tau * tau + kSmag * cs * cs * piMag / rmax));
+    const float keep = 1.0f - 1.0f / tauEff;
+    ReconstructOffset<Q, S>(drho, jx, jy, jz, jx * ux + keep * pxx, jy * uy + keep * pyy, jz * uz + keep * pzz,
+                            jx * uy + keep * pxy, jx * uz + keep * pxz, jy * uz + keep * pyz, go);
+    return bad;
+}
+
 // ----------------------------------------------------------- step kernel
 // Work decomposition: one workgroup = a 64 (x) by 4 (y) tile of one z layer, 256
 // threads, no integer division per thread.  Workgroups are dealt to the 8 XCDs
@@ -378,7 +513,7 @@ __device__ __forceinline__ void WriteMoments(typename S::R* side_mom, const type
     R v[Q];
     StaticFor<0, Q>([&](auto I) {
         constexpr int d = I;
-        v[d] = S::Load(S::Store(fo[d], d), d);
+        v[d] = S::Raw(S::Enc(fo[d]));
     });
     R m0, m1, m2, m3;
     MomentSums<Q, R>(v, m0, m1, m2, m3);
@@ -426,12 +561,15 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
         // cells are inactive; only the outlet-column moments they feed are written.
         StaticFor<0, Q>([&](auto I) {
             constexpr int d = I;
-            fo[d] = W<Q, R>(d);
+            if constexpr (S::kOffset)
+                fo[d] = R(0);
+            else
+                fo[d] = W<Q, R>(d);
         });
         if constexpr (!IP) {
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                PutPop(&dst[d * P + self], S::Store(fo[d], d));
+                PutPop(&dst[d * P + self], S::Enc(fo[d]));
             });
         }
         if (mom_writer) WriteMoments<Q, S>(side_mom, fo, par, mom_col, plane_yz, z * a.ny + y);
@@ -448,7 +586,10 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
 
     if (x == inletX) {
         // Equilibrium inlet (LbmReference.cpp:269-273).
-        FeqAll<Q, R>(R(1), a.reverse ? -uin : uin, R(0), R(0), fo);
+        if constexpr (S::kOffset)
+            EqOffset<Q, S>(0.0f, a.reverse ? -uin : uin, 0.0f, 0.0f, fo);
+        else
+            FeqAll<Q, R>(R(1), a.reverse ? -uin : uin, R(0), R(0), fo);
     } else if (x == outletX) {
         // Zero-gradient outlet from the neighbour's previous post-collision moments,
         // density relaxed toward 1 (LbmReference.cpp:275-327).
@@ -459,7 +600,7 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
             R v[Q];
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                v[d] = S::Load(src[d * P + nb], d);
+                v[d] = S::Raw(src[d * P + nb]);
             });
             MomentSums<Q, R>(v, nr, nu, nv, nw);
         } else {
@@ -470,13 +611,29 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
             nv = m[2];
             nw = m[3];
         }
-        if (nr > K<R>(1e-6f, 1e-6)) {
-            nu /= nr;
-            nv /= nr;
-            nw /= nr;
+        if constexpr (S::kOffset) {
+            // sums of offsets: rho_n - 1 and j_n in population units
+            constexpr float invS = 1.0f / S::kScale;
+            const float dn = nr * invS;
+            const float rn = 1.0f + dn;
+            float ux = nu * invS, uy = nv * invS, uz = nw * invS, drho = 0.0f;
+            if (rn > 1e-6f) {
+                const float r = 1.0f / rn;
+                ux *= r;
+                uy *= r;
+                uz *= r;
+                drho = 0.999f * dn;
+            }
+            EqOffset<Q, S>(drho, ux, uy, uz, fo);
+        } else {
+            if (nr > K<R>(1e-6f, 1e-6)) {
+                nu /= nr;
+                nv /= nr;
+                nw /= nr;
+            }
+            const R rho = nr > K<R>(1e-6f, 1e-6) ? R(1) + K<R>(0.999f, 0.999) * (nr - R(1)) : R(1);
+            FeqAll<Q, R>(rho, nu, nv, nw, fo);
         }
-        const R rho = nr > K<R>(1e-6f, 1e-6) ? R(1) + K<R>(0.999f, 0.999) * (nr - R(1)) : R(1);
-        FeqAll<Q, R>(rho, nu, nv, nw, fo);
     } else {
         // ---------------------------------------------------------- gather
         R f[Q];
@@ -487,13 +644,13 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                 constexpr int d = I;
                 if constexpr (!IP) {
                     const int off = Lat<Q>::ex[d] + Lat<Q>::ey[d] * g.sy + Lat<Q>::ez[d] * g.sz;
-                    f[d] = S::Load(src[d * P + (self - off)], d);
+                    f[d] = S::Raw(src[d * P + (self - off)]);
                 } else {
                     constexpr int pa = PairAnchor<Q>(d);
                     const int anc = self + Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
                     int pl, c;
                     LoadSlot(d, par, self, anc, pl, c);
-                    f[d] = S::Load(dst[pl * P + c], d);
+                    f[d] = S::Raw(dst[pl * P + c]);
                 }
             });
         } else {
@@ -516,13 +673,13 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                 if (d == 0 || (!outY && !outZ && kd == kPull)) {
                     if constexpr (!IP) {
                         const int sidx = g.Nbr(x, y, z, -Lat<Q>::ex[d], -Lat<Q>::ey[d], -Lat<Q>::ez[d]);
-                        f[d] = S::Load(src[d * P + sidx], d);
+                        f[d] = S::Raw(src[d * P + sidx]);
                     } else {
                         constexpr int pa = PairAnchor<Q>(d);
                         const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
                         int pl, c;
                         LoadSlot(d, par, self, anc, pl, c);
-                        f[d] = S::Load(dst[pl * P + c], d);
+                        f[d] = S::Raw(dst[pl * P + c]);
                     }
                 }
             });
@@ -530,13 +687,13 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
             // outgoing populations (LbmReference.cpp:100-122, 336-381).
             auto own_prev = [&](int d) -> R {
                 if constexpr (!IP) {
-                    return S::Load(src[d * P + self], d);
+                    return S::Raw(src[d * P + self]);
                 } else {
                     const int pa = PairAnchor<Q>(d);
                     const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
                     int pl, c;
                     StoreSlot(d, par ^ 1, self, anc, pl, c);
-                    return S::Load(dst[pl * P + c], d);
+                    return S::Raw(dst[pl * P + c]);
                 }
             };
             StaticFor<1, Q>([&](auto I) {
@@ -571,7 +728,7 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                     const R inv2q = R(1) / (R(2) * q);
                     R fi;
                     if constexpr (!IP)
-                        fi = S::Load(src[d * P + self], d);
+                        fi = S::Raw(src[d * P + self]);
                     else
                         fi = rec->side[d];
                     f[d] = inv2q * plain + (R(1) - inv2q) * fi;
@@ -579,14 +736,19 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
             });
         }
         // ------------------------------------------------------- collide
-        if (Collide<Q, R>(f, fo, tau, cs)) atomicOr(a.diverged, 1);
+        bool bad;
+        if constexpr (S::kOffset)
+            bad = CollideOffset<Q, S>(f, fo, tau, cs);
+        else
+            bad = Collide<Q, R>(f, fo, tau, cs);
+        if (bad) atomicOr(a.diverged, 1);
     }
 
     // ---------------------------------------------------------------- store
     if constexpr (!IP) {
         StaticFor<0, Q>([&](auto I) {
             constexpr int d = I;
-            PutPop(&dst[d * P + self], S::Store(fo[d], d));
+            PutPop(&dst[d * P + self], S::Enc(fo[d]));
         });
     } else {
         const bool interior = x > 0 && x < a.nx - 1 && y > 0 && y < a.ny - 1 && z > 0 && z < a.nz - 1;
@@ -599,13 +761,13 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                                : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
             int pl, c;
             StoreSlot(d, par, self, anc, pl, c);
-            PutPop(&dst[pl * P + c], S::Store(fo[d], d));
+            PutPop(&dst[pl * P + c], S::Enc(fo[d]));
         });
         if (rec != nullptr) {
             const uint32_t kinds[3] = {rec->kind[0], rec->kind[1], rec->kind[2]};
             StaticFor<1, Q>([&](auto I) {
                 constexpr int d = I;
-                if (KindOf(kinds, d) == kBouzidiHi) rec->side[d] = S::Load(S::Store(fo[d], d), d);
+                if (KindOf(kinds, d) == kBouzidiHi) rec->side[d] = S::Raw(S::Enc(fo[d]));
             });
         }
         if (mom_writer) WriteMoments<Q, S>(side_mom, fo, par, mom_col, plane_yz, z * a.ny + y);

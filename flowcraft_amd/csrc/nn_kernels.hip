@@ -415,6 +415,12 @@ __global__ void KAdam(float* __restrict__ p, const float* __restrict__ g, float*
     if (pbf) pbf[i] = F2Bf(pi);
 }
 
+// y += x (bf16 tensors, fp32 add): accumulation of activation gradients.
+__global__ void KAddBf16(const u16* __restrict__ x, u16* __restrict__ y, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = F2Bf(Bf2F(y[i]) + Bf2F(x[i]));
+}
+
 __global__ void KF32ToBf16(const float* __restrict__ x, u16* __restrict__ y, long long n) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) y[i] = F2Bf(x[i]);
@@ -589,6 +595,12 @@ int fc_nn_adam(float* p, const float* g, float* m, float* v, void* pbf, long lon
     const float bc2 = 1.f - powf(b2, static_cast<float>(step));
     hipLaunchKernelGGL(KAdam, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), p, g, m, v,
                        static_cast<u16*>(pbf), n, lr, b1, b2, eps, bc1, bc2, gscale);
+    return static_cast<int>(hipGetLastError());
+}
+
+int fc_nn_add_bf16(const void* x, void* y, long long n, void* stream) {
+    hipLaunchKernelGGL(KAddBf16, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const u16*>(x), static_cast<u16*>(y), n);
     return static_cast<int>(hipGetLastError());
 }
 

@@ -61,7 +61,7 @@ def nn_lib() -> ctypes.CDLL:
         raise RuntimeError(f"{NN_PATH} is missing: build with __graft_entry__.build()")
     L = ctypes.CDLL(NN_PATH)
     i, ll, vp, f = ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_float
-    L.fc_nn_conv.argtypes = [i, vp, vp, vp, vp, vp, vp] + [i] * 13 + [vp]
+    L.fc_nn_conv.argtypes = [i, vp, vp, vp, vp, vp, vp] + [i] * 12 + [vp]
     L.fc_nn_wgrad.argtypes = [i, vp, vp, vp, vp, vp, i, i] + [i] * 11 + [vp]
     L.fc_nn_colsum.argtypes = [vp, ll, i, vp, vp, i, vp]
     L.fc_nn_leaky_backward.argtypes = [vp, vp, vp, ll, vp]
@@ -69,6 +69,7 @@ def nn_lib() -> ctypes.CDLL:
     L.fc_nn_upcat_split.argtypes = [vp, i, i, i, i, i, i, vp, vp, vp]
     L.fc_nn_head_loss.argtypes = [vp, i, vp, vp, vp, vp, ll, f, vp, vp, vp, vp, vp]
     L.fc_nn_adam.argtypes = [vp, vp, vp, vp, vp, ll, f, f, f, f, i, f, vp]
+    L.fc_nn_add_bf16.argtypes = [vp, vp, ll, vp]
     L.fc_nn_f32_to_bf16.argtypes = [vp, vp, ll, vp]
     L.fc_nn_bf16_to_f32.argtypes = [vp, vp, ll, vp]
     L.fc_nn_features.argtypes = [vp, vp, f, i, i, i, i, vp, vp]
@@ -249,7 +250,7 @@ class Surrogate:
                 _chk(L.fc_nn_conv(ADJ, _p(dz), None, _p(wt), None, _p(dx), None, nb, dd, hd, wd, si[0], si[1], si[2],
                                   cout, 0, kin, stride, 0, s), name + " dgrad")
                 if src in grads:
-                    grads[src] = grads[src] + dx
+                    _chk(L.fc_nn_add_bf16(_p(dx), _p(grads[src]), dx.numel(), s), "add")
                 else:
                     grads[src] = dx
             else:
@@ -263,7 +264,7 @@ class Surrogate:
                 _chk(L.fc_nn_upcat_split(_p(dcat), c1, kin - c1, nb, dd, hd, wd, _p(grads[skip]), _p(dcoarse), s),
                      name + " split")
                 if coarse in grads:
-                    grads[coarse] = grads[coarse] + dcoarse
+                    _chk(L.fc_nn_add_bf16(_p(dcoarse), _p(grads[coarse]), dcoarse.numel(), s), "add")
                 else:
                     grads[coarse] = dcoarse
         return grads

@@ -1,0 +1,297 @@
+"""Warm-start pipeline: training data from the engine itself, surrogate training,
+and the warm-vs-plain convergence study.
+
+Dataset: random obstacles (half NACA 4-digit wings extruded over z, half
+ellipsoids; ``geometry.random_shape_grid``) on a D3Q19 Smagorinsky lattice,
+each run from the uniform rest start for a fixed number of steps; the target is
+the final (rho, u) field in normalised units ((rho-1)*3/u^2, u/u_in).
+
+Convergence study: for each held-out shape, a plain run (uniform rest start)
+to ``horizon`` steps sets the reference drag coefficient; a warm run starts from
+the surrogate's field.  Cd(t) is the momentum-exchange drag Fx / (0.5 U^2 A)
+averaged over the trailing ``window`` steps (sampled every ``every`` steps); a run
+has converged at the first sample after which Cd(t) stays within ``tol`` of the
+plain run's final Cd.  A warm run that diverges or never converges falls back to
+the plain start (counted).  The reference's own warm start — a coarse lattice at
+1/``coarse_div`` resolution run ``coarse_steps`` steps and nearest-sampled onto the
+fine grid (FlowCraft/main.cpp:1413-1564) — is the third arm.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import geometry
+from .lbm import LatticeModel, Layout, LbmSolver, Params, Storage
+from . import surrogate as sg
+
+
+def make_solver(grid, storage=Storage.F16, layout=Layout.PINGPONG):
+    s = LbmSolver()
+    if not s.Init(None, grid, LatticeModel.D3Q19, storage=storage, layout=layout):
+        raise RuntimeError(s.LastError())
+    return s
+
+
+def shape_stream(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def generate_dataset(n, dims, steps, u_in, tau, cs, seed, storage=Storage.F16, log=print, time_budget_s=None):
+    """Runs n random shapes; returns a list of device samples (solid, sdf, tgt, wmask) and
+    their descriptions.  dims = (nx, ny, nz)."""
+    import torch
+    nx, ny, nz = dims
+    rng = shape_stream(seed)
+    params = Params(tau=tau, inletVelocity=u_in, smagorinskyCs=cs)
+    out, desc = [], []
+    V = nx * ny * nz
+    t0 = time.time()
+    for i in range(n):
+        g, d = geometry.random_shape_grid(nx, ny, nz, rng)
+        s = make_solver(g, storage)
+        s.Reset(params, False)
+        s.Step(None, steps)
+        field = torch.empty(V * 4, dtype=torch.float32, device="cuda")
+        torch.cuda.current_stream().synchronize()
+        s.FieldDevice(field)
+        s.Sync()
+        solid = torch.from_numpy(np.ascontiguousarray(g.solid.reshape(-1))).cuda()
+        sdf = torch.from_numpy(np.ascontiguousarray(g.signedDistance.reshape(-1))).cuda()
+        tgt, wm = sg.targets_from_field(field, solid, u_in, V)
+        st = s.ReadbackStats()
+        s.Shutdown()
+        if st.hasNaN:
+            log(f"  shape {i} diverged; skipped")
+            continue
+        out.append((solid, sdf, tgt, wm))
+        desc.append(d)
+        if (i + 1) % 50 == 0 or i == n - 1:
+            log(f"  dataset {i + 1}/{n} shapes, {time.time() - t0:.1f}s")
+        if time_budget_s is not None and time.time() - t0 > time_budget_s:
+            log(f"  dataset time budget reached at {i + 1} shapes")
+            break
+    return out, desc
+
+
+def batch_tensors(samples, idx, u_in, net, dims):
+    """Concatenates the samples idx into one batch: features (bf16), targets, weights."""
+    import torch
+    solid = torch.cat([samples[i][0] for i in idx])
+    sdf = torch.cat([samples[i][1] for i in idx])
+    tgt = torch.cat([samples[i][2] for i in idx])
+    wm = torch.cat([samples[i][3] for i in idx])
+    nx, ny, nz = dims
+    x = net.features(solid, sdf, u_in, len(idx), (nz, ny, nx))
+    return x, tgt, wm
+
+
+def train(net, samples, dims, u_in, epochs=20, batch=4, lr=1e-3, seed=0, time_budget_s=None, log=print):
+    """Adam on shuffled batches; returns the per-epoch mean loss history."""
+    import torch
+    nx, ny, nz = dims
+    rng = np.random.Generator(np.random.PCG64(seed))
+    hist = []
+    t0 = time.time()
+    for ep in range(epochs):
+        order = rng.permutation(len(samples))
+        losses = []
+        for b in range(0, len(order) - batch + 1, batch):
+            idx = order[b:b + batch]
+            x, tgt, wm = batch_tensors(samples, idx, u_in, net, dims)
+            losses.append(net.train_step(x, tgt, wm, len(idx), (nz, ny, nx), lr=lr))
+        torch.cuda.synchronize()
+        mean = float(torch.stack(losses).mean().cpu())
+        hist.append(mean)
+        log(f"  epoch {ep + 1}: loss {mean:.5f}  ({time.time() - t0:.1f}s)")
+        if time_budget_s is not None and time.time() - t0 > time_budget_s:
+            log("  training time budget reached")
+            break
+    return hist
+
+
+def field_l2(net, samples, dims, u_in, idx):
+    """Relative L2 error of the surrogate's normalised field over fluid cells."""
+    import torch
+    nx, ny, nz = dims
+    errs = []
+    for i in idx:
+        x, tgt, wm = batch_tensors(samples, [i], u_in, net, dims)
+        pred = net.predict(x, 1, (nz, ny, nx))
+        w = wm.repeat_interleave(4)
+        e = ((pred - tgt) ** 2 * w).sum() / ((tgt ** 2 * w).sum() + 1e-30)
+        errs.append(float(torch.sqrt(e).cpu()))
+    return errs
+
+
+def frontal_area(desc, dims):
+    return float(desc["area"])
+
+
+def run_cd_series(solver, steps, every, area, u_in):
+    """Steps the solver, sampling the FP64 force every `every` steps on the device;
+    returns (Cd samples, Cl samples, ms per step)."""
+    import torch
+    n = steps // every
+    buf = torch.empty((n, 4), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n):
+        solver.Step(None, every)
+        solver.ForcesDevice(buf[k])
+    solver.Sync()
+    dt = time.perf_counter() - t0
+    f = buf.cpu().numpy()
+    q = 0.5 * u_in * u_in * area
+    return f[:, 0] / q, f[:, 1] / q, dt * 1e3 / steps
+
+
+def trailing(x, w):
+    c = np.cumsum(np.concatenate([[0.0], x]))
+    out = np.empty_like(x)
+    for i in range(len(x)):
+        lo = max(0, i + 1 - w)
+        out[i] = (c[i + 1] - c[lo]) / (i + 1 - lo)
+    return out
+
+
+def converged_index(cd_avg, ref, tol):
+    """First index after which |cd_avg - ref| <= tol*|ref| holds to the end, or None."""
+    ok = np.abs(cd_avg - ref) <= tol * abs(ref)
+    if not ok[-1]:
+        return None
+    bad = np.nonzero(~ok)[0]
+    return 0 if bad.size == 0 else int(bad[-1]) + 1
+
+
+def coarse_seed(desc, dims, div, steps, params, rng_state=None):
+    """The reference's warm start: the same shape at 1/div resolution, `steps` coarse
+    steps, nearest-sampled onto the fine grid with solid coarse cells -> (1,0,0,0)
+    (FlowCraft/main.cpp:1413-1461, 1496-1564)."""
+    nx, ny, nz = dims
+    cx, cy, cz = max(nx // div, 4), max(ny // div, 1), max(nz // div, 1)
+    sc = cx / nx
+    g = geometry.empty_grid(cx, cy, cz)
+    if desc["kind"] == "naca":
+        geometry.add_naca4_extruded(g, desc["m"], desc["p"], desc["t"], 64.0 * sc, nx * 0.25 * sc, ny / 2.0 * sc,
+                                    desc["aoa"])
+    else:
+        a, b, c = desc["axes"]
+        geometry.add_ellipsoid(g, (nx * 0.25 * sc, ny / 2.0 * sc, nz / 2.0 * sc), (a * sc, b * sc, c * sc))
+    g.solid = (g.signedDistance < 0).astype(np.uint8)
+    s = make_solver(g, Storage.F32)
+    s.Reset(params, False)
+    s.Step(None, steps)
+    cf = s.ReadbackField().reshape(cz, cy, cx, 4)
+    s.Shutdown()
+    # nearest coarse cell of each fine cell centre
+    ix = np.clip(np.floor((np.arange(nx) + 0.5) * cx / nx).astype(int), 0, cx - 1)
+    iy = np.clip(np.floor((np.arange(ny) + 0.5) * cy / ny).astype(int), 0, cy - 1)
+    iz = np.clip(np.floor((np.arange(nz) + 0.5) * cz / nz).astype(int), 0, cz - 1)
+    fine = cf[iz][:, iy][:, :, ix].copy()
+    csolid = g.solid[iz][:, iy][:, :, ix].astype(bool)
+    fine[csolid] = np.array([1.0, 0.0, 0.0, 0.0], np.float32)
+    return fine.reshape(-1)
+
+
+def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, cs, coarse_div=10, coarse_steps=1000,
+             storage=Storage.F16, log=print):
+    """The warm-vs-plain study on n_shapes held-out shapes.  Returns per-shape records."""
+    import torch
+    nx, ny, nz = dims
+    V = nx * ny * nz
+    rng = shape_stream(seed)
+    params = Params(tau=tau, inletVelocity=u_in, smagorinskyCs=cs)
+    w = max(1, window // every)
+    recs = []
+    for i in range(n_shapes):
+        g, d = geometry.random_shape_grid(nx, ny, nz, rng)
+        area = frontal_area(d, dims)
+        rec = {"kind": d["kind"], "area": area}
+        s = make_solver(g, storage)
+        # plain arm
+        s.Reset(params, False)
+        cd, cl, ms = run_cd_series(s, horizon, every, area, u_in)
+        cda = trailing(cd, w)
+        ref = cda[-1]
+        k = converged_index(cda, ref, tol)
+        rec.update(cd_ref=float(ref), cl_ref=float(trailing(cl, w)[-1]), ms_per_step=ms,
+                   plain_steps=None if k is None else (k + 1) * every)
+        # learned arm
+        solid = torch.from_numpy(np.ascontiguousarray(g.solid.reshape(-1))).cuda()
+        sdf = torch.from_numpy(np.ascontiguousarray(g.signedDistance.reshape(-1))).cuda()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = net.features(solid, sdf, u_in, 1, (nz, ny, nx))
+        pred = net.predict(x, 1, (nz, ny, nx))
+        seed_field = sg.field_from_prediction(pred, u_in, V)
+        torch.cuda.synchronize()
+        infer_ms = (time.perf_counter() - t0) * 1e3
+        s.SetParams(params)
+        s.ResetFromField(seed_field)
+        cdw, clw, _ = run_cd_series(s, horizon, every, area, u_in)
+        cdwa = trailing(cdw, w)
+        kw = None if not np.all(np.isfinite(cdw)) else converged_index(cdwa, ref, tol)
+        rec.update(infer_ms=infer_ms, warm_steps=None if kw is None else (kw + 1) * every,
+                   warm_cd_final=float(cdwa[-1]), warm_cl_final=float(trailing(clw, w)[-1]))
+        # reference coarse-grid arm
+        try:
+            t0 = time.perf_counter()
+            cseed = coarse_seed(d, dims, coarse_div, coarse_steps, params)
+            coarse_ms = (time.perf_counter() - t0) * 1e3
+            s.SetParams(params)
+            s.ResetFromField(cseed)
+            cdc, _, _ = run_cd_series(s, horizon, every, area, u_in)
+            kc = None if not np.all(np.isfinite(cdc)) else converged_index(trailing(cdc, w), ref, tol)
+            rec.update(coarse_ms=coarse_ms, coarse_steps=None if kc is None else (kc + 1) * every)
+        except Exception as e:  # the comparison arm must not stop the study
+            rec.update(coarse_error=str(e))
+        s.Shutdown()
+        recs.append(rec)
+        log(f"  eval {i + 1}/{n_shapes} {d['kind']}: Cd_ref {ref:.4f} plain {rec['plain_steps']} "
+            f"warm {rec['warm_steps']} coarse {rec.get('coarse_steps')} (warm Cd {rec['warm_cd_final']:.4f})")
+    return recs
+
+
+def summarize(recs, horizon):
+    """Steps/wall time to convergence (mean, p50, p95) per arm, speedups, fallbacks."""
+    def stats(v):
+        v = np.asarray(v, float)
+        return {"mean": float(v.mean()), "p50": float(np.percentile(v, 50)), "p95": float(np.percentile(v, 95))}
+
+    plain_s, warm_s, plain_t, warm_t, coarse_s = [], [], [], [], []
+    fallbacks = 0
+    unconverged_plain = 0
+    for r in recs:
+        ps = r["plain_steps"]
+        if ps is None:
+            unconverged_plain += 1
+            continue
+        ws = r["warm_steps"]
+        ms = r["ms_per_step"]
+        if ws is None:
+            fallbacks += 1
+            ws_eff, wt = ps, ps * ms + r["infer_ms"]
+        else:
+            ws_eff, wt = ws, ws * ms + r["infer_ms"]
+        plain_s.append(ps)
+        warm_s.append(ws_eff)
+        plain_t.append(ps * ms)
+        warm_t.append(wt)
+        if r.get("coarse_steps") is not None:
+            coarse_s.append(r["coarse_steps"])
+    out = {"shapes": len(recs), "plain_unconverged_at_horizon": unconverged_plain, "fallbacks": fallbacks,
+           "horizon_steps": horizon}
+    if plain_s:
+        sp = np.asarray(plain_t) / np.asarray(warm_t)
+        out.update(plain_steps=stats(plain_s), warm_steps=stats(warm_s), plain_ms=stats(plain_t),
+                   warm_ms=stats(warm_t), speedup_wall=stats(sp),
+                   speedup_wall_of_means=float(np.mean(plain_t) / np.mean(warm_t)))
+    if coarse_s:
+        out["coarse_steps"] = stats(coarse_s)
+    dcd = [abs(r["warm_cd_final"] - r["cd_ref"]) / abs(r["cd_ref"]) for r in recs if r["cd_ref"] != 0]
+    if dcd:
+        out["final_cd_rel_diff_warm_vs_plain"] = stats(dcd)
+    return out

@@ -26,7 +26,10 @@
 //   ReadbackField ............. FlowCraft/LbmReference.cpp:475-500
 //   ComputeStats .............. FlowCraft/LbmReference.cpp:502-585
 // Extensions with no reference counterpart (pinned only by the FP64 build and
-// the invariants in tests/): periodic y/z wrap, FP16/BF16 offset storage.
+// the invariants in tests/): periodic y/z wrap, FP16/BF16 offset storage.  The
+// offset formats run the engine's offset-space form of the same collision
+// (StepOffset below), operation for operation, on the stored offsets
+// g = (f - w) * S; their state arrays hold g, and readbacks decode f = w + g / S.
 //
 // Exposed through a small extern "C" API (orc_*) for ctypes.
 
@@ -243,16 +246,136 @@ struct Oracle {
         for (auto& x : th) x.join();
     }
 
-    // Quantise a post-collision value through the emulated storage format.
-    Real Store(Real v, int d) const {
+    // Offset formats: the stored element decoded to g (exact), and f = w + g / S.
+    float Scale() const { return storage == 1 ? kF16Scale : 1.0f; }
+    float EncDec(float g) const { return storage == 1 ? F16ToF32(F32ToF16(g)) : BF16ToF32(F32ToBF16(g)); }
+    Real ToF(Real v, int d) const {
+        if (storage == 0) return v;
+        return storage == 1 ? static_cast<Real>(wf[d] + static_cast<float>(v) * kF16InvScale)
+                            : static_cast<Real>(wf[d] + static_cast<float>(v));
+    }
+    // f -> stored state value (the engine's Store: g = (f - w) * S, encoded).
+    Real FromF(Real v, int d) const {
         if (storage == 0) return v;
         float vf = static_cast<float>(v);
-        if (storage == 1) {
-            float g = (vf - wf[d]) * kF16Scale;
-            return static_cast<Real>(wf[d] + F16ToF32(F32ToF16(g)) * kF16InvScale);
+        return static_cast<Real>(storage == 1 ? EncDec((vf - wf[d]) * kF16Scale) : EncDec(vf - wf[d]));
+    }
+    // The state as populations f (decoded for the offset formats).
+    const std::vector<Real>& FState(std::vector<Real>& tmp) const {
+        if (storage == 0) return f[cur];
+        tmp.resize(f[cur].size());
+        for (int d = 0; d < Q; ++d)
+            for (size_t c = 0; c < N; ++c) tmp[Pop(d, c)] = ToF(f[cur][Pop(d, c)], d);
+        return tmp;
+    }
+
+    // ---- offset-space collision (engine: lbm_kernels.hpp ReconstructOffset/CollideOffset)
+    void ReconstructOffset(float drho, float jx, float jy, float jz, float txx, float tyy, float tzz, float txy,
+                           float txz, float tyz, float* go) const {
+        const float c3 = 1.0f / 3.0f;
+        const float tr3 = (txx + tyy + tzz) * c3;
+        const float j3[3] = {3.0f * jx, 3.0f * jy, 3.0f * jz};
+        const float t2xy = 2.0f * txy, t2xz = 2.0f * txz, t2yz = 2.0f * tyz;
+        const float tdiag[3] = {txx, tyy, tzz};
+        for (int d = 0; d < Q; ++d) {
+            const int c[3] = {ex[d], ey[d], ez[d]};
+            const bool rest = c[0] == 0 && c[1] == 0 && c[2] == 0;
+            float qt;
+            if (rest) {
+                qt = -tr3;
+            } else {
+                bool have = false;
+                float diag = 0.0f;
+                for (int a = 0; a < 3; ++a)
+                    if (c[a] != 0) {
+                        diag = have ? diag + tdiag[a] : tdiag[a];
+                        have = true;
+                    }
+                qt = diag - tr3;
+            }
+            if (c[0] * c[1] == 1) qt += t2xy;
+            if (c[0] * c[1] == -1) qt -= t2xy;
+            if (c[0] * c[2] == 1) qt += t2xz;
+            if (c[0] * c[2] == -1) qt -= t2xz;
+            if (c[1] * c[2] == 1) qt += t2yz;
+            if (c[1] * c[2] == -1) qt -= t2yz;
+            float base = drho;
+            if (!rest) {
+                bool have = false;
+                float dot = 0.0f;
+                for (int a = 0; a < 3; ++a)
+                    if (c[a] != 0) {
+                        const float term = c[a] == 1 ? j3[a] : -j3[a];
+                        dot = have ? dot + term : term;
+                        have = true;
+                    }
+                base = drho + dot;
+            }
+            const float ws = wf[d] * Scale();
+            go[d] = ws * (base + 4.5f * qt);
         }
-        float g = vf - wf[d];
-        return static_cast<Real>(wf[d] + BF16ToF32(F32ToBF16(g)));
+    }
+
+    void EqOffset(float drho, float ux, float uy, float uz, float* go) const {
+        const float rho = 1.0f + drho;
+        const float jx = rho * ux, jy = rho * uy, jz = rho * uz;
+        ReconstructOffset(drho, jx, jy, jz, jx * ux, jy * uy, jz * uz, jx * uy, jx * uz, jy * uz, go);
+    }
+
+    // Sums of stored values in index order, zero-coefficient terms skipped.
+    void RawSums(const float* g, float& sg, float& Jx, float& Jy, float& Jz) const {
+        sg = Jx = Jy = Jz = 0.0f;
+        for (int d = 0; d < Q; ++d) {
+            sg += g[d];
+            if (ex[d] == 1) Jx += g[d];
+            if (ex[d] == -1) Jx -= g[d];
+            if (ey[d] == 1) Jy += g[d];
+            if (ey[d] == -1) Jy -= g[d];
+            if (ez[d] == 1) Jz += g[d];
+            if (ez[d] == -1) Jz -= g[d];
+        }
+    }
+
+    bool CollideOffset(const float* g, float* go) const {
+        const float invS = 1.0f / Scale();
+        float sg, Jx, Jy, Jz;
+        RawSums(g, sg, Jx, Jy, Jz);
+        float gxx = 0, gyy = 0, gzz = 0, gxy = 0, gxz = 0, gyz = 0;
+        for (int d = 0; d < Q; ++d) {
+            if (ex[d] != 0) gxx += g[d];
+            if (ey[d] != 0) gyy += g[d];
+            if (ez[d] != 0) gzz += g[d];
+            const int sxy = ex[d] * ey[d], sxz = ex[d] * ez[d], syz = ey[d] * ez[d];
+            if (sxy == 1) gxy += g[d];
+            if (sxy == -1) gxy -= g[d];
+            if (sxz == 1) gxz += g[d];
+            if (sxz == -1) gxz -= g[d];
+            if (syz == 1) gyz += g[d];
+            if (syz == -1) gyz -= g[d];
+        }
+        const float drho = sg * invS;
+        const float jx = Jx * invS, jy = Jy * invS, jz = Jz * invS;
+        const float rho = 1.0f + drho;
+        const float r = rho > 1e-6f ? 1.0f / rho : 1.0f;
+        const float ux = jx * r, uy = jy * r, uz = jz * r;
+        const bool bad = !std::isfinite(rho) || !std::isfinite(ux) || !std::isfinite(uy) || !std::isfinite(uz) ||
+                         rho <= 0.0f || (ux * ux + uy * uy + uz * uz) > 4.0f;
+        const float c3 = 1.0f / 3.0f;
+        const float pxx = (gxx * invS - jx * ux) - drho * c3;
+        const float pyy = (gyy * invS - jy * uy) - drho * c3;
+        const float pzz = (gzz * invS - jz * uz) - drho * c3;
+        const float pxy = gxy * invS - jx * uy;
+        const float pxz = gxz * invS - jx * uz;
+        const float pyz = gyz * invS - jy * uz;
+        const float piMag = std::sqrt(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+        const float rmax = rho < 1e-6f ? 1e-6f : rho;
+        const float kSmag = 18.0f * 1.41421356f;
+        const float ftau = tau, fcs = cs;
+        const float tauEff = 0.5f * (ftau + std::sqrt(ftau * ftau + kSmag * fcs * fcs * piMag / rmax));
+        const float keep = 1.0f - 1.0f / tauEff;
+        ReconstructOffset(drho, jx, jy, jz, jx * ux + keep * pxx, jy * uy + keep * pyy, jz * uz + keep * pzz,
+                          jx * uy + keep * pxy, jx * uz + keep * pxz, jy * uz + keep * pyz, go);
+        return bad;
     }
 
     // LbmReference.cpp:9-13, same association: w*rho*(((1+3eu)+(4.5eu)eu)-1.5u2)
@@ -322,9 +445,20 @@ struct Oracle {
         stress = wsum > Real(0) ? mag / (Real(6) * wsum) : Real(0);
     }
 
+    void SeedValues(Real rho, Real ux, Real uy, Real uz, Real* out) const {
+        if (storage == 0) {
+            for (int i = 0; i < Q; ++i) out[i] = Feq(i, rho, ux, uy, uz);
+            return;
+        }
+        float go[19];
+        EqOffset(static_cast<float>(rho) - 1.0f, static_cast<float>(ux), static_cast<float>(uy),
+                 static_cast<float>(uz), go);
+        for (int i = 0; i < Q; ++i) out[i] = static_cast<Real>(EncDec(go[i]));
+    }
+
     void FillUniform(Real rho, Real ux, Real uy, Real uz) {
         std::vector<Real> fe(Q);
-        for (int i = 0; i < Q; ++i) fe[i] = Store(Feq(i, rho, ux, uy, uz), i);
+        SeedValues(rho, ux, uy, uz, fe.data());
         for (int i = 0; i < Q; ++i) std::fill(f[0].begin() + Pop(i, 0), f[0].begin() + Pop(i, 0) + N, fe[i]);
         cur = 0;
         diverged = false;
@@ -352,7 +486,9 @@ struct Oracle {
                 Real ux = solid[c] ? Real(0) : static_cast<Real>(field[c * 4 + 1]);
                 Real uy = solid[c] ? Real(0) : static_cast<Real>(field[c * 4 + 2]);
                 Real uz = solid[c] ? Real(0) : static_cast<Real>(field[c * 4 + 3]);
-                for (int i = 0; i < Q; ++i) f[0][Pop(i, c)] = Store(Feq(i, rho, ux, uy, uz), i);
+                Real v[19];
+                SeedValues(rho, ux, uy, uz, v);
+                for (int i = 0; i < Q; ++i) f[0][Pop(i, c)] = v[i];
             }
         });
         cur = 0;
@@ -360,6 +496,10 @@ struct Oracle {
 
     // LbmReference.cpp:241-469
     void Step() {
+        if (storage != 0) {
+            StepOffset();
+            return;
+        }
         const std::vector<Real>& fin = f[cur];
         std::vector<Real>& fout = f[1 - cur];
         const int inletX = reverseFlow ? nx - 1 : 0;
@@ -382,7 +522,7 @@ struct Oracle {
                         }
                         if (x == inletX) {
                             for (int i = 0; i < Q; ++i)
-                                fout[Pop(i, c)] = Store(Feq(i, Real(1), inletUx, Real(0), Real(0)), i);
+                                fout[Pop(i, c)] = Feq(i, Real(1), inletUx, Real(0), Real(0));
                             continue;
                         }
                         if (x == outletX) {
@@ -391,7 +531,7 @@ struct Oracle {
                             Moments(fin, Cell(nbx, y, z), nr, nu, nv, nw);
                             Real rho = nr > K<Real>(1e-6f, 1e-6) ? Real(1) + K<Real>(0.999f, 0.999) * (nr - Real(1))
                                                                  : Real(1);
-                            for (int i = 0; i < Q; ++i) fout[Pop(i, c)] = Store(Feq(i, rho, nu, nv, nw), i);
+                            for (int i = 0; i < Q; ++i) fout[Pop(i, c)] = Feq(i, rho, nu, nv, nw);
                             continue;
                         }
                         for (int i = 0; i < Q; ++i) {
@@ -459,8 +599,79 @@ struct Oracle {
                             Real qxy = fx * fy, qxz = fx * fz, qyz = fy * fz;
                             Real reg = (w[i] * Real(4.5)) * (qxx * pxx + qyy * pyy + qzz * pzz + Real(2) * qxy * pxy +
                                                              Real(2) * qxz * pxz + Real(2) * qyz * pyz);
-                            fout[Pop(i, c)] = Store(fpre[i] + (Real(1) - omega) * reg, i);
+                            fout[Pop(i, c)] = fpre[i] + (Real(1) - omega) * reg;
                         }
+                    }
+            if (localDiv) diverged = true;
+        });
+        cur = 1 - cur;
+    }
+
+    // The offset formats' step: the same gather and boundary rules as Step() applied to
+    // the stored offsets g, then CollideOffset / EqOffset, then encode-decode.
+    void StepOffset() {
+        const std::vector<Real>& fin = f[cur];
+        std::vector<Real>& fout = f[1 - cur];
+        const int inletX = reverseFlow ? nx - 1 : 0;
+        const int outletX = reverseFlow ? 0 : nx - 1;
+        const float inletUx = reverseFlow ? -inletVelocity : inletVelocity;
+        const float invS = 1.0f / Scale();
+        ParallelForZ([&](int z0, int z1, int) {
+            bool localDiv = false;
+            float g[19], go[19];
+            for (int z = z0; z < z1; ++z)
+                for (int y = 0; y < ny; ++y)
+                    for (int x = 0; x < nx; ++x) {
+                        size_t c = Cell(x, y, z);
+                        if (solid[c]) {
+                            for (int i = 0; i < Q; ++i) fout[Pop(i, c)] = Real(0);
+                            continue;
+                        }
+                        if (x == inletX) {
+                            EqOffset(0.0f, inletUx, 0.0f, 0.0f, go);
+                        } else if (x == outletX) {
+                            const int nbx = reverseFlow ? x + 1 : x - 1;
+                            const size_t nb = Cell(nbx, y, z);
+                            for (int i = 0; i < Q; ++i) g[i] = static_cast<float>(fin[Pop(i, nb)]);
+                            float sg, Jx, Jy, Jz;
+                            RawSums(g, sg, Jx, Jy, Jz);
+                            const float dn = sg * invS;
+                            const float rn = 1.0f + dn;
+                            float ux = Jx * invS, uy = Jy * invS, uz = Jz * invS, drho = 0.0f;
+                            if (rn > 1e-6f) {
+                                const float r = 1.0f / rn;
+                                ux *= r;
+                                uy *= r;
+                                uz *= r;
+                                drho = 0.999f * dn;
+                            }
+                            EqOffset(drho, ux, uy, uz, go);
+                        } else {
+                            for (int i = 0; i < Q; ++i) {
+                                int sx = x - ex[i], sy = y - ey[i], sz = z - ez[i];
+                                int rx = sx, ry = sy, rz = sz;
+                                bool inb = Resolve(rx, ry, rz);
+                                if (inb && !solid[Cell(rx, ry, rz)]) {
+                                    g[i] = static_cast<float>(fin[Pop(i, Cell(rx, ry, rz))]);
+                                } else if (inb) {
+                                    g[i] = static_cast<float>(Bouzidi(fin, x, y, z, c, i, rx, ry, rz));
+                                } else {
+                                    bool outY = !periodicY && (sy < 0 || sy >= ny);
+                                    bool outZ = !periodicZ && (sz < 0 || sz >= nz);
+                                    bool ok = (!outY || freeSlipY) && (!outZ || freeSlipZ);
+                                    if ((outY || outZ) && ok) {
+                                        int d = i;
+                                        if (outY) d = mirrorY[d];
+                                        if (outZ) d = mirrorZ[d];
+                                        g[i] = static_cast<float>(fin[Pop(d, c)]);
+                                    } else {
+                                        g[i] = static_cast<float>(fin[Pop(opp[i], c)]);
+                                    }
+                                }
+                            }
+                            if (CollideOffset(g, go)) localDiv = true;
+                        }
+                        for (int i = 0; i < Q; ++i) fout[Pop(i, c)] = static_cast<Real>(EncDec(go[i]));
                     }
             if (localDiv) diverged = true;
         });
@@ -470,7 +681,8 @@ struct Oracle {
     // LbmReference.cpp:475-500
     template <typename Out>
     void ReadbackField(Out* out) const {
-        const std::vector<Real>& fc = f[cur];
+        std::vector<Real> tmp;
+        const std::vector<Real>& fc = FState(tmp);
         ParallelForZ([&](int z0, int z1, int) {
             for (int z = z0; z < z1; ++z)
                 for (int y = 0; y < ny; ++y)
@@ -501,7 +713,8 @@ struct Oracle {
             bool nan = false;
         };
         std::vector<Part> parts(nz);
-        const std::vector<Real>& fc = f[cur];
+        std::vector<Real> tmp;
+        const std::vector<Real>& fc = FState(tmp);
         ParallelForZ([&](int z0, int z1, int) {
             for (int z = z0; z < z1; ++z) {
                 Part& p = parts[z];
@@ -673,7 +886,9 @@ int orc_has_diverged(void* p) {
 // Current post-collision populations f[d*N + cell] as double.
 void orc_get_pops(void* p, double* out) {
     Dispatch(static_cast<Handle*>(p), [&](auto& o) {
-        const auto& fc = o.f[o.cur];
+        using R = typename std::remove_reference<decltype(o.f[0][0])>::type;
+        std::vector<R> tmp;
+        const auto& fc = o.FState(tmp);
         for (size_t k = 0; k < fc.size(); ++k) out[k] = static_cast<double>(fc[k]);
     });
 }
@@ -685,7 +900,7 @@ void orc_set_pops(void* p, const double* in) {
         for (int d = 0; d < o.Q; ++d)
             for (size_t c = 0; c < o.N; ++c) {
                 using R = typename std::remove_reference<decltype(fc[0])>::type;
-                fc[o.Pop(d, c)] = o.Store(static_cast<R>(in[o.Pop(d, c)]), d);
+                fc[o.Pop(d, c)] = o.FromF(static_cast<R>(in[o.Pop(d, c)]), d);
             }
     });
 }

@@ -34,14 +34,29 @@ def bf(x):
     return t.tensor(np.asarray(x, np.float32)).to(t.bfloat16).to(t.float32).numpy().astype(np.float64)
 
 
+_KEEP = []  # device inputs stay referenced until the test ends: a freed temporary's block
+           # could otherwise be refilled by the next upload before the kernel that reads it runs
+
+
+@pytest.fixture(autouse=True)
+def _keep_alive():
+    yield
+    torch_mod().cuda.synchronize()
+    _KEEP.clear()
+
+
 def dev_bf16(x):
     t = torch_mod()
-    return t.tensor(np.asarray(x, np.float32).reshape(-1)).to(t.bfloat16).cuda()
+    d = t.tensor(np.asarray(x, np.float32).reshape(-1)).to(t.bfloat16).cuda()
+    _KEEP.append(d)
+    return d
 
 
 def dev_f32(x):
     t = torch_mod()
-    return t.tensor(np.asarray(x, np.float32).reshape(-1)).cuda()
+    d = t.tensor(np.asarray(x, np.float32).reshape(-1)).cuda()
+    _KEEP.append(d)
+    return d
 
 
 def ptr(t):
