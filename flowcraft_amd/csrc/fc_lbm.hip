@@ -55,6 +55,7 @@ struct fc_lbm {
 namespace {
 
 const char* kNoHandle = "null fc_lbm handle";
+constexpr size_t kPopMargin = 256;
 
 void DevFree(void* p) {
     if (p) (void)hipFree(p);
@@ -271,7 +272,9 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
         FC_HIP(h, hipStreamCreate(&h->stream));
         h->own_stream = true;
     }
-    const size_t popbytes = static_cast<size_t>(h->Q) * h->P * h->tbytes;
+    // kPopMargin elements past the last plane: the step's speculative bulk pull of a
+    // cell at the far corner may address up to one element beyond it.
+    const size_t popbytes = (static_cast<size_t>(h->Q) * h->P + kPopMargin) * h->tbytes;
     const int copies = h->layout == FC_LBM_LAYOUT_INPLACE ? 1 : 2;
     for (int i = 0; i < copies; ++i) {
         if (hipMalloc(&h->pops[i], popbytes) != hipSuccess) {
@@ -659,7 +662,7 @@ long long fc_lbm_step_count(const fc_lbm* h) { return h ? h->t : -1; }
 
 size_t fc_lbm_population_bytes(const fc_lbm* h) {
     if (!h) return 0;
-    return static_cast<size_t>(h->Q) * h->P * h->tbytes * (h->layout == FC_LBM_LAYOUT_INPLACE ? 1 : 2);
+    return (static_cast<size_t>(h->Q) * h->P + kPopMargin) * h->tbytes * (h->layout == FC_LBM_LAYOUT_INPLACE ? 1 : 2);
 }
 
 void* fc_lbm_stream(const fc_lbm* h) { return h ? static_cast<void*>(h->stream) : nullptr; }

@@ -76,17 +76,19 @@ struct StoreF32 {
     static __device__ __forceinline__ T Enc(R v) { return v; }
 };
 
-// FP16 offset: g = (f - w) * 4096 as IEEE binary16 (RNE); arithmetic in FP32.
+// FP16 offset: g = (f - w) * 4096 as IEEE binary16 (RNE); arithmetic in FP32.  Offsets
+// below the smallest normal half (|g| < 2^-14, i.e. |f - w| < 1.5e-8) are stored as 0,
+// so the format never depends on the device's half-precision denormal mode.
 template <int Q>
 struct StoreF16 {
     using T = _Float16;
     using R = float;
     static constexpr bool kOffset = true;
     static constexpr float kScale = 4096.0f;
-    static __device__ __forceinline__ R Load(T v, int d) { return Lat<Q>::w[d] + static_cast<float>(v) * (1.0f / 4096.0f); }
-    static __device__ __forceinline__ T Store(R f, int d) { return static_cast<T>((f - Lat<Q>::w[d]) * 4096.0f); }
     static __device__ __forceinline__ R Raw(T v) { return static_cast<float>(v); }
-    static __device__ __forceinline__ T Enc(R g) { return static_cast<T>(g); }
+    static __device__ __forceinline__ T Enc(R g) { return static_cast<T>(fabsf(g) < 6.103515625e-05f ? 0.0f : g); }
+    static __device__ __forceinline__ R Load(T v, int d) { return Lat<Q>::w[d] + Raw(v) * (1.0f / 4096.0f); }
+    static __device__ __forceinline__ T Store(R f, int d) { return Enc((f - Lat<Q>::w[d]) * 4096.0f); }
 };
 
 // BF16 offset: g = f - w as bfloat16 (RNE, NaN kept quiet); arithmetic in FP32.
@@ -553,6 +555,23 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     const int mom_col = x == 1 ? 0 : 1;
     const bool mom_writer = IP && (x == 1 || x == a.nx - 2);
 
+    // Bulk pull, issued for every cell before its flag byte arrives (the two loads do
+    // not wait on each other).  Addresses stay inside the padded allocation for every
+    // cell of the lattice; values for cells that take another branch are discarded.
+    R fs[Q];
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        if constexpr (!IP) {
+            const int off = Lat<Q>::ex[d] + Lat<Q>::ey[d] * g.sy + Lat<Q>::ez[d] * g.sz;
+            fs[d] = S::Raw(src[d * P + (self - off)]);
+        } else {
+            constexpr int pa = PairAnchor<Q>(d);
+            const int anc = self + Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
+            int pl, c;
+            LoadSlot(d, par, self, anc, pl, c);
+            fs[d] = S::Raw(dst[pl * P + c]);
+        }
+    });
     const uint8_t fl = a.flags[self];
     R fo[Q];
 
@@ -636,24 +655,15 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
         }
     } else {
         // ---------------------------------------------------------- gather
+        // f starts as the bulk pull (issued before the flag byte is known, see fs below);
+        // cells on a wall or next to a solid then correct the affected directions.
         R f[Q];
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            f[d] = fs[d];
+        });
         const bool edge = (y == 0) | (y == a.ny - 1) | (z == 0) | (z == a.nz - 1);
-        if (!(edge | near)) {
-            // Bulk fast path: every source is an in-domain fluid cell, no wrap.
-            StaticFor<0, Q>([&](auto I) {
-                constexpr int d = I;
-                if constexpr (!IP) {
-                    const int off = Lat<Q>::ex[d] + Lat<Q>::ey[d] * g.sy + Lat<Q>::ez[d] * g.sz;
-                    f[d] = S::Raw(src[d * P + (self - off)]);
-                } else {
-                    constexpr int pa = PairAnchor<Q>(d);
-                    const int anc = self + Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
-                    int pl, c;
-                    LoadSlot(d, par, self, anc, pl, c);
-                    f[d] = S::Raw(dst[pl * P + c]);
-                }
-            });
-        } else {
+        if (edge | near) {
             uint32_t kinds[3] = {0u, 0u, 0u};
             if (near) {
                 rec = static_cast<BRec<R>*>(a.recs) + a.bidx[self];
@@ -661,28 +671,26 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
                 kinds[1] = rec->kind[1];
                 kinds[2] = rec->kind[2];
             }
-            // Phase 1: ordinary streaming for every direction whose source is an
-            // in-domain fluid cell (periodic wrap included).
-            StaticFor<0, Q>([&](auto I) {
-                constexpr int d = I;
-                const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
-                const bool outY = !a.pery && (sy_ < 0 || sy_ >= a.ny);
-                const bool outZ = !a.perz && (sz_ < 0 || sz_ >= a.nz);
-                const unsigned kd = KindOf(kinds, d);
-                f[d] = R(0);
-                if (d == 0 || (!outY && !outZ && kd == kPull)) {
+            // Phase 1: pulls across a periodic y/z seam read the wrapped cell.
+            if (a.pery | a.perz) {
+                StaticFor<1, Q>([&](auto I) {
+                    constexpr int d = I;
+                    const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
+                    const bool wrapY = a.pery && (sy_ < 0 || sy_ >= a.ny);
+                    const bool wrapZ = a.perz && (sz_ < 0 || sz_ >= a.nz);
+                    if (!(wrapY | wrapZ)) return;
                     if constexpr (!IP) {
                         const int sidx = g.Nbr(x, y, z, -Lat<Q>::ex[d], -Lat<Q>::ey[d], -Lat<Q>::ez[d]);
                         f[d] = S::Raw(src[d * P + sidx]);
                     } else {
                         constexpr int pa = PairAnchor<Q>(d);
-                        const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
+                        const int anc = g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
                         int pl, c;
                         LoadSlot(d, par, self, anc, pl, c);
                         f[d] = S::Raw(dst[pl * P + c]);
                     }
-                }
-            });
+                });
+            }
             // Phase 2: walls and solid sources, from the cell's own previous
             // outgoing populations (LbmReference.cpp:100-122, 336-381).
             auto own_prev = [&](int d) -> R {

@@ -248,7 +248,11 @@ struct Oracle {
 
     // Offset formats: the stored element decoded to g (exact), and f = w + g / S.
     float Scale() const { return storage == 1 ? kF16Scale : 1.0f; }
-    float EncDec(float g) const { return storage == 1 ? F16ToF32(F32ToF16(g)) : BF16ToF32(F32ToBF16(g)); }
+    // FP16: offsets below the smallest normal half (2^-14) are stored as 0.
+    float EncDec(float g) const {
+        if (storage == 1) return std::fabs(g) < 6.103515625e-05f ? 0.0f : F16ToF32(F32ToF16(g));
+        return BF16ToF32(F32ToBF16(g));
+    }
     Real ToF(Real v, int d) const {
         if (storage == 0) return v;
         return storage == 1 ? static_cast<Real>(wf[d] + static_cast<float>(v) * kF16InvScale)

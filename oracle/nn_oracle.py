@@ -17,6 +17,23 @@ import numpy as np
 LEAK = 0.05
 
 
+def bf16(x):
+    """Round float values to bfloat16 (round-to-nearest-even), returned as float64."""
+    a = np.asarray(x, np.float64).astype(np.float32)
+    b = a.view(np.uint32).astype(np.uint64)
+    r = (b + 0x7FFF + ((b >> 16) & 1)) >> 16
+    out = (r << 16).astype(np.uint32).view(np.float32)
+    return out.astype(np.float64)
+
+
+def f32(x):
+    return np.asarray(x, np.float64).astype(np.float32).astype(np.float64)
+
+
+def _ident(x):
+    return x
+
+
 def taps():
     for t in range(27):
         yield t, t // 9, (t // 3) % 3, t % 3
@@ -76,16 +93,19 @@ def leaky_grad(y, g):
     return np.where(y > 0, g, LEAK * g)
 
 
-def forward(params, x, layers, head_w, head_b):
-    """params: {name: (w, b)}; x: [n, D, H, W, 8].  Returns (acts, pred)."""
+def forward(params, x, layers, head_w, head_b, quant=False):
+    """params: {name: (w, b)}; x: [n, D, H, W, 8].  Returns (acts, pred).
+    quant=True rounds every activation to bf16 where the GPU stores one (the weights
+    passed in are expected to be bf16 values already)."""
+    q = bf16 if quant else _ident
     acts = {"x": x}
     for name, mode, ins, kin, c1, cout, stride, lvl in layers:
         w, b = params[name]
         if mode == 0:
-            acts[name] = leaky(conv_fwd(acts[ins[0]], w, b, stride))
+            acts[name] = q(leaky(conv_fwd(acts[ins[0]], w, b, stride)))
         else:
             cat = np.concatenate([up2(acts[ins[0]]), acts[ins[1]]], axis=-1)
-            acts[name] = leaky(conv_fwd(cat, w, b, 1))
+            acts[name] = q(leaky(conv_fwd(cat, w, b, 1)))
     pred = np.einsum("ndhwc,oc->ndhwo", acts["d0"], head_w) + head_b
     return acts, pred
 
@@ -96,27 +116,34 @@ def loss_fn(pred, tgt, wmask):
     return float((wmask[..., None] * e * e).sum() / (4.0 * V))
 
 
-def backward(params, acts, pred, tgt, wmask, layers, head_w):
-    """Gradients of loss_fn(forward(...)) for every parameter."""
+def backward(params, acts, pred, tgt, wmask, layers, head_w, quant=False):
+    """Gradients of loss_fn(forward(...)) for every parameter.  quant=True rounds each
+    activation gradient to bf16 where the GPU backward stores one: the head's dh, dz
+    after the leaky-ReLU derivative, each data gradient, each accumulation, and the
+    two halves of a concatenated input's gradient."""
+    q = bf16 if quant else _ident
     V = pred.shape[0] * pred.shape[1] * pred.shape[2] * pred.shape[3]
     g = 2.0 * wmask[..., None] * (pred - tgt) / (4.0 * V)
     grads = {"head.w": np.einsum("ndhwo,ndhwc->oc", g, acts["d0"]), "head.b": g.sum(axis=(0, 1, 2, 3))}
-    dact = {"d0": np.einsum("ndhwo,oc->ndhwc", g, head_w)}
+    dact = {"d0": q(np.einsum("ndhwo,oc->ndhwc", g, head_w))}
+
+    def accumulate(key, v):
+        dact[key] = q(dact[key] + v) if key in dact else v
+
     for name, mode, ins, kin, c1, cout, stride, lvl in reversed(layers):
         w, b = params[name]
-        dz = leaky_grad(acts[name], dact[name])
+        dz = q(leaky_grad(acts[name], dact[name]))
         grads[name + ".b"] = dz.sum(axis=(0, 1, 2, 3))
         if mode == 0:
             src = acts[ins[0]]
             grads[name + ".w"] = conv_wgrad(src, dz, stride)
             if name != "e0":
-                dx = conv_adj(dz, w, src.shape, stride)
-                dact[ins[0]] = dact.get(ins[0], 0) + dx
+                accumulate(ins[0], q(conv_adj(dz, w, src.shape, stride)))
         else:
             coarse, skip = acts[ins[0]], acts[ins[1]]
             cat = np.concatenate([up2(coarse), skip], axis=-1)
             grads[name + ".w"] = conv_wgrad(cat, dz, 1)
-            dcat = conv_adj(dz, w, cat.shape, 1)
-            dact[ins[1]] = dact.get(ins[1], 0) + dcat[..., c1:]
-            dact[ins[0]] = dact.get(ins[0], 0) + down2_sum(dcat[..., :c1])
+            dcat = q(conv_adj(dz, w, cat.shape, 1))
+            dact[ins[1]] = q(dact.get(ins[1], 0) + dcat[..., c1:])
+            accumulate(ins[0], q(down2_sum(dcat[..., :c1])))
     return grads

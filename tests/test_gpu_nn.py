@@ -194,13 +194,15 @@ def test_head_loss_colsum_adam():
         ve = 0.999 * ve + 0.001 * g1.astype(np.float64) ** 2
         pe = pe - 1e-3 * (me / (1 - 0.9 ** k)) / (np.sqrt(ve / (1 - 0.999 ** k)) + 1e-8)
     assert np.abs(p.cpu().numpy() - pe).max() < 1e-6
-    assert np.array_equal(pb.cpu(), p.cpu().to(t.bfloat16))
+    assert t.equal(pb.cpu(), p.cpu().to(t.bfloat16))
 
 
 def test_network_gradients_match_fp64_oracle():
-    """Production layer list at a small resolution (nb=2, 16x8x8): every parameter
-    gradient of the GPU backward within 5% relative norm of the FP64 oracle's, and the
-    loss within 1%."""
+    """Production layer list at a small resolution (nb=2, 16x8x8).  Against the FP64
+    oracle with the GPU's bf16 rounding points emulated (quant=True): every parameter
+    gradient within 2e-2 relative norm (what remains is fp32-vs-fp64 accumulation order
+    flipping an occasional bf16 rounding); against the unquantised FP64 oracle, within
+    10% (the cost of bf16 activations and activation gradients), loss within 1%."""
     from flowcraft_amd.surrogate import LAYERS, HEAD_IN, Surrogate
     t = torch_mod()
     net = Surrogate(seed=3)
@@ -211,9 +213,12 @@ def test_network_gradients_match_fp64_oracle():
     wm = (rng.random((nb, *dims)) > 0.2).astype(np.float64)
     sd = net.state_dict()
     params = {name: (bf(sd[f"{name}.w"]), sd[f"{name}.b"].astype(np.float64)) for name, *_ in LAYERS}
-    acts, pred = no.forward(params, x, LAYERS, sd["head.w"].astype(np.float64), sd["head.b"].astype(np.float64))
+    hw, hb = sd["head.w"].astype(np.float64), sd["head.b"].astype(np.float64)
+    acts, pred = no.forward(params, x, LAYERS, hw, hb)
     ref_loss = no.loss_fn(pred, tgt, wm)
-    ref = no.backward(params, acts, pred, tgt, wm, LAYERS, sd["head.w"].astype(np.float64))
+    exact = no.backward(params, acts, pred, tgt, wm, LAYERS, hw)
+    qacts, qpred = no.forward(params, x, LAYERS, hw, hb, quant=True)
+    ref = no.backward(params, qacts, qpred, tgt, wm, LAYERS, hw, quant=True)
     xa = net.forward(dev_bf16(x), nb, dims)
     loss, dh = net.head(xa, nb, dims, tgt=dev_f32(tgt), wmask=dev_f32(wm), want_grad=True)
     net.backward(xa, dh, nb, dims)
@@ -222,9 +227,13 @@ def test_network_gradients_match_fp64_oracle():
     for name, *_ in LAYERS:
         p = net.params[name]
         for kind, got in (("w", p["gw"]), ("b", p["gb"])):
+            gv = got.cpu().numpy().reshape(ref[f"{name}.{kind}"].shape)
             r = ref[f"{name}.{kind}"]
-            e = np.linalg.norm(got.cpu().numpy().reshape(r.shape) - r) / max(np.linalg.norm(r), 1e-30)
-            print(name, kind, f"{e:.3e}")
-            assert e < 5e-2, (name, kind, e)
+            e = np.linalg.norm(gv - r) / max(np.linalg.norm(r), 1e-30)
+            ex = exact[f"{name}.{kind}"]
+            e2 = np.linalg.norm(gv - ex) / max(np.linalg.norm(ex), 1e-30)
+            print(name, kind, f"vs emulated {e:.3e}  vs exact {e2:.3e}")
+            assert e < 2e-2, (name, kind, e)
+            assert e2 < 1e-1, (name, kind, e2)
     hg = net.head_gw.cpu().numpy()
     assert np.linalg.norm(hg[:4 * HEAD_IN].reshape(4, HEAD_IN) - ref["head.w"]) < 1e-2 * np.linalg.norm(ref["head.w"])
