@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -48,6 +49,7 @@ struct fc_lbm {
     long long t = 0;
     bool has_geometry = false;
     bool has_state = false;
+    int cells_per_thread = 1;  // cells each step-kernel thread updates (1 or 2)
     std::vector<uint8_t> solid_host;
     std::string err;
 };
@@ -265,6 +267,10 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     h->tbytes = h->storage == FC_LBM_STORE_F64 ? 8 : (h->storage == FC_LBM_STORE_F32 ? 4 : 2);
     h->rbytes = h->storage == FC_LBM_STORE_F64 ? 8 : 4;
     fc_lbm_params_default(&h->params);
+    // Two cells per thread keeps twice the bytes in flight per wave; it pays for the
+    // 16-bit formats, whose waves are latency-bound.  FC_LBM_CELLS_PER_THREAD overrides.
+    h->cells_per_thread = h->tbytes == 2 ? 2 : 1;
+    if (const char* e = std::getenv("FC_LBM_CELLS_PER_THREAD")) h->cells_per_thread = std::atoi(e) == 2 ? 2 : 1;
     FC_HIP(h, hipGetDevice(&h->device));
     if (desc->stream) {
         h->stream = static_cast<hipStream_t>(desc->stream);
@@ -456,10 +462,14 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     a.tau_d = h->params.tau;
     a.u_in_d = h->params.inlet_velocity;
     a.cs_d = h->params.smagorinsky_cs;
-    const long long tiles = static_cast<long long>((h->nx + kTileX - 1) / kTileX) * ((h->ny + kTileY - 1) / kTileY) *
-                            zcount;
+    const int cpt = h->cells_per_thread;
+    const long long tiles = static_cast<long long>((h->nx + kTileX - 1) / kTileX) *
+                            ((h->ny + kTileY * cpt - 1) / (kTileY * cpt)) * zcount;
     Dispatch(h, [&]<int Q, class S, class Lay>() {
-        hipLaunchKernelGGL((KStep<Q, S, Lay>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, h->stream, a);
+        if (cpt == 2)
+            hipLaunchKernelGGL((KStep<Q, S, Lay, 2>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, h->stream, a);
+        else
+            hipLaunchKernelGGL((KStep<Q, S, Lay, 1>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, h->stream, a);
     });
     FC_HIP(h, hipGetLastError());
     return FC_LBM_OK;

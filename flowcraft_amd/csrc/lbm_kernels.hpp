@@ -48,6 +48,47 @@ __device__ __forceinline__ void PutPop(T* p, T v) {
 #endif
 }
 
+// ------------------------------------------------- uniform-base addressing
+// A workgroup covers one z layer, so the address of population d of a cell splits into
+// a wave-uniform part (plane, layer, the direction's fixed neighbour offset), formed in
+// scalar registers, and a per-lane 32-bit byte offset that every direction shares.  The
+// loads and stores then issue as scalar base + vector offset with no per-direction
+// vector address arithmetic.  readfirstlane pins the base in scalar registers (the
+// compiler would otherwise fold the lane offset in first and add 64-bit bases per
+// access in vector registers); every caller passes a value that is uniform across the
+// wave.
+using GlobalConstBytes = const char __attribute__((address_space(1)))*;
+using GlobalBytes = char __attribute__((address_space(1)))*;
+
+__device__ __forceinline__ uint64_t WaveUniform(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+template <class T>
+__device__ __forceinline__ T LoadU(uint64_t base, uint32_t vo) {
+    const GlobalConstBytes p = reinterpret_cast<GlobalConstBytes>(WaveUniform(base));
+    return *reinterpret_cast<const T __attribute__((address_space(1)))*>(p + vo);
+}
+
+template <class T>
+__device__ __forceinline__ void StoreU(uint64_t base, uint32_t vo, T v) {
+    const GlobalBytes p = reinterpret_cast<GlobalBytes>(WaveUniform(base));
+    T __attribute__((address_space(1)))* q = reinterpret_cast<T __attribute__((address_space(1)))*>(p + vo);
+#if FC_LBM_NT_STORES
+    __builtin_nontemporal_store(v, q);
+#else
+    *q = v;
+#endif
+}
+
+// Byte address of element `elem` (may be negative) relative to `base`.
+template <class T>
+__device__ __forceinline__ uint64_t ElemAddr(const void* base, long long elem) {
+    return reinterpret_cast<uint64_t>(base) + static_cast<uint64_t>(elem * static_cast<long long>(sizeof(T)));
+}
+
 // -------------------------------------------------------------- constants
 template <class R>
 __host__ __device__ constexpr R K(float f, double d) {
@@ -526,24 +567,41 @@ __device__ __forceinline__ void WriteMoments(typename S::R* side_mom, const type
     m[3] = m3;
 }
 
+// Bulk pull of one cell, issued before its flag byte is known.  `lay` is the cell's
+// element offset from the start of its z layer's lower neighbour layer (z*sz), the
+// same for every direction.  Addresses are those of an interior cell and stay inside
+// the padded allocation for every cell of the lattice; values for cells that take
+// another branch are discarded.
 template <int Q, class S, class Lay>
-__global__ __launch_bounds__(256) void KStep(const StepArgs a) {
+__device__ __forceinline__ void SpecLoad(const StepArgs& a, const Geo& g, int z, uint32_t lay,
+                                         typename S::R (&fs)[Q]) {
+    using T = typename S::T;
+    const long long P = a.P;
+    const long long zoff = static_cast<long long>(z) * g.sz;
+    const uint32_t vo = lay * static_cast<uint32_t>(sizeof(T));
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        if constexpr (!Lay::kInPlace) {
+            const int off = Lat<Q>::ex[d] + Lat<Q>::ey[d] * g.sy + Lat<Q>::ez[d] * g.sz;
+            fs[d] = S::Raw(LoadU<T>(ElemAddr<T>(a.pop_src, d * P + zoff - off), vo));
+        } else {
+            constexpr int pa = PairAnchor<Q>(d);
+            const int anc = Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
+            int pl, c;
+            LoadSlot(d, a.parity, 0, anc, pl, c);
+            fs[d] = S::Raw(LoadU<T>(ElemAddr<T>(a.pop_dst, pl * P + zoff + c), vo));
+        }
+    });
+}
+
+// Everything after the bulk pull for one cell: branch on the cell type, patch wall and
+// solid directions, collide, store.
+template <int Q, class S, class Lay>
+__device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, int x, int y, int z,
+                                           const typename S::R (&fs)[Q], uint8_t fl) {
     using T = typename S::T;
     using R = typename S::R;
     constexpr bool IP = Lay::kInPlace;
-
-    const unsigned tile = XcdContiguous(blockIdx.x, gridDim.x);
-    const unsigned tiles_x = static_cast<unsigned>((a.nx + kTileX - 1) / kTileX);
-    const unsigned tiles_y = static_cast<unsigned>((a.ny + kTileY - 1) / kTileY);
-    const unsigned tx = tile % tiles_x;
-    const unsigned rest = tile / tiles_x;
-    const unsigned ty = rest % tiles_y;
-    const int x = static_cast<int>(tx * kTileX + (threadIdx.x & (kTileX - 1)));
-    const int y = static_cast<int>(ty * kTileY + (threadIdx.x / kTileX));
-    const int z = a.z0 + static_cast<int>(rest / tiles_y);
-    if (x >= a.nx || y >= a.ny) return;
-
-    const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2)};
     const int self = g.Idx(x, y, z);
     const long long P = a.P;
     const int par = a.parity;
@@ -554,25 +612,6 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     const int plane_yz = a.ny * a.nz;
     const int mom_col = x == 1 ? 0 : 1;
     const bool mom_writer = IP && (x == 1 || x == a.nx - 2);
-
-    // Bulk pull, issued for every cell before its flag byte arrives (the two loads do
-    // not wait on each other).  Addresses stay inside the padded allocation for every
-    // cell of the lattice; values for cells that take another branch are discarded.
-    R fs[Q];
-    StaticFor<0, Q>([&](auto I) {
-        constexpr int d = I;
-        if constexpr (!IP) {
-            const int off = Lat<Q>::ex[d] + Lat<Q>::ey[d] * g.sy + Lat<Q>::ez[d] * g.sz;
-            fs[d] = S::Raw(src[d * P + (self - off)]);
-        } else {
-            constexpr int pa = PairAnchor<Q>(d);
-            const int anc = self + Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
-            int pl, c;
-            LoadSlot(d, par, self, anc, pl, c);
-            fs[d] = S::Raw(dst[pl * P + c]);
-        }
-    });
-    const uint8_t fl = a.flags[self];
     R fo[Q];
 
     if (fl & kSolid) {
@@ -753,24 +792,39 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     }
 
     // ---------------------------------------------------------------- store
+    const long long zoff = static_cast<long long>(z) * g.sz;
+    const uint32_t vo = static_cast<uint32_t>(self - zoff) * static_cast<uint32_t>(sizeof(T));
     if constexpr (!IP) {
         StaticFor<0, Q>([&](auto I) {
             constexpr int d = I;
-            PutPop(&dst[d * P + self], S::Enc(fo[d]));
+            StoreU<T>(ElemAddr<T>(dst, d * P + zoff), vo, S::Enc(fo[d]));
         });
     } else {
-        const bool interior = x > 0 && x < a.nx - 1 && y > 0 && y < a.ny - 1 && z > 0 && z < a.nz - 1;
-        StaticFor<0, Q>([&](auto I) {
-            constexpr int d = I;
-            constexpr int pa = PairAnchor<Q>(d);
-            int anc = self;
-            if constexpr (d != 0)
-                anc = interior ? self + Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz
-                               : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
-            int pl, c;
-            StoreSlot(d, par, self, anc, pl, c);
-            PutPop(&dst[pl * P + c], S::Enc(fo[d]));
-        });
+        // Anchors reach across a seam only in the inlet/outlet columns (x wraps) and on
+        // periodic y/z edges; everywhere else they are fixed offsets from the cell (on a
+        // non-periodic wall they land in the ghost shell).
+        const bool fixed = x > 0 && x < a.nx - 1 && !(a.pery && (y == 0 || y == a.ny - 1)) &&
+                           !(a.perz && (z == 0 || z == a.nz - 1));
+        if (fixed) {
+            StaticFor<0, Q>([&](auto I) {
+                constexpr int d = I;
+                constexpr int pa = PairAnchor<Q>(d);
+                const int anc = Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
+                int pl, c;
+                StoreSlot(d, par, 0, anc, pl, c);
+                StoreU<T>(ElemAddr<T>(dst, pl * P + zoff + c), vo, S::Enc(fo[d]));
+            });
+        } else {
+            StaticFor<0, Q>([&](auto I) {
+                constexpr int d = I;
+                constexpr int pa = PairAnchor<Q>(d);
+                int anc = self;
+                if constexpr (d != 0) anc = g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
+                int pl, c;
+                StoreSlot(d, par, self, anc, pl, c);
+                PutPop(&dst[pl * P + c], S::Enc(fo[d]));
+            });
+        }
         if (rec != nullptr) {
             const uint32_t kinds[3] = {rec->kind[0], rec->kind[1], rec->kind[2]};
             StaticFor<1, Q>([&](auto I) {
@@ -779,6 +833,45 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
             });
         }
         if (mom_writer) WriteMoments<Q, S>(side_mom, fo, par, mom_col, plane_yz, z * a.ny + y);
+    }
+}
+
+
+// One workgroup = a 64 (x) by 4*CPT (y) tile of one z layer; each thread updates CPT
+// cells 4 rows apart.  The bulk pulls of all CPT cells are issued first, so a wave keeps
+// CPT times more bytes in flight while it computes.
+template <int Q, class S, class Lay, int CPT>
+__global__ __launch_bounds__(256) void KStep(const StepArgs a) {
+    using R = typename S::R;
+    const unsigned tile = XcdContiguous(blockIdx.x, gridDim.x);
+    const unsigned tiles_x = static_cast<unsigned>((a.nx + kTileX - 1) / kTileX);
+    const unsigned tiles_y = static_cast<unsigned>((a.ny + kTileY * CPT - 1) / (kTileY * CPT));
+    const unsigned tx = tile % tiles_x;
+    const unsigned rest = tile / tiles_x;
+    const unsigned ty = rest % tiles_y;
+    const int x = static_cast<int>(tx * kTileX + (threadIdx.x & (kTileX - 1)));
+    const int y0 = static_cast<int>(ty * kTileY * CPT + (threadIdx.x / kTileX));
+    const int z = a.z0 + static_cast<int>(rest / tiles_y);
+    if (x >= a.nx) return;
+    const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2)};
+    R fs[CPT][Q];
+    uint8_t fl[CPT];
+    // Rows past the lattice edge (a ragged last tile) load row ny-1 and are not updated.
+    const uint64_t flag_layer = reinterpret_cast<uint64_t>(a.flags) + static_cast<uint64_t>(z) * g.sz;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int y = min(y0 + k * kTileY, a.ny - 1);
+        SpecLoad<Q, S, Lay>(a, g, z, static_cast<uint32_t>(g.sz + (y + 1) * g.sy + x), fs[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int y = min(y0 + k * kTileY, a.ny - 1);
+        fl[k] = LoadU<uint8_t>(flag_layer, static_cast<uint32_t>(g.sz + (y + 1) * g.sy + x));
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int y = y0 + k * kTileY;
+        if (y < a.ny) UpdateCell<Q, S, Lay>(a, g, x, y, z, fs[k], fl[k]);
     }
 }
 
