@@ -69,7 +69,19 @@ def cpu_baseline(grid, budget_s=15.0, max_steps=20):
     on all host threads, over a bounded number of steps of the same lattice."""
     import oracle
     p = dict(tau=0.5005, u=0.08, cs=0.1)
-    cores = os.cpu_count() or 1
+    # The reference splits each step over std::thread::hardware_concurrency() threads;
+    # the run is pinned to at most 16 host cores (one GPU's share of the host).
+    saved = os.sched_getaffinity(0)
+    pinned = sorted(saved)[:16]
+    os.sched_setaffinity(0, pinned)
+    cores = len(pinned)
+    try:
+        return _cpu_baseline_run(oracle, grid, p, cores, budget_s, max_steps)
+    finally:
+        os.sched_setaffinity(0, saved)
+
+
+def _cpu_baseline_run(oracle, grid, p, cores, budget_s, max_steps):
     if oracle.ref_available():
         sim = oracle.RefLbm(grid.solid, grid.signedDistance, q=19, tau=p["tau"], u=p["u"], cs=p["cs"])
         kind = "reference"
@@ -84,7 +96,8 @@ def cpu_baseline(grid, budget_s=15.0, max_steps=20):
     mlups = grid.CellCount() * n / t / 1e6
     del sim
     return {"value": round(mlups, 2), "unit": "MLUPS", "cores": cores, "kind": kind,
-            "sample": f"{n} steps (after 1 untimed) of the full {NX}x{NY}x{NZ} D3Q19 FP32 lattice, all host threads"}
+            "sample": f"{n} steps (after 1 untimed) of the full {NX}x{NY}x{NZ} D3Q19 FP32 lattice, "
+                      f"process pinned to {cores} host cores"}
 
 
 def load_traffic(name):

@@ -185,7 +185,8 @@ class OracleLbm:
 
 
 class RefLbm:
-    """FlowCraft's own LbmReference, compiled into oracle/_ref (this container only)."""
+    """FlowCraft's own LbmReference, compiled from its sources into oracle/_ref by `make ref`
+    (built where the reference sources are present; the built library travels with the tree)."""
 
     def __init__(self, solid: np.ndarray, sdf: np.ndarray | None, q: int = 19, tau=0.8, u=0.05,
                  reverse=False, free_slip_y=True, free_slip_z=True, cs=0.12, seed_mid=False):
