@@ -160,7 +160,7 @@ def main():
             variants[f"{model.name}_{storage.name}"] = {
                 "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
                 "bytes_per_cell": 2 * q * b, "achieved_GBs": round(gbs, 1),
-                "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+                "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "finite": r["finite"]}
             print(f"# {model.name} {storage.name}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         variants["fp16_over_fp32"] = round(variants["D3Q19_F16"]["mlups"] / main_r["mlups"], 3)
 

@@ -265,6 +265,10 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     h->P = static_cast<long long>(h->nx) * (h->ny + 2) * (h->nz + 2);
     if (h->P >= (1LL << 31)) return Fail(h, FC_LBM_EINVAL, "lattice too large (padded cells must be < 2^31)");
     h->tbytes = h->storage == FC_LBM_STORE_F64 ? 8 : (h->storage == FC_LBM_STORE_F32 ? 4 : 2);
+    // The step kernel addresses three padded z layers of all planes with 32-bit byte
+    // offsets from one base.
+    if (3LL * h->Q * h->nx * (h->ny + 2) * h->tbytes >= (1LL << 31))
+        return Fail(h, FC_LBM_EINVAL, "lattice cross-section too large (3*Q*nx*(ny+2) population bytes must be < 2 GiB)");
     h->rbytes = h->storage == FC_LBM_STORE_F64 ? 8 : 4;
     fc_lbm_params_default(&h->params);
     // Two cells per thread keeps twice the bytes in flight per wave; it pays for the
@@ -425,6 +429,66 @@ int fc_lbm_reset_from_field(fc_lbm* h, const float* field4) {
     return FC_LBM_OK;
 }
 
+// Byte offsets of the step kernel's bulk pull and stores from the base of the padded
+// z layer below the cells' own (layer-major populations, see lbm_kernels.hpp): for
+// direction d of an interior cell at padded (x, y, 1) relative to that base.
+static void StepOffsets(const fc_lbm* h, int par, StepArgs& a) {
+    const long long LS = static_cast<long long>(h->nx) * (h->ny + 2);
+    const long long sy = h->nx;
+    const long long tb = h->tbytes;
+    a.layer_bytes = h->Q * LS * tb;
+    auto elem = [&](int layer, int plane, int dy, int dx) { return (layer * h->Q + plane) * LS + dy * sy + dx; };
+    auto lat = [&](int d, int& ex, int& ey, int& ez) {
+        if (h->Q == 19) {
+            ex = Lat<19>::ex[d]; ey = Lat<19>::ey[d]; ez = Lat<19>::ez[d];
+        } else {
+            ex = Lat<15>::ex[d]; ey = Lat<15>::ey[d]; ez = Lat<15>::ez[d];
+        }
+    };
+    for (int d = 0; d < 19; ++d) {
+        a.ld_off[d] = a.st_off[d] = 0;
+        if (d >= h->Q) continue;
+        long long ld, st, prev;
+        if (h->layout != FC_LBM_LAYOUT_INPLACE) {
+            int ex, ey, ez;
+            lat(d, ex, ey, ez);
+            ld = elem(1 - ez, d, -ey, -ex);  // pull from x - e_d
+            st = prev = elem(1, d, 0, 0);
+        } else {
+            const int pa = d == 0 ? 0 : ((d & 1) ? d : d - 1);
+            int ex, ey, ez;
+            lat(pa, ex, ey, ez);
+            int pl;
+            bool anc;
+            LoadSlot(d, par, pl, anc);
+            ld = anc ? elem(1 + ez, pl, ey, ex) : elem(1, pl, 0, 0);
+            StoreSlot(d, par, pl, anc);
+            st = anc ? elem(1 + ez, pl, ey, ex) : elem(1, pl, 0, 0);
+            StoreSlot(d, par ^ 1, pl, anc);
+            prev = anc ? elem(1 + ez, pl, ey, ex) : elem(1, pl, 0, 0);
+        }
+        a.ld_off[d] = static_cast<uint32_t>(ld * tb);
+        a.st_off[d] = static_cast<uint32_t>(st * tb);
+        a.prev_off[d] = static_cast<uint32_t>(prev * tb);
+    }
+}
+
+// Inlet populations for the current parameters, with the step's own arithmetic.
+extern "C++" template <int Q, class S>
+static void InletPops(const fc_lbm* h, StepArgs& a) {
+    using R = typename S::R;
+    R fo[Q];
+    const R u = static_cast<R>(h->params.reverse_flow ? -h->params.inlet_velocity : h->params.inlet_velocity);
+    if constexpr (S::kOffset)
+        EqOffset<Q, S>(0.0f, u, 0.0f, 0.0f, fo);
+    else
+        FeqAll<Q, R>(R(1), u, R(0), R(0), fo);
+    for (int d = 0; d < 19; ++d) {
+        a.inlet_f[d] = d < Q ? static_cast<float>(fo[d]) : 0.0f;
+        a.inlet_d[d] = d < Q ? static_cast<double>(fo[d]) : 0.0;
+    }
+}
+
 int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     if (!h) return FC_LBM_EINVAL;
     int rc = CheckReady(h);
@@ -445,6 +509,7 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     a.side_mom = h->side_mom;
     a.diverged = h->dflag;
     a.P = h->P;
+    StepOffsets(h, par, a);
     a.nx = h->nx;
     a.ny = h->ny;
     a.nz = h->nz;
@@ -466,10 +531,19 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     const long long tiles = static_cast<long long>((h->nx + kTileX - 1) / kTileX) *
                             ((h->ny + kTileY * cpt - 1) / (kTileY * cpt)) * zcount;
     Dispatch(h, [&]<int Q, class S, class Lay>() {
-        if (cpt == 2)
-            hipLaunchKernelGGL((KStep<Q, S, Lay, 2>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, h->stream, a);
-        else
-            hipLaunchKernelGGL((KStep<Q, S, Lay, 1>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, h->stream, a);
+        InletPops<Q, S>(h, a);
+        const dim3 grid(static_cast<unsigned>(tiles));
+        if (h->pery | h->perz) {
+            if (cpt == 2)
+                hipLaunchKernelGGL((KStep<Q, S, Lay, 2, true>), grid, dim3(256), 0, h->stream, a);
+            else
+                hipLaunchKernelGGL((KStep<Q, S, Lay, 1, true>), grid, dim3(256), 0, h->stream, a);
+        } else {
+            if (cpt == 2)
+                hipLaunchKernelGGL((KStep<Q, S, Lay, 2, false>), grid, dim3(256), 0, h->stream, a);
+            else
+                hipLaunchKernelGGL((KStep<Q, S, Lay, 1, false>), grid, dim3(256), 0, h->stream, a);
+        }
     });
     FC_HIP(h, hipGetLastError());
     return FC_LBM_OK;

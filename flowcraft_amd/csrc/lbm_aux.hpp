@@ -193,7 +193,7 @@ __global__ void KSeed(void* pops0, void* pops1, long long P, long long t, int nx
     if (c >= N) return;
     int x, y, z;
     Coords(c, nx, ny, x, y, z);
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2)};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q};
     R rho, ux, uy, uz;
     if (field4 == nullptr) {
         rho = static_cast<R>(rho0);
@@ -209,27 +209,14 @@ __global__ void KSeed(void* pops0, void* pops1, long long P, long long t, int nx
         uy = static_cast<R>(field4[c * 4 + 2]);
         uz = static_cast<R>(field4[c * 4 + 3]);
     }
-    const int self = g.Idx(x, y, z);
     R feq[Q];
     if constexpr (S::kOffset)
         EqOffset<Q, S>(rho - 1.0f, ux, uy, uz, feq);
     else
         FeqAll<Q, R>(rho, ux, uy, uz, feq);
+    T* f = static_cast<T*>(const_cast<void*>(CurrentCopy<Lay>(pops0, pops1, t)));
 #pragma unroll
-    for (int d = 0; d < Q; ++d) {
-        const R fe = feq[d];
-        if constexpr (!Lay::kInPlace) {
-            T* f = static_cast<T*>((t & 1) ? pops1 : pops0);
-            f[d * P + self] = S::Enc(fe);
-        } else {
-            T* f = static_cast<T*>(pops0);
-            const int pa = PairAnchor<Q>(d);
-            const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
-            int pl, cc;
-            StoreSlot(d, static_cast<int>((t - 1) & 1), self, anc, pl, cc);
-            f[pl * P + cc] = S::Enc(fe);
-        }
-    }
+    for (int d = 0; d < Q; ++d) f[CurrentIndex<Q, Lay>(g, t, x, y, z, d)] = S::Enc(feq[d]);
 }
 
 // Exact state restore: values f[d*N + cell] in the storage's compute type.
@@ -242,23 +229,10 @@ __global__ void KSetPops(void* pops0, void* pops1, long long P, long long t, int
     if (c >= N) return;
     int x, y, z;
     Coords(c, nx, ny, x, y, z);
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2)};
-    const int self = g.Idx(x, y, z);
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q};
+    T* f = static_cast<T*>(const_cast<void*>(CurrentCopy<Lay>(pops0, pops1, t)));
 #pragma unroll
-    for (int d = 0; d < Q; ++d) {
-        const typename S::R v = in[d * N + c];
-        if constexpr (!Lay::kInPlace) {
-            T* f = static_cast<T*>((t & 1) ? pops1 : pops0);
-            f[d * P + self] = S::Store(v, d);
-        } else {
-            T* f = static_cast<T*>(pops0);
-            const int pa = PairAnchor<Q>(d);
-            const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
-            int pl, cc;
-            StoreSlot(d, static_cast<int>((t - 1) & 1), self, anc, pl, cc);
-            f[pl * P + cc] = S::Store(v, d);
-        }
-    }
+    for (int d = 0; d < Q; ++d) f[CurrentIndex<Q, Lay>(g, t, x, y, z, d)] = S::Store(in[d * N + c], d);
 }
 
 // State dump.  In place, solid cells are never updated after the first step;
@@ -272,11 +246,11 @@ __global__ void KGetPops(const void* pops0, const void* pops1, long long P, long
     if (c >= N) return;
     int x, y, z;
     Coords(c, nx, ny, x, y, z);
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2)};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q};
     const bool stale = Lay::kInPlace && t > 0 && solid[c];
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-        out[d * N + c] = stale ? W<Q, R>(d) : CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
+        out[d * N + c] = stale ? W<Q, R>(d) : CurrentPop<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
 }
 
 // In place: rebuild the side values (q >= 1/2 links, outlet-column moments)
@@ -286,7 +260,7 @@ __global__ void KRefreshSide(const void* pops0, const void* pops1, long long P, 
                              int pery, int perz, const int* __restrict__ list, int count, BRec<typename S::R>* recs,
                              typename S::R* side_mom) {
     using R = typename S::R;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2)};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q};
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = 2 * ny * nz;
     if (k < count) {
@@ -296,7 +270,7 @@ __global__ void KRefreshSide(const void* pops0, const void* pops1, long long P, 
         const int z = n / (nx * (ny + 2)) - 1;
         BRec<R>& r = recs[k];
         for (int d = 1; d < Q; ++d)
-            if (KindOf(r.kind, d) == kBouzidiHi) r.side[d] = CurrentRaw<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
+            if (KindOf(r.kind, d) == kBouzidiHi) r.side[d] = CurrentRaw<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
     } else if (k < count + ncol) {
         const int j = k - count;
         const int col = j / (ny * nz);
@@ -305,7 +279,7 @@ __global__ void KRefreshSide(const void* pops0, const void* pops1, long long P, 
         const int x = col == 0 ? 1 : nx - 2;
         R m0 = 0, m1 = 0, m2 = 0, m3 = 0;
         for (int d = 0; d < Q; ++d) {
-            R v = CurrentRaw<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
+            R v = CurrentRaw<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
             m0 += v;
             m1 += v * static_cast<R>(Lat<Q>::ex[d]);
             m2 += v * static_cast<R>(Lat<Q>::ey[d]);
@@ -330,7 +304,7 @@ __global__ void KSurfaceForce(const void* pops0, const void* pops1, long long P,
     using R = typename S::R;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsurf) return;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2)};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q};
     const int n = surf[k];
     const int x = n % nx;
     const int y = (n / nx) % (ny + 2) - 1;
@@ -342,7 +316,7 @@ __global__ void KSurfaceForce(const void* pops0, const void* pops1, long long P,
         const int fn = g.Idx(a, b, e);
         if (flags[fn] & kSolid) continue;
         const int o = Opp(d);
-        const R plain = CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, a, b, e, o);
+        const R plain = CurrentPop<Q, S, Lay>(pops0, pops1, t, g, a, b, e, o);
         R fb = plain;
         const BRec<R>& r = recs[bidx[fn]];
         const unsigned kd = KindOf(r.kind, d);
@@ -350,11 +324,11 @@ __global__ void KSurfaceForce(const void* pops0, const void* pops1, long long P,
             int ffx = a + Lat<Q>::ex[d], ffy = b + Lat<Q>::ey[d], ffz = e + Lat<Q>::ez[d];
             Resolve(ffx, ffy, ffz, nx, ny, nz, pery, perz);
             const R q = r.q[d];
-            fb = R(2) * q * plain + (R(1) - R(2) * q) * CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, ffx, ffy, ffz, o);
+            fb = R(2) * q * plain + (R(1) - R(2) * q) * CurrentPop<Q, S, Lay>(pops0, pops1, t, g, ffx, ffy, ffz, o);
         } else if (kd == kBouzidiHi) {
             const R q = r.q[d];
             const R inv2q = R(1) / (R(2) * q);
-            fb = inv2q * plain + (R(1) - inv2q) * CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, a, b, e, d);
+            fb = inv2q * plain + (R(1) - inv2q) * CurrentPop<Q, S, Lay>(pops0, pops1, t, g, a, b, e, d);
         }
         const R dp = (plain + fb) - R(2) * W<Q, R>(d);
         fx -= static_cast<R>(Lat<Q>::ex[d]) * dp;
@@ -383,13 +357,13 @@ __global__ void KField(const void* pops0, const void* pops1, long long P, long l
     int x, y, zr;
     Coords(j, nx, ny, x, y, zr);
     const int z = z0 + zr;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2)};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q};
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (!(flags[g.Idx(x, y, z)] & kSolid)) {
         R rho = 0, ux = 0, uy = 0, uz = 0;
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
-            const R fi = CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
+            const R fi = CurrentPop<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
             rho += fi;
             ux += fi * static_cast<R>(Lat<Q>::ex[d]);
             uy += fi * static_cast<R>(Lat<Q>::ey[d]);
@@ -470,7 +444,7 @@ __global__ __launch_bounds__(256) void KStatsPartial(const void* pops0, const vo
                                                      const uint8_t* __restrict__ flags, double* __restrict__ part) {
     using R = typename S::R;
     const long long N = static_cast<long long>(nx) * ny * nz;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2)};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q};
     double dsum = 0, ssum = 0, cnt = 0, nan = 0;
     double mn = 1e30, mx = -1e30, ms = 0;
     for (long long c = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; c < N;
@@ -481,7 +455,7 @@ __global__ __launch_bounds__(256) void KStatsPartial(const void* pops0, const vo
         R rho = 0, ux = 0, uy = 0, uz = 0;
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
-            const R fi = CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
+            const R fi = CurrentPop<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
             rho += fi;
             ux += fi * static_cast<R>(Lat<Q>::ex[d]);
             uy += fi * static_cast<R>(Lat<Q>::ey[d]);
@@ -563,12 +537,12 @@ __global__ void KProbe(const void* pops0, const void* pops1, long long P, long l
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nx * ny) return;
     const int y = j / nx, x = j - y * nx;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2)};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q};
     if (flags[g.Idx(x, y, z)] & kSolid) return;
     R rho = 0, ux = 0, uy = 0, uz = 0;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-        const R fi = CurrentPop<Q, S, Lay>(pops0, pops1, P, t, g, x, y, z, d);
+        const R fi = CurrentPop<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
         rho += fi;
         ux += fi * static_cast<R>(Lat<Q>::ex[d]);
         uy += fi * static_cast<R>(Lat<Q>::ey[d]);

@@ -15,13 +15,17 @@
 // reference's rounding.  The macro field is NOT written every step (the GLSL
 // kernel's macroOut); it is derived on demand from the current populations.
 //
-// HBM layout.  Populations are SoA planes f[d*P + n] with a one-cell ghost
-// shell in y and z only:  n = ((z+1)*(ny+2) + (y+1))*nx + x,
-// P = nx*(ny+2)*(nz+2).  Rows stay nx long, so a 64-lane wave reading 64
-// consecutive x of one plane is one contiguous 256 B (FP32) / 128 B (16-bit)
-// segment.  Ghost cells are never updated; in the in-place layout they hold
-// the populations that leave the domain through a y/z wall, which is what
-// makes wall reads race-free (see InPlace below).
+// HBM layout.  Layer-major SoA with a one-cell ghost shell in y and z only: the
+// Q population planes of one padded z layer are contiguous,
+//   f[((z+1)*Q + d)*LS + (y+1)*nx + x],   LS = nx*(ny+2),
+// for z in -1..nz (Q*(nz+2)*LS elements in all).  Rows stay nx long, so a
+// 64-lane wave reading 64 consecutive x of one plane is one contiguous 256 B
+// (FP32) / 128 B (16-bit) segment; and everything a z layer's update touches
+// lies within three consecutive layers (3*Q*LS elements), so the step kernel
+// addresses it with 32-bit offsets from one per-workgroup base at any lattice
+// size.  Ghost cells are never updated; in the in-place layout they hold the
+// populations that leave the domain through a y/z wall, which is what makes
+// wall reads race-free (see InPlace below).
 
 #pragma once
 
@@ -48,18 +52,47 @@ __device__ __forceinline__ void PutPop(T* p, T v) {
 #endif
 }
 
-// ------------------------------------------------- uniform-base addressing
-// A workgroup covers one z layer, so the address of population d of a cell splits into
-// a wave-uniform part (plane, layer, the direction's fixed neighbour offset), formed in
-// scalar registers, and a per-lane 32-bit byte offset that every direction shares.  The
-// loads and stores then issue as scalar base + vector offset with no per-direction
-// vector address arithmetic.  readfirstlane pins the base in scalar registers (the
-// compiler would otherwise fold the lane offset in first and add 64-bit bases per
-// access in vector registers); every caller passes a value that is uniform across the
-// wave.
-using GlobalConstBytes = const char __attribute__((address_space(1)))*;
-using GlobalBytes = char __attribute__((address_space(1)))*;
+// ---------------------------------------------------- layer-relative access
+// The step kernel reads and writes populations through a buffer resource whose base
+// is the workgroup's lower z layer (uniform, in scalar registers); each access adds a
+// per-direction byte offset (a scalar, precomputed on the host) to a per-lane byte
+// offset shared by every direction, so no vector address arithmetic is spent per
+// population.  The resource's range is the full 32-bit offset space.
+using BufRsrc = __amdgpu_buffer_rsrc_t;
 
+__device__ __forceinline__ BufRsrc LayerRsrc(const void* base) {
+    // word 3: 32-bit data format, no swizzle, no thread-id offset
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
+}
+
+#if FC_LBM_NT_STORES
+constexpr int kStoreAux = 2;  // non-temporal
+#else
+constexpr int kStoreAux = 0;
+#endif
+
+template <class T>
+__device__ __forceinline__ T BufLoad(BufRsrc r, uint32_t vo, uint32_t so) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    else if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
+}
+
+template <class T>
+__device__ __forceinline__ void BufStore(BufRsrc r, uint32_t vo, uint32_t so, T v) {
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, vo, so, kStoreAux);
+    else if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, vo, so, kStoreAux);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, vo, so, kStoreAux);
+}
+
+// Byte loads of per-cell arrays (flags) from a wave-uniform base: readfirstlane keeps
+// the base in scalar registers, so the load issues as scalar base + 32-bit lane offset.
 __device__ __forceinline__ uint64_t WaveUniform(uint64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
     const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
@@ -68,25 +101,9 @@ __device__ __forceinline__ uint64_t WaveUniform(uint64_t v) {
 
 template <class T>
 __device__ __forceinline__ T LoadU(uint64_t base, uint32_t vo) {
+    using GlobalConstBytes = const char __attribute__((address_space(1)))*;
     const GlobalConstBytes p = reinterpret_cast<GlobalConstBytes>(WaveUniform(base));
     return *reinterpret_cast<const T __attribute__((address_space(1)))*>(p + vo);
-}
-
-template <class T>
-__device__ __forceinline__ void StoreU(uint64_t base, uint32_t vo, T v) {
-    const GlobalBytes p = reinterpret_cast<GlobalBytes>(WaveUniform(base));
-    T __attribute__((address_space(1)))* q = reinterpret_cast<T __attribute__((address_space(1)))*>(p + vo);
-#if FC_LBM_NT_STORES
-    __builtin_nontemporal_store(v, q);
-#else
-    *q = v;
-#endif
-}
-
-// Byte address of element `elem` (may be negative) relative to `base`.
-template <class T>
-__device__ __forceinline__ uint64_t ElemAddr(const void* base, long long elem) {
-    return reinterpret_cast<uint64_t>(base) + static_cast<uint64_t>(elem * static_cast<long long>(sizeof(T)));
 }
 
 // -------------------------------------------------------------- constants
@@ -186,7 +203,13 @@ struct StepArgs {
     void* recs;      // BRec<R>*
     void* side_mom;  // R[2 parity][2 column][ny*nz][4]: moments of columns 1 and nx-2
     int* diverged;
-    long long P;     // plane stride (cells)
+    long long P;     // padded cells per plane set: nx*(ny+2)*(nz+2)
+    long long layer_bytes;  // Q*LS*sizeof(T): one padded z layer of all planes
+    uint32_t ld_off[19];    // bulk-pull byte offset of direction d from the lower layer's base
+    uint32_t st_off[19];    // store byte offset of direction d (interior anchors in place)
+    uint32_t prev_off[19];  // byte offset of the cell's own outgoing f_d of the previous step
+    float inlet_f[19];      // inlet populations (working values: f, or the stored offsets)
+    double inlet_d[19];     // the same for the FP64 step
     int nx, ny, nz;
     int z0, zc;
     int parity;      // step index & 1
@@ -197,8 +220,24 @@ struct StepArgs {
 
 struct Geo {
     int nx, ny, nz, pery, perz;
-    int sy, sz;  // padded strides: sy = nx, sz = nx*(ny+2)
+    int sy, sz;  // padded strides of the per-cell arrays: sy = nx, sz = LS = nx*(ny+2)
+    int Q;
+    // index of a padded cell in per-cell arrays (flags, boundary-record index)
     __device__ __forceinline__ int Idx(int x, int y, int z) const { return (z + 1) * sz + (y + 1) * sy + x; }
+    // element index of population plane `pl` of padded cell (x, y, z) (x may be -1 or nx)
+    __device__ __forceinline__ long long Pop(int pl, int x, int y, int z) const {
+        return (static_cast<long long>(z + 1) * Q + pl) * sz + (y + 1) * sy + x;
+    }
+    // neighbour coordinates with the wrap rules of Nbr
+    __device__ __forceinline__ void NbrXYZ(int x, int y, int z, int dx, int dy, int dz, int& ax, int& ay,
+                                           int& az) const {
+        ax = x + dx;
+        ay = y + dy;
+        az = z + dz;
+        ax = ax >= nx ? ax - nx : (ax < 0 ? ax + nx : ax);
+        if (pery) ay = ay >= ny ? ay - ny : (ay < 0 ? ay + ny : ay);
+        if (perz) az = az >= nz ? az - nz : (az < 0 ? az + nz : az);
+    }
     // Neighbour index with the layout's wrap rules: x always wraps (only the
     // inlet/outlet columns ever reach across it, and they never read a slot);
     // y/z wrap when periodic, otherwise land in the ghost shell.
@@ -231,31 +270,31 @@ struct InPlace {
     static constexpr bool kInPlace = true;
 };
 
-// Slot (plane, cell) of population d read (load) / written (store) at a parity.
-// `self` is the cell, `anc` the anchor of d's pair.
-__device__ __forceinline__ void LoadSlot(int d, int par, int self, int anc, int& plane, int& cell) {
+// Slot of population d read (load) / written (store) at a parity: its plane, and
+// whether it sits at the cell itself or at the anchor of d's pair.
+__host__ __device__ inline void LoadSlot(int d, int par, int& plane, bool& at_anchor) {
     if (d == 0) {
         plane = 0;
-        cell = self;
+        at_anchor = false;
     } else if (d & 1) {
         plane = par == 0 ? d + 1 : d;
-        cell = self;
+        at_anchor = false;
     } else {
         plane = par == 0 ? d - 1 : d;
-        cell = anc;
+        at_anchor = true;
     }
 }
 
-__device__ __forceinline__ void StoreSlot(int d, int par, int self, int anc, int& plane, int& cell) {
+__host__ __device__ inline void StoreSlot(int d, int par, int& plane, bool& at_anchor) {
     if (d == 0) {
         plane = 0;
-        cell = self;
+        at_anchor = false;
     } else if (d & 1) {
         plane = par == 0 ? d : d + 1;
-        cell = anc;
+        at_anchor = true;
     } else {
         plane = par == 0 ? d : d - 1;
-        cell = self;
+        at_anchor = false;
     }
 }
 
@@ -264,42 +303,55 @@ __host__ __device__ constexpr int PairAnchor(int d) {
     return d == 0 ? 0 : ((d & 1) ? d : d - 1);
 }
 
+// Element index of an in-place slot of population d of cell (x,y,z) at a parity.  The
+// anchor wraps across periodic y/z seams; in x it does not: the inlet and outlet
+// columns, whose anchors would leave the row, use the neighbouring row's end cell
+// (or the ghost row), a slot no other cell owns and that neither column reads back
+// during a step.
+template <int Q, bool kStore>
+__device__ __forceinline__ long long SlotIndex(const Geo& g, int d, int par, int x, int y, int z) {
+    int pl;
+    bool at_anchor;
+    if constexpr (kStore)
+        StoreSlot(d, par, pl, at_anchor);
+    else
+        LoadSlot(d, par, pl, at_anchor);
+    if (!at_anchor) return g.Pop(pl, x, y, z);
+    const int a = PairAnchor<Q>(d);
+    int ax, ay, az;
+    g.NbrXYZ(x, y, z, Lat<Q>::ex[a], Lat<Q>::ey[a], Lat<Q>::ez[a], ax, ay, az);
+    return g.Pop(pl, x + Lat<Q>::ex[a], ay, az);
+}
+
+// Element index holding the current post-collision population d of cell (x,y,z)
+// after `t` steps, and the copy it is in.
+template <int Q, class Lay>
+__device__ __forceinline__ long long CurrentIndex(const Geo& g, long long t, int x, int y, int z, int d) {
+    if constexpr (!Lay::kInPlace)
+        return g.Pop(d, x, y, z);
+    else
+        return SlotIndex<Q, true>(g, d, static_cast<int>((t - 1) & 1), x, y, z);
+}
+
+template <class Lay>
+__device__ __forceinline__ const void* CurrentCopy(const void* pops0, const void* pops1, long long t) {
+    return (!Lay::kInPlace && (t & 1)) ? pops1 : pops0;
+}
+
 // Current post-collision population d of cell (x,y,z) after `t` steps.
 template <int Q, class S, class Lay>
-__device__ __forceinline__ typename S::R CurrentPop(const void* pops0, const void* pops1, long long P, long long t,
-                                                   const Geo& g, int x, int y, int z, int d) {
-    using T = typename S::T;
-    const int self = g.Idx(x, y, z);
-    if constexpr (!Lay::kInPlace) {
-        const T* f = static_cast<const T*>((t & 1) ? pops1 : pops0);
-        return S::Load(f[d * P + self], d);
-    } else {
-        const T* f = static_cast<const T*>(pops0);
-        const int a = PairAnchor<Q>(d);
-        const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[a], Lat<Q>::ey[a], Lat<Q>::ez[a]);
-        int pl, c;
-        StoreSlot(d, static_cast<int>((t - 1) & 1), self, anc, pl, c);
-        return S::Load(f[pl * P + c], d);
-    }
+__device__ __forceinline__ typename S::R CurrentPop(const void* pops0, const void* pops1, long long t, const Geo& g,
+                                                   int x, int y, int z, int d) {
+    const auto* f = static_cast<const typename S::T*>(CurrentCopy<Lay>(pops0, pops1, t));
+    return S::Load(f[CurrentIndex<Q, Lay>(g, t, x, y, z, d)], d);
 }
 
 // The same slot as the step's working value (S::Raw): f, or the offset g.
 template <int Q, class S, class Lay>
-__device__ __forceinline__ typename S::R CurrentRaw(const void* pops0, const void* pops1, long long P, long long t,
-                                                   const Geo& g, int x, int y, int z, int d) {
-    using T = typename S::T;
-    const int self = g.Idx(x, y, z);
-    if constexpr (!Lay::kInPlace) {
-        const T* f = static_cast<const T*>((t & 1) ? pops1 : pops0);
-        return S::Raw(f[d * P + self]);
-    } else {
-        const T* f = static_cast<const T*>(pops0);
-        const int a = PairAnchor<Q>(d);
-        const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[a], Lat<Q>::ey[a], Lat<Q>::ez[a]);
-        int pl, c;
-        StoreSlot(d, static_cast<int>((t - 1) & 1), self, anc, pl, c);
-        return S::Raw(f[pl * P + c]);
-    }
+__device__ __forceinline__ typename S::R CurrentRaw(const void* pops0, const void* pops1, long long t, const Geo& g,
+                                                   int x, int y, int z, int d) {
+    const auto* f = static_cast<const typename S::T*>(CurrentCopy<Lay>(pops0, pops1, t));
+    return S::Raw(f[CurrentIndex<Q, Lay>(g, t, x, y, z, d)]);
 }
 
 // ------------------------------------------------- compile-time lattice sums
@@ -309,7 +361,7 @@ __device__ __forceinline__ typename S::R CurrentRaw(const void* pops0, const voi
 // for finite operands the result is identical, except possibly the sign of an exact
 // zero.  This removes most of the collision's arithmetic without changing rounding.
 template <int B, int E, class F>
-__device__ __forceinline__ void StaticFor(F&& f) {
+__host__ __device__ __forceinline__ void StaticFor(F&& f) {
     if constexpr (B < E) {
         f(std::integral_constant<int, B>{});
         StaticFor<B + 1, E>(f);
@@ -317,7 +369,7 @@ __device__ __forceinline__ void StaticFor(F&& f) {
 }
 
 template <int c, class R>
-__device__ __forceinline__ void AccC(R& a, R v) {
+__host__ __device__ __forceinline__ void AccC(R& a, R v) {
     if constexpr (c == 1)
         a += v;
     else if constexpr (c == -1)
@@ -325,7 +377,7 @@ __device__ __forceinline__ void AccC(R& a, R v) {
 }
 
 template <int c, class R>
-__device__ __forceinline__ R MulC(R v) {
+__host__ __device__ __forceinline__ R MulC(R v) {
     if constexpr (c == 1)
         return v;
     else
@@ -334,7 +386,7 @@ __device__ __forceinline__ R MulC(R v) {
 
 // c1*u + c2*v + c3*w, left to right, zero terms skipped.
 template <int c1, int c2, int c3, class R>
-__device__ __forceinline__ R Dot3(R u, R v, R w) {
+__host__ __device__ __forceinline__ R Dot3(R u, R v, R w) {
     if constexpr (c1 != 0) {
         R r = MulC<c1>(u);
         if constexpr (c2 != 0) r = r + MulC<c2>(v);
@@ -354,13 +406,13 @@ __device__ __forceinline__ R Dot3(R u, R v, R w) {
 // Equilibrium of direction d (LbmReference.cpp:9-13): w*rho*(((1+3eu)+(4.5eu)eu)-1.5u2),
 // u2 = ux*ux + uy*uy + uz*uz computed once by the caller.
 template <int Q, int d, class R>
-__device__ __forceinline__ R FeqD(R rho, R ux, R uy, R uz, R u2) {
+__host__ __device__ __forceinline__ R FeqD(R rho, R ux, R uy, R uz, R u2) {
     const R eu = Dot3<Lat<Q>::ex[d], Lat<Q>::ey[d], Lat<Q>::ez[d]>(ux, uy, uz);
     return W<Q, R>(d) * rho * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * u2);
 }
 
 template <int Q, class R>
-__device__ __forceinline__ void FeqAll(R rho, R ux, R uy, R uz, R (&out)[Q]) {
+__host__ __device__ __forceinline__ void FeqAll(R rho, R ux, R uy, R uz, R (&out)[Q]) {
     const R u2 = ux * ux + uy * uy + uz * uz;
     StaticFor<0, Q>([&](auto I) {
         constexpr int d = I;
@@ -447,7 +499,7 @@ __device__ __forceinline__ bool Collide(const R (&f)[Q], R (&fo)[Q], R tau, R cs
 // on small offsets instead of O(w) populations and needs about half the arithmetic.
 // The CPU oracle emulates exactly these operations for the offset formats.
 template <int Q, class S>
-__device__ __forceinline__ void ReconstructOffset(float drho, float jx, float jy, float jz, float txx, float tyy,
+__host__ __device__ __forceinline__ void ReconstructOffset(float drho, float jx, float jy, float jz, float txx, float tyy,
                                                   float tzz, float txy, float txz, float tyz, float (&go)[Q]) {
     constexpr float c3 = 1.0f / 3.0f;
     const float tr3 = (txx + tyy + tzz) * c3;
@@ -488,7 +540,7 @@ __device__ __forceinline__ void ReconstructOffset(float drho, float jx, float jy
 
 // Equilibrium offsets of (1 + drho, u).
 template <int Q, class S>
-__device__ __forceinline__ void EqOffset(float drho, float ux, float uy, float uz, float (&go)[Q]) {
+__host__ __device__ __forceinline__ void EqOffset(float drho, float ux, float uy, float uz, float (&go)[Q]) {
     const float rho = 1.0f + drho;
     const float jx = rho * ux, jy = rho * uy, jz = rho * uz;
     ReconstructOffset<Q, S>(drho, jx, jy, jz, jx * ux, jy * uy, jz * uz, jx * uy, jx * uz, jy * uz, go);
@@ -567,56 +619,79 @@ __device__ __forceinline__ void WriteMoments(typename S::R* side_mom, const type
     m[3] = m3;
 }
 
-// Bulk pull of one cell, issued before its flag byte is known.  `lay` is the cell's
-// element offset from the start of its z layer's lower neighbour layer (z*sz), the
-// same for every direction.  Addresses are those of an interior cell and stay inside
-// the padded allocation for every cell of the lattice; values for cells that take
-// another branch are discarded.
-template <int Q, class S, class Lay>
-__device__ __forceinline__ void SpecLoad(const StepArgs& a, const Geo& g, int z, uint32_t lay,
-                                         typename S::R (&fs)[Q]) {
+// Bulk pull of one cell, issued before its flag byte is known.  `src` addresses the
+// workgroup's lower z layer; `vo` is the cell's byte offset within a layer, shared by
+// every direction.  The offsets are those of an interior cell and stay inside the
+// padded allocation for every cell of the lattice; values for cells that take another
+// branch are discarded.
+template <int Q, class S>
+__device__ __forceinline__ void SpecLoad(const StepArgs& a, BufRsrc src, uint32_t vo, typename S::R (&fs)[Q]) {
     using T = typename S::T;
-    const long long P = a.P;
-    const long long zoff = static_cast<long long>(z) * g.sz;
-    const uint32_t vo = lay * static_cast<uint32_t>(sizeof(T));
     StaticFor<0, Q>([&](auto I) {
         constexpr int d = I;
-        if constexpr (!Lay::kInPlace) {
-            const int off = Lat<Q>::ex[d] + Lat<Q>::ey[d] * g.sy + Lat<Q>::ez[d] * g.sz;
-            fs[d] = S::Raw(LoadU<T>(ElemAddr<T>(a.pop_src, d * P + zoff - off), vo));
-        } else {
-            constexpr int pa = PairAnchor<Q>(d);
-            const int anc = Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
-            int pl, c;
-            LoadSlot(d, a.parity, 0, anc, pl, c);
-            fs[d] = S::Raw(LoadU<T>(ElemAddr<T>(a.pop_dst, pl * P + zoff + c), vo));
-        }
+        fs[d] = S::Raw(BufLoad<T>(src, vo, a.ld_off[d]));
     });
 }
 
+// Stores of a cell's post-collision populations (and, in place, the side data that
+// other cells read next step).  `seam`: a cell on a periodic y/z edge, whose anchors
+// wrap (only instantiated for periodic lattices).
+template <int Q, class S, class Lay, bool PER>
+__device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRsrc dst_layer, uint32_t vo, int x,
+                                          int y, int z, bool seam, const typename S::R (&fo)[Q], BRec<typename S::R>* rec) {
+    using T = typename S::T;
+    using R = typename S::R;
+    if constexpr (PER && Lay::kInPlace) {
+        if (seam) {
+            T* dst = static_cast<T*>(a.pop_dst);
+            StaticFor<0, Q>([&](auto I) {
+                constexpr int d = I;
+                PutPop(&dst[SlotIndex<Q, true>(g, d, a.parity, x, y, z)], S::Enc(fo[d]));
+            });
+        } else {
+            StaticFor<0, Q>([&](auto I) {
+                constexpr int d = I;
+                BufStore<T>(dst_layer, vo, a.st_off[d], S::Enc(fo[d]));
+            });
+        }
+    } else {
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            BufStore<T>(dst_layer, vo, a.st_off[d], S::Enc(fo[d]));
+        });
+    }
+    if constexpr (Lay::kInPlace) {
+        if (rec != nullptr) {
+            const uint32_t kinds[3] = {rec->kind[0], rec->kind[1], rec->kind[2]};
+            StaticFor<1, Q>([&](auto I) {
+                constexpr int d = I;
+                if (KindOf(kinds, d) == kBouzidiHi) rec->side[d] = S::Raw(S::Enc(fo[d]));
+            });
+        }
+        if (x == 1 || x == a.nx - 2)
+            WriteMoments<Q, S>(static_cast<R*>(a.side_mom), fo, a.parity, x == 1 ? 0 : 1, a.ny * a.nz, z * a.ny + y);
+    }
+}
+
 // Everything after the bulk pull for one cell: branch on the cell type, patch wall and
-// solid directions, collide, store.
-template <int Q, class S, class Lay>
-__device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, int x, int y, int z,
-                                           const typename S::R (&fs)[Q], uint8_t fl) {
+// solid directions, collide, store.  Each branch stores its own results, so that no
+// branch's values have to be merged (and moved) on the path every wave takes.
+template <int Q, class S, class Lay, bool PER>
+__device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufRsrc src_layer, BufRsrc dst_layer,
+                                           int x, int y, int z, const typename S::R (&fs)[Q], uint8_t fl) {
     using T = typename S::T;
     using R = typename S::R;
     constexpr bool IP = Lay::kInPlace;
-    const int self = g.Idx(x, y, z);
-    const long long P = a.P;
     const int par = a.parity;
-
-    T* __restrict__ dst = static_cast<T*>(a.pop_dst);
-    const T* src = static_cast<const T*>(a.pop_src);
-    R* side_mom = static_cast<R*>(a.side_mom);
-    const int plane_yz = a.ny * a.nz;
-    const int mom_col = x == 1 ? 0 : 1;
-    const bool mom_writer = IP && (x == 1 || x == a.nx - 2);
-    R fo[Q];
+    const uint32_t vo = static_cast<uint32_t>((y + 1) * g.sy + x) * static_cast<uint32_t>(sizeof(T));
+    const bool seam = PER && ((a.pery && (y == 0 || y == a.ny - 1)) || (a.perz && (z == 0 || z == a.nz - 1)));
+    const int inletX = a.reverse ? a.nx - 1 : 0;
+    const int outletX = a.reverse ? 0 : a.nx - 1;
 
     if (fl & kSolid) {
         // Solid cell: populations w_i (LbmReference.cpp:264-267).  In place, solid
         // cells are inactive; only the outlet-column moments they feed are written.
+        R fo[Q];
         StaticFor<0, Q>([&](auto I) {
             constexpr int d = I;
             if constexpr (S::kOffset)
@@ -627,48 +702,52 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, int 
         if constexpr (!IP) {
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                PutPop(&dst[d * P + self], S::Enc(fo[d]));
+                BufStore<T>(dst_layer, vo, a.st_off[d], S::Enc(fo[d]));
             });
+        } else {
+            if (x == 1 || x == a.nx - 2)
+                WriteMoments<Q, S>(static_cast<R*>(a.side_mom), fo, par, x == 1 ? 0 : 1, a.ny * a.nz, z * a.ny + y);
         }
-        if (mom_writer) WriteMoments<Q, S>(side_mom, fo, par, mom_col, plane_yz, z * a.ny + y);
         return;
     }
 
-    const R tau = sizeof(R) == 4 ? static_cast<R>(a.tau) : static_cast<R>(a.tau_d);
-    const R uin = sizeof(R) == 4 ? static_cast<R>(a.u_in) : static_cast<R>(a.u_in_d);
-    const R cs = sizeof(R) == 4 ? static_cast<R>(a.cs) : static_cast<R>(a.cs_d);
-    const int inletX = a.reverse ? a.nx - 1 : 0;
-    const int outletX = a.reverse ? 0 : a.nx - 1;
-    const bool near = (fl & kNearSolid) != 0;
-    BRec<R>* rec = nullptr;
-
     if (x == inletX) {
-        // Equilibrium inlet (LbmReference.cpp:269-273).
-        if constexpr (S::kOffset)
-            EqOffset<Q, S>(0.0f, a.reverse ? -uin : uin, 0.0f, 0.0f, fo);
-        else
-            FeqAll<Q, R>(R(1), a.reverse ? -uin : uin, R(0), R(0), fo);
-    } else if (x == outletX) {
+        // Equilibrium inlet (LbmReference.cpp:269-273), evaluated once per launch.
+        R fo[Q];
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            if constexpr (sizeof(R) == 4)
+                fo[d] = a.inlet_f[d];
+            else
+                fo[d] = a.inlet_d[d];
+        });
+        StoreCell<Q, S, Lay, PER>(a, g, dst_layer, vo, x, y, z, seam, fo, nullptr);
+        return;
+    }
+
+    if (x == outletX) {
         // Zero-gradient outlet from the neighbour's previous post-collision moments,
         // density relaxed toward 1 (LbmReference.cpp:275-327).
         const int nbx = a.reverse ? x + 1 : x - 1;
         R nr, nu, nv, nw;
         if constexpr (!IP) {
-            const int nb = self + (nbx - x);
             R v[Q];
+            const uint32_t nvo = nbx < x ? vo - static_cast<uint32_t>(sizeof(T)) : vo + static_cast<uint32_t>(sizeof(T));
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                v[d] = S::Raw(src[d * P + nb]);
+                v[d] = S::Raw(BufLoad<T>(src_layer, nvo, a.st_off[d]));
             });
             MomentSums<Q, R>(v, nr, nu, nv, nw);
         } else {
             const int col = nbx == 1 ? 0 : 1;
-            const R* m = side_mom + ((static_cast<long long>(par ^ 1) * 2 + col) * plane_yz + z * a.ny + y) * 4;
+            const R* m = static_cast<const R*>(a.side_mom) +
+                         ((static_cast<long long>(par ^ 1) * 2 + col) * (a.ny * a.nz) + z * a.ny + y) * 4;
             nr = m[0];
             nu = m[1];
             nv = m[2];
             nw = m[3];
         }
+        R fo[Q];
         if constexpr (S::kOffset) {
             // sums of offsets: rho_n - 1 and j_n in population units
             constexpr float invS = 1.0f / S::kScale;
@@ -692,157 +771,121 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, int 
             const R rho = nr > K<R>(1e-6f, 1e-6) ? R(1) + K<R>(0.999f, 0.999) * (nr - R(1)) : R(1);
             FeqAll<Q, R>(rho, nu, nv, nw, fo);
         }
-    } else {
-        // ---------------------------------------------------------- gather
-        // f starts as the bulk pull (issued before the flag byte is known, see fs below);
-        // cells on a wall or next to a solid then correct the affected directions.
-        R f[Q];
-        StaticFor<0, Q>([&](auto I) {
-            constexpr int d = I;
-            f[d] = fs[d];
-        });
-        const bool edge = (y == 0) | (y == a.ny - 1) | (z == 0) | (z == a.nz - 1);
-        if (edge | near) {
-            uint32_t kinds[3] = {0u, 0u, 0u};
-            if (near) {
-                rec = static_cast<BRec<R>*>(a.recs) + a.bidx[self];
-                kinds[0] = rec->kind[0];
-                kinds[1] = rec->kind[1];
-                kinds[2] = rec->kind[2];
-            }
-            // Phase 1: pulls across a periodic y/z seam read the wrapped cell.
-            if (a.pery | a.perz) {
-                StaticFor<1, Q>([&](auto I) {
-                    constexpr int d = I;
-                    const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
-                    const bool wrapY = a.pery && (sy_ < 0 || sy_ >= a.ny);
-                    const bool wrapZ = a.perz && (sz_ < 0 || sz_ >= a.nz);
-                    if (!(wrapY | wrapZ)) return;
-                    if constexpr (!IP) {
-                        const int sidx = g.Nbr(x, y, z, -Lat<Q>::ex[d], -Lat<Q>::ey[d], -Lat<Q>::ez[d]);
-                        f[d] = S::Raw(src[d * P + sidx]);
-                    } else {
-                        constexpr int pa = PairAnchor<Q>(d);
-                        const int anc = g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
-                        int pl, c;
-                        LoadSlot(d, par, self, anc, pl, c);
-                        f[d] = S::Raw(dst[pl * P + c]);
-                    }
-                });
-            }
-            // Phase 2: walls and solid sources, from the cell's own previous
-            // outgoing populations (LbmReference.cpp:100-122, 336-381).
-            auto own_prev = [&](int d) -> R {
-                if constexpr (!IP) {
-                    return S::Raw(src[d * P + self]);
-                } else {
-                    const int pa = PairAnchor<Q>(d);
-                    const int anc = d == 0 ? self : g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
-                    int pl, c;
-                    StoreSlot(d, par ^ 1, self, anc, pl, c);
-                    return S::Raw(dst[pl * P + c]);
-                }
-            };
+        StoreCell<Q, S, Lay, PER>(a, g, dst_layer, vo, x, y, z, seam, fo, nullptr);
+        return;
+    }
+
+    // -------------------------------------------------------------- gather
+    // f starts as the bulk pull (issued before the flag byte is known); cells on a wall
+    // or next to a solid then correct the affected directions.
+    R f[Q];
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        f[d] = fs[d];
+    });
+    const bool near = (fl & kNearSolid) != 0;
+    const bool edge = (y == 0) | (y == a.ny - 1) | (z == 0) | (z == a.nz - 1);
+    BRec<R>* rec = nullptr;
+    if (edge | near) {
+        uint32_t kinds[3] = {0u, 0u, 0u};
+        if (near) {
+            rec = static_cast<BRec<R>*>(a.recs) + a.bidx[g.Idx(x, y, z)];
+            kinds[0] = rec->kind[0];
+            kinds[1] = rec->kind[1];
+            kinds[2] = rec->kind[2];
+        }
+        // Phase 1: pulls across a periodic y/z seam read the wrapped cell.
+        if constexpr (PER) {
             StaticFor<1, Q>([&](auto I) {
                 constexpr int d = I;
                 const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
-                const bool outY = !a.pery && (sy_ < 0 || sy_ >= a.ny);
-                const bool outZ = !a.perz && (sz_ < 0 || sz_ >= a.nz);
-                if (outY || outZ) {
-                    const bool ok = (!outY || a.fsy) && (!outZ || a.fsz);
-                    if (ok) {
-                        if (outY && outZ)
-                            f[d] = own_prev(MirrorZ<Q>(MirrorY<Q>(d)));
-                        else if (outY)
-                            f[d] = own_prev(MirrorY<Q>(d));
-                        else
-                            f[d] = own_prev(MirrorZ<Q>(d));
-                    } else {
-                        f[d] = own_prev(Opp(d));
-                    }
-                    return;
-                }
-                const unsigned kd = KindOf(kinds, d);
-                if (kd == kPull) return;
-                const R plain = own_prev(Opp(d));
-                if (kd == kPlainBB) {
-                    f[d] = plain;
-                } else if (kd == kBouzidiLo) {
-                    const R q = rec->q[d];
-                    f[d] = R(2) * q * plain + (R(1) - R(2) * q) * f[Opp(d)];
+                const bool wrapY = a.pery && (sy_ < 0 || sy_ >= a.ny);
+                const bool wrapZ = a.perz && (sz_ < 0 || sz_ >= a.nz);
+                if (!(wrapY | wrapZ)) return;
+                if constexpr (!IP) {
+                    int sx, sy2, sz2;
+                    g.NbrXYZ(x, y, z, -Lat<Q>::ex[d], -Lat<Q>::ey[d], -Lat<Q>::ez[d], sx, sy2, sz2);
+                    f[d] = S::Raw(static_cast<const T*>(a.pop_src)[g.Pop(d, sx, sy2, sz2)]);
                 } else {
-                    const R q = rec->q[d];
-                    const R inv2q = R(1) / (R(2) * q);
-                    R fi;
-                    if constexpr (!IP)
-                        fi = S::Raw(src[d * P + self]);
-                    else
-                        fi = rec->side[d];
-                    f[d] = inv2q * plain + (R(1) - inv2q) * fi;
+                    f[d] = S::Raw(static_cast<const T*>(a.pop_dst)[SlotIndex<Q, false>(g, d, par, x, y, z)]);
                 }
             });
         }
-        // ------------------------------------------------------- collide
-        bool bad;
-        if constexpr (S::kOffset)
-            bad = CollideOffset<Q, S>(f, fo, tau, cs);
-        else
-            bad = Collide<Q, R>(f, fo, tau, cs);
-        if (bad) atomicOr(a.diverged, 1);
-    }
-
-    // ---------------------------------------------------------------- store
-    const long long zoff = static_cast<long long>(z) * g.sz;
-    const uint32_t vo = static_cast<uint32_t>(self - zoff) * static_cast<uint32_t>(sizeof(T));
-    if constexpr (!IP) {
-        StaticFor<0, Q>([&](auto I) {
+        // Phase 2: walls and solid sources, from the cell's own previous outgoing
+        // populations (LbmReference.cpp:100-122, 336-381).
+        auto own_prev = [&](int d) -> R {
+            if constexpr (!IP) {
+                return S::Raw(BufLoad<T>(src_layer, vo, a.st_off[d]));
+            } else {
+                if constexpr (PER) {
+                    if (seam)
+                        return S::Raw(static_cast<const T*>(a.pop_dst)[SlotIndex<Q, true>(g, d, par ^ 1, x, y, z)]);
+                }
+                return S::Raw(BufLoad<T>(dst_layer, vo, a.prev_off[d]));
+            }
+        };
+        StaticFor<1, Q>([&](auto I) {
             constexpr int d = I;
-            StoreU<T>(ElemAddr<T>(dst, d * P + zoff), vo, S::Enc(fo[d]));
+            const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
+            const bool outY = !a.pery && (sy_ < 0 || sy_ >= a.ny);
+            const bool outZ = !a.perz && (sz_ < 0 || sz_ >= a.nz);
+            if (outY || outZ) {
+                const bool ok = (!outY || a.fsy) && (!outZ || a.fsz);
+                if (ok) {
+                    if (outY && outZ)
+                        f[d] = own_prev(MirrorZ<Q>(MirrorY<Q>(d)));
+                    else if (outY)
+                        f[d] = own_prev(MirrorY<Q>(d));
+                    else
+                        f[d] = own_prev(MirrorZ<Q>(d));
+                } else {
+                    f[d] = own_prev(Opp(d));
+                }
+                return;
+            }
+            const unsigned kd = KindOf(kinds, d);
+            if (kd == kPull) return;
+            const R plain = own_prev(Opp(d));
+            if (kd == kPlainBB) {
+                f[d] = plain;
+            } else if (kd == kBouzidiLo) {
+                const R q = rec->q[d];
+                f[d] = R(2) * q * plain + (R(1) - R(2) * q) * f[Opp(d)];
+            } else {
+                const R q = rec->q[d];
+                const R inv2q = R(1) / (R(2) * q);
+                R fi;
+                if constexpr (!IP)
+                    fi = S::Raw(BufLoad<T>(src_layer, vo, a.st_off[d]));
+                else
+                    fi = rec->side[d];
+                f[d] = inv2q * plain + (R(1) - inv2q) * fi;
+            }
         });
-    } else {
-        // Anchors reach across a seam only in the inlet/outlet columns (x wraps) and on
-        // periodic y/z edges; everywhere else they are fixed offsets from the cell (on a
-        // non-periodic wall they land in the ghost shell).
-        const bool fixed = x > 0 && x < a.nx - 1 && !(a.pery && (y == 0 || y == a.ny - 1)) &&
-                           !(a.perz && (z == 0 || z == a.nz - 1));
-        if (fixed) {
-            StaticFor<0, Q>([&](auto I) {
-                constexpr int d = I;
-                constexpr int pa = PairAnchor<Q>(d);
-                const int anc = Lat<Q>::ex[pa] + Lat<Q>::ey[pa] * g.sy + Lat<Q>::ez[pa] * g.sz;
-                int pl, c;
-                StoreSlot(d, par, 0, anc, pl, c);
-                StoreU<T>(ElemAddr<T>(dst, pl * P + zoff + c), vo, S::Enc(fo[d]));
-            });
-        } else {
-            StaticFor<0, Q>([&](auto I) {
-                constexpr int d = I;
-                constexpr int pa = PairAnchor<Q>(d);
-                int anc = self;
-                if constexpr (d != 0) anc = g.Nbr(x, y, z, Lat<Q>::ex[pa], Lat<Q>::ey[pa], Lat<Q>::ez[pa]);
-                int pl, c;
-                StoreSlot(d, par, self, anc, pl, c);
-                PutPop(&dst[pl * P + c], S::Enc(fo[d]));
-            });
-        }
-        if (rec != nullptr) {
-            const uint32_t kinds[3] = {rec->kind[0], rec->kind[1], rec->kind[2]};
-            StaticFor<1, Q>([&](auto I) {
-                constexpr int d = I;
-                if (KindOf(kinds, d) == kBouzidiHi) rec->side[d] = S::Raw(S::Enc(fo[d]));
-            });
-        }
-        if (mom_writer) WriteMoments<Q, S>(side_mom, fo, par, mom_col, plane_yz, z * a.ny + y);
     }
+    // ------------------------------------------------------------- collide
+    const R tau = sizeof(R) == 4 ? static_cast<R>(a.tau) : static_cast<R>(a.tau_d);
+    const R cs = sizeof(R) == 4 ? static_cast<R>(a.cs) : static_cast<R>(a.cs_d);
+    R fo[Q];
+    bool bad;
+    if constexpr (S::kOffset)
+        bad = CollideOffset<Q, S>(f, fo, tau, cs);
+    else
+        bad = Collide<Q, R>(f, fo, tau, cs);
+    // One store per wave, from its first diverged lane: a lattice that has blown up
+    // everywhere would otherwise serialise millions of same-address atomics.
+    const uint64_t bad_lanes = __builtin_amdgcn_ballot_w64(bad);
+    if (bad_lanes != 0 && __lane_id() == static_cast<unsigned>(__builtin_ctzll(bad_lanes)))
+        __hip_atomic_store(a.diverged, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    StoreCell<Q, S, Lay, PER>(a, g, dst_layer, vo, x, y, z, seam, fo, rec);
 }
 
-
 // One workgroup = a 64 (x) by 4*CPT (y) tile of one z layer; each thread updates CPT
-// cells 4 rows apart.  The bulk pulls of all CPT cells are issued first, so a wave keeps
-// CPT times more bytes in flight while it computes.
-template <int Q, class S, class Lay, int CPT>
+// cells 4 rows apart.  The bulk pulls of all CPT cells are issued first.
+template <int Q, class S, class Lay, int CPT, bool PER>
 __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     using R = typename S::R;
+    using T = typename S::T;
     const unsigned tile = XcdContiguous(blockIdx.x, gridDim.x);
     const unsigned tiles_x = static_cast<unsigned>((a.nx + kTileX - 1) / kTileX);
     const unsigned tiles_y = static_cast<unsigned>((a.ny + kTileY * CPT - 1) / (kTileY * CPT));
@@ -853,25 +896,30 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     const int y0 = static_cast<int>(ty * kTileY * CPT + (threadIdx.x / kTileX));
     const int z = a.z0 + static_cast<int>(rest / tiles_y);
     if (x >= a.nx) return;
-    const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2)};
+    const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2), Q};
+    // workgroup bases: padded layer z (the lower neighbour of the cells' own layer z+1)
+    const long long zb = static_cast<long long>(z) * a.layer_bytes;
+    const BufRsrc src_layer = LayerRsrc(static_cast<const char*>(Lay::kInPlace ? a.pop_dst : a.pop_src) + zb);
+    const BufRsrc dst_layer = LayerRsrc(static_cast<char*>(a.pop_dst) + zb);
     R fs[CPT][Q];
     uint8_t fl[CPT];
     // Rows past the lattice edge (a ragged last tile) load row ny-1 and are not updated.
-    const uint64_t flag_layer = reinterpret_cast<uint64_t>(a.flags) + static_cast<uint64_t>(z) * g.sz;
+    const uint64_t flag_layer = reinterpret_cast<uint64_t>(a.flags) + static_cast<uint64_t>(z + 1) * g.sz;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int y = min(y0 + k * kTileY, a.ny - 1);
-        SpecLoad<Q, S, Lay>(a, g, z, static_cast<uint32_t>(g.sz + (y + 1) * g.sy + x), fs[k]);
+        SpecLoad<Q, S>(a, src_layer, static_cast<uint32_t>((y + 1) * g.sy + x) * static_cast<uint32_t>(sizeof(T)),
+                       fs[k]);
     }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int y = min(y0 + k * kTileY, a.ny - 1);
-        fl[k] = LoadU<uint8_t>(flag_layer, static_cast<uint32_t>(g.sz + (y + 1) * g.sy + x));
+        fl[k] = LoadU<uint8_t>(flag_layer, static_cast<uint32_t>((y + 1) * g.sy + x));
     }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int y = y0 + k * kTileY;
-        if (y < a.ny) UpdateCell<Q, S, Lay>(a, g, x, y, z, fs[k], fl[k]);
+        if (y < a.ny) UpdateCell<Q, S, Lay, PER>(a, g, src_layer, dst_layer, x, y, z, fs[k], fl[k]);
     }
 }
 
