@@ -59,6 +59,13 @@ namespace {
 const char* kNoHandle = "null fc_lbm handle";
 constexpr size_t kPopMargin = 256;
 
+// Elements of one population copy: Q planes per padded z layer, nz+2 layers, and
+// kPopMargin elements past the last plane (the step's speculative bulk pull of a cell
+// at the far corner may address up to one element beyond it).
+size_t PopElements(const fc_lbm* h) {
+    return static_cast<size_t>(h->Q) * PlaneStride(h->nx, h->ny, h->tbytes) * (h->nz + 2) + kPopMargin;
+}
+
 void DevFree(void* p) {
     if (p) (void)hipFree(p);
 }
@@ -267,13 +274,14 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     h->tbytes = h->storage == FC_LBM_STORE_F64 ? 8 : (h->storage == FC_LBM_STORE_F32 ? 4 : 2);
     // The step kernel addresses three padded z layers of all planes with 32-bit byte
     // offsets from one base.
-    if (3LL * h->Q * h->nx * (h->ny + 2) * h->tbytes >= (1LL << 31))
+    if (3LL * h->Q * PlaneStride(h->nx, h->ny, h->tbytes) * h->tbytes >= (1LL << 31))
         return Fail(h, FC_LBM_EINVAL, "lattice cross-section too large (3*Q*nx*(ny+2) population bytes must be < 2 GiB)");
     h->rbytes = h->storage == FC_LBM_STORE_F64 ? 8 : 4;
     fc_lbm_params_default(&h->params);
-    // Two cells per thread keeps twice the bytes in flight per wave; it pays for the
-    // 16-bit formats, whose waves are latency-bound.  FC_LBM_CELLS_PER_THREAD overrides.
-    h->cells_per_thread = h->tbytes == 2 ? 2 : 1;
+    // Two cells per thread keeps twice the bytes in flight per wave: +2% for the 16-bit
+    // formats in ping-pong, a loss for FP32 and in place (512x256x256 measurements).
+    // FC_LBM_CELLS_PER_THREAD=1|2 overrides.
+    h->cells_per_thread = (h->tbytes == 2 && h->layout == FC_LBM_LAYOUT_PINGPONG) ? 2 : 1;
     if (const char* e = std::getenv("FC_LBM_CELLS_PER_THREAD")) h->cells_per_thread = std::atoi(e) == 2 ? 2 : 1;
     FC_HIP(h, hipGetDevice(&h->device));
     if (desc->stream) {
@@ -282,9 +290,7 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
         FC_HIP(h, hipStreamCreate(&h->stream));
         h->own_stream = true;
     }
-    // kPopMargin elements past the last plane: the step's speculative bulk pull of a
-    // cell at the far corner may address up to one element beyond it.
-    const size_t popbytes = (static_cast<size_t>(h->Q) * h->P + kPopMargin) * h->tbytes;
+    const size_t popbytes = PopElements(h) * h->tbytes;
     const int copies = h->layout == FC_LBM_LAYOUT_INPLACE ? 1 : 2;
     for (int i = 0; i < copies; ++i) {
         if (hipMalloc(&h->pops[i], popbytes) != hipSuccess) {
@@ -433,11 +439,11 @@ int fc_lbm_reset_from_field(fc_lbm* h, const float* field4) {
 // z layer below the cells' own (layer-major populations, see lbm_kernels.hpp): for
 // direction d of an interior cell at padded (x, y, 1) relative to that base.
 static void StepOffsets(const fc_lbm* h, int par, StepArgs& a) {
-    const long long LS = static_cast<long long>(h->nx) * (h->ny + 2);
+    const long long PS = PlaneStride(h->nx, h->ny, h->tbytes);
     const long long sy = h->nx;
     const long long tb = h->tbytes;
-    a.layer_bytes = h->Q * LS * tb;
-    auto elem = [&](int layer, int plane, int dy, int dx) { return (layer * h->Q + plane) * LS + dy * sy + dx; };
+    a.layer_bytes = h->Q * PS * tb;
+    auto elem = [&](int layer, int plane, int dy, int dx) { return (layer * h->Q + plane) * PS + dy * sy + dx; };
     auto lat = [&](int d, int& ex, int& ey, int& ez) {
         if (h->Q == 19) {
             ex = Lat<19>::ex[d]; ey = Lat<19>::ey[d]; ez = Lat<19>::ez[d];
@@ -746,7 +752,7 @@ long long fc_lbm_step_count(const fc_lbm* h) { return h ? h->t : -1; }
 
 size_t fc_lbm_population_bytes(const fc_lbm* h) {
     if (!h) return 0;
-    return (static_cast<size_t>(h->Q) * h->P + kPopMargin) * h->tbytes * (h->layout == FC_LBM_LAYOUT_INPLACE ? 1 : 2);
+    return PopElements(h) * h->tbytes * (h->layout == FC_LBM_LAYOUT_INPLACE ? 1 : 2);
 }
 
 void* fc_lbm_stream(const fc_lbm* h) { return h ? static_cast<void*>(h->stream) : nullptr; }

@@ -10,6 +10,8 @@
 //   pull19_4   the D3Q19 pull pattern without arithmetic: 19 SoA planes of 4-byte
 //              values, each lane reads plane d at x - e_d and writes plane d at x
 //   pull19_2   same with 2-byte values
+//   pull19a_4  pull19_4 without the x shift (every load 256-byte aligned)
+//   pull19x4_4 pull19_4 with four consecutive cells per lane (16-byte loads and stores)
 // Prints one line per kernel: bytes moved per launch and GB/s (HIP events).
 #include <hip/hip_runtime.h>
 
@@ -54,6 +56,52 @@ __global__ void KPull19(const T* __restrict__ a, T* __restrict__ b, int nx, int 
         const int src = (z + 1 - kEz[d]) * sz + (y + 1 - kEy[d]) * sy + xs;
         acc += static_cast<float>(a[d * P + src]);
         b[d * P + self] = static_cast<T>(acc);
+    }
+}
+
+// As KPull19 with the x shift dropped (kAligned) and/or four cells per lane.
+template <bool kAligned>
+__global__ void KPull19Var(const float* __restrict__ a, float* __restrict__ b, int nx, int ny, int nz, long long P) {
+    const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lin >= static_cast<long long>(nx) * ny * nz) return;
+    const int x = static_cast<int>(lin % nx);
+    const int y = static_cast<int>((lin / nx) % ny);
+    const int z = static_cast<int>(lin / (static_cast<long long>(nx) * ny));
+    const int sy = nx, sz = nx * (ny + 2);
+    const int self = (z + 1) * sz + (y + 1) * sy + x;
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        int xs = kAligned ? x : x - kEx[d];
+        xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+        const int src = (z + 1 - kEz[d]) * sz + (y + 1 - kEy[d]) * sy + xs;
+        acc += a[d * P + src];
+        b[d * P + self] = acc;
+    }
+}
+
+// Four consecutive cells per lane; the shifted loads are 4-byte aligned 16-byte loads
+// (rows are read one element early or late, staying inside the padded plane).
+__global__ void KPull19x4(const float* __restrict__ a, float* __restrict__ b, int nx, int ny, int nz, long long P) {
+    const int gx = nx / 4;
+    const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lin >= static_cast<long long>(gx) * ny * nz) return;
+    const int x = static_cast<int>(lin % gx) * 4;
+    const int y = static_cast<int>((lin / gx) % ny);
+    const int z = static_cast<int>(lin / (static_cast<long long>(gx) * ny));
+    const int sy = nx, sz = nx * (ny + 2);
+    const int self = (z + 1) * sz + (y + 1) * sy + x;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        const int src = (z + 1 - kEz[d]) * sz + (y + 1 - kEy[d]) * sy + x - kEx[d];
+        float4 v;
+        __builtin_memcpy(&v, a + d * P + src, sizeof(v));
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+        *reinterpret_cast<float4*>(b + d * P + self) = acc;
     }
 }
 
@@ -120,6 +168,14 @@ int main(int argc, char** argv) {
             ms = TimeMs([&] { hipLaunchKernelGGL(KPull19<_Float16>, dim3((cells + 255) / 256), dim3(256), 0, 0,
                                                  (const _Float16*)a, (_Float16*)b, nx, ny, nz, P); }, reps);
         report(s == 4 ? "pull19_4" : "pull19_2", 2.0 * 19 * s * cells, ms);
+        if (s == 4) {
+            ms = TimeMs([&] { hipLaunchKernelGGL(KPull19Var<true>, dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const float*)a, (float*)b, nx, ny, nz, P); }, reps);
+            report("pull19a_4", 2.0 * 19 * s * cells, ms);
+            ms = TimeMs([&] { hipLaunchKernelGGL(KPull19x4, dim3((cells / 4 + 255) / 256), dim3(256), 0, 0,
+                                                 (const float*)a, (float*)b, nx, ny, nz, P); }, reps);
+            report("pull19x4_4", 2.0 * 19 * s * cells, ms);
+        }
     }
     CHECK(hipFree(a));
     CHECK(hipFree(b));

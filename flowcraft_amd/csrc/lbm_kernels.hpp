@@ -16,12 +16,12 @@
 // kernel's macroOut); it is derived on demand from the current populations.
 //
 // HBM layout.  Layer-major SoA with a one-cell ghost shell in y and z only: the
-// Q population planes of one padded z layer are contiguous,
-//   f[((z+1)*Q + d)*LS + (y+1)*nx + x],   LS = nx*(ny+2),
-// for z in -1..nz (Q*(nz+2)*LS elements in all).  Rows stay nx long, so a
+// Q population planes of one padded z layer follow each other,
+//   f[((z+1)*Q + d)*PS + (y+1)*nx + x],   PS = nx*(ny+2) + 16 KiB / element size,
+// for z in -1..nz (Q*(nz+2)*PS elements in all).  Rows stay nx long, so a
 // 64-lane wave reading 64 consecutive x of one plane is one contiguous 256 B
 // (FP32) / 128 B (16-bit) segment; and everything a z layer's update touches
-// lies within three consecutive layers (3*Q*LS elements), so the step kernel
+// lies within three consecutive layers (3*Q*PS elements), so the step kernel
 // addresses it with 32-bit offsets from one per-workgroup base at any lattice
 // size.  Ghost cells are never updated; in the in-place layout they hold the
 // populations that leave the domain through a y/z wall, which is what makes
@@ -204,7 +204,7 @@ struct StepArgs {
     void* side_mom;  // R[2 parity][2 column][ny*nz][4]: moments of columns 1 and nx-2
     int* diverged;
     long long P;     // padded cells per plane set: nx*(ny+2)*(nz+2)
-    long long layer_bytes;  // Q*LS*sizeof(T): one padded z layer of all planes
+    long long layer_bytes;  // Q*PS*sizeof(T): one padded z layer of all planes
     uint32_t ld_off[19];    // bulk-pull byte offset of direction d from the lower layer's base
     uint32_t st_off[19];    // store byte offset of direction d (interior anchors in place)
     uint32_t prev_off[19];  // byte offset of the cell's own outgoing f_d of the previous step
@@ -218,15 +218,27 @@ struct StepArgs {
     double tau_d, u_in_d, cs_d;
 };
 
+// Population plane stride: one padded z layer's cells plus a 16 KiB gap.  The gap
+// staggers the planes of a layer against each other in the HBM address map; without it
+// the step kernel's concurrent plane streams lose about 15% of bandwidth (measured on
+// the 512x256x256 lattice, FP32 ping-pong: gaps of 0 / 4 / 8 / 12-32 KiB gave 29.6 /
+// 32.0 / 34.2 / 34.5-34.7 GLUPS, 64 KiB 33.5).
+constexpr long long kPlaneGapBytes = 16384;
+
+__host__ __device__ inline long long PlaneStride(int nx, int ny, int elem_bytes) {
+    return static_cast<long long>(nx) * (ny + 2) + kPlaneGapBytes / elem_bytes;
+}
+
 struct Geo {
     int nx, ny, nz, pery, perz;
     int sy, sz;  // padded strides of the per-cell arrays: sy = nx, sz = LS = nx*(ny+2)
     int Q;
+    long long ps;  // population plane stride, PlaneStride(nx, ny, element bytes)
     // index of a padded cell in per-cell arrays (flags, boundary-record index)
     __device__ __forceinline__ int Idx(int x, int y, int z) const { return (z + 1) * sz + (y + 1) * sy + x; }
     // element index of population plane `pl` of padded cell (x, y, z) (x may be -1 or nx)
     __device__ __forceinline__ long long Pop(int pl, int x, int y, int z) const {
-        return (static_cast<long long>(z + 1) * Q + pl) * sz + (y + 1) * sy + x;
+        return (static_cast<long long>(z + 1) * Q + pl) * ps + (y + 1) * sy + x;
     }
     // neighbour coordinates with the wrap rules of Nbr
     __device__ __forceinline__ void NbrXYZ(int x, int y, int z, int dx, int dy, int dz, int& ax, int& ay,
@@ -589,12 +601,14 @@ __device__ __forceinline__ bool CollideOffset(const float (&g)[Q], float (&go)[Q
 }
 
 // ----------------------------------------------------------- step kernel
-// Work decomposition: one workgroup = a 64 (x) by 4 (y) tile of one z layer, 256
-// threads, no integer division per thread.  Workgroups are dealt to the 8 XCDs
-// round-robin by hardware id; the id is remapped so that each XCD processes one
-// contiguous range of tiles, keeping the cache lines shared by neighbouring tiles
-// (x-shifted pulls, y/z neighbour rows) inside one XCD's L2.
-constexpr int kTileX = 64, kTileY = 4;
+// Work decomposition: one workgroup = a 256-cell run of one row (kTileX by kTileY),
+// 256 threads, no integer division per thread.  A workgroup's access to each plane is
+// one contiguous 1 KiB (FP32) / 512 B (16-bit) run, which measured faster than square
+// 64x4 tiles (FP32 ping-pong 35.3 vs 34.8 GLUPS, FP16 62.3 vs 58.4 on 512x256x256).
+// Workgroups are dealt to the 8 XCDs round-robin by hardware id; the id is remapped so
+// that each XCD processes one contiguous range of tiles, keeping the cache lines shared
+// by neighbouring tiles (x-shifted pulls, y/z neighbour rows) inside one XCD's L2.
+constexpr int kTileX = 256, kTileY = 1;
 
 __device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
     const unsigned q = n >> 3, r = n & 7u, xcd = hw & 7u, slot = hw >> 3;
@@ -674,8 +688,7 @@ __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRs
 }
 
 // Everything after the bulk pull for one cell: branch on the cell type, patch wall and
-// solid directions, collide, store.  Each branch stores its own results, so that no
-// branch's values have to be merged (and moved) on the path every wave takes.
+// solid directions, collide, store.
 template <int Q, class S, class Lay, bool PER>
 __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufRsrc src_layer, BufRsrc dst_layer,
                                            int x, int y, int z, const typename S::R (&fs)[Q], uint8_t fl) {
@@ -711,9 +724,10 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
         return;
     }
 
+    R fo[Q];
+    BRec<R>* rec = nullptr;
     if (x == inletX) {
         // Equilibrium inlet (LbmReference.cpp:269-273), evaluated once per launch.
-        R fo[Q];
         StaticFor<0, Q>([&](auto I) {
             constexpr int d = I;
             if constexpr (sizeof(R) == 4)
@@ -721,11 +735,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
             else
                 fo[d] = a.inlet_d[d];
         });
-        StoreCell<Q, S, Lay, PER>(a, g, dst_layer, vo, x, y, z, seam, fo, nullptr);
-        return;
-    }
-
-    if (x == outletX) {
+    } else if (x == outletX) {
         // Zero-gradient outlet from the neighbour's previous post-collision moments,
         // density relaxed toward 1 (LbmReference.cpp:275-327).
         const int nbx = a.reverse ? x + 1 : x - 1;
@@ -747,7 +757,6 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
             nv = m[2];
             nw = m[3];
         }
-        R fo[Q];
         if constexpr (S::kOffset) {
             // sums of offsets: rho_n - 1 and j_n in population units
             constexpr float invS = 1.0f / S::kScale;
@@ -771,117 +780,115 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
             const R rho = nr > K<R>(1e-6f, 1e-6) ? R(1) + K<R>(0.999f, 0.999) * (nr - R(1)) : R(1);
             FeqAll<Q, R>(rho, nu, nv, nw, fo);
         }
-        StoreCell<Q, S, Lay, PER>(a, g, dst_layer, vo, x, y, z, seam, fo, nullptr);
-        return;
-    }
-
-    // -------------------------------------------------------------- gather
-    // f starts as the bulk pull (issued before the flag byte is known); cells on a wall
-    // or next to a solid then correct the affected directions.
-    R f[Q];
-    StaticFor<0, Q>([&](auto I) {
-        constexpr int d = I;
-        f[d] = fs[d];
-    });
-    const bool near = (fl & kNearSolid) != 0;
-    const bool edge = (y == 0) | (y == a.ny - 1) | (z == 0) | (z == a.nz - 1);
-    BRec<R>* rec = nullptr;
-    if (edge | near) {
-        uint32_t kinds[3] = {0u, 0u, 0u};
-        if (near) {
-            rec = static_cast<BRec<R>*>(a.recs) + a.bidx[g.Idx(x, y, z)];
-            kinds[0] = rec->kind[0];
-            kinds[1] = rec->kind[1];
-            kinds[2] = rec->kind[2];
-        }
-        // Phase 1: pulls across a periodic y/z seam read the wrapped cell.
-        if constexpr (PER) {
+    } else {
+        // -------------------------------------------------------------- gather
+        // f starts as the bulk pull (issued before the flag byte is known); cells on a wall
+        // or next to a solid then correct the affected directions.
+        R f[Q];
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            f[d] = fs[d];
+        });
+        const bool near = (fl & kNearSolid) != 0;
+        const bool edge = (y == 0) | (y == a.ny - 1) | (z == 0) | (z == a.nz - 1);
+        if (edge | near) {
+            uint32_t kinds[3] = {0u, 0u, 0u};
+            if (near) {
+                rec = static_cast<BRec<R>*>(a.recs) + a.bidx[g.Idx(x, y, z)];
+                kinds[0] = rec->kind[0];
+                kinds[1] = rec->kind[1];
+                kinds[2] = rec->kind[2];
+            }
+            // Phase 1: pulls across a periodic y/z seam read the wrapped cell.
+            if constexpr (PER) {
+                StaticFor<1, Q>([&](auto I) {
+                    constexpr int d = I;
+                    const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
+                    const bool wrapY = a.pery && (sy_ < 0 || sy_ >= a.ny);
+                    const bool wrapZ = a.perz && (sz_ < 0 || sz_ >= a.nz);
+                    if (!(wrapY | wrapZ)) return;
+                    if constexpr (!IP) {
+                        int sx, sy2, sz2;
+                        g.NbrXYZ(x, y, z, -Lat<Q>::ex[d], -Lat<Q>::ey[d], -Lat<Q>::ez[d], sx, sy2, sz2);
+                        f[d] = S::Raw(static_cast<const T*>(a.pop_src)[g.Pop(d, sx, sy2, sz2)]);
+                    } else {
+                        f[d] = S::Raw(static_cast<const T*>(a.pop_dst)[SlotIndex<Q, false>(g, d, par, x, y, z)]);
+                    }
+                });
+            }
+            // Phase 2: walls and solid sources, from the cell's own previous outgoing
+            // populations (LbmReference.cpp:100-122, 336-381).
+            auto own_prev = [&](int d) -> R {
+                if constexpr (!IP) {
+                    return S::Raw(BufLoad<T>(src_layer, vo, a.st_off[d]));
+                } else {
+                    if constexpr (PER) {
+                        if (seam)
+                            return S::Raw(static_cast<const T*>(a.pop_dst)[SlotIndex<Q, true>(g, d, par ^ 1, x, y, z)]);
+                    }
+                    return S::Raw(BufLoad<T>(dst_layer, vo, a.prev_off[d]));
+                }
+            };
             StaticFor<1, Q>([&](auto I) {
                 constexpr int d = I;
                 const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
-                const bool wrapY = a.pery && (sy_ < 0 || sy_ >= a.ny);
-                const bool wrapZ = a.perz && (sz_ < 0 || sz_ >= a.nz);
-                if (!(wrapY | wrapZ)) return;
-                if constexpr (!IP) {
-                    int sx, sy2, sz2;
-                    g.NbrXYZ(x, y, z, -Lat<Q>::ex[d], -Lat<Q>::ey[d], -Lat<Q>::ez[d], sx, sy2, sz2);
-                    f[d] = S::Raw(static_cast<const T*>(a.pop_src)[g.Pop(d, sx, sy2, sz2)]);
+                const bool outY = !a.pery && (sy_ < 0 || sy_ >= a.ny);
+                const bool outZ = !a.perz && (sz_ < 0 || sz_ >= a.nz);
+                if (outY || outZ) {
+                    const bool ok = (!outY || a.fsy) && (!outZ || a.fsz);
+                    if (ok) {
+                        if (outY && outZ)
+                            f[d] = own_prev(MirrorZ<Q>(MirrorY<Q>(d)));
+                        else if (outY)
+                            f[d] = own_prev(MirrorY<Q>(d));
+                        else
+                            f[d] = own_prev(MirrorZ<Q>(d));
+                    } else {
+                        f[d] = own_prev(Opp(d));
+                    }
+                    return;
+                }
+                const unsigned kd = KindOf(kinds, d);
+                if (kd == kPull) return;
+                const R plain = own_prev(Opp(d));
+                if (kd == kPlainBB) {
+                    f[d] = plain;
+                } else if (kd == kBouzidiLo) {
+                    const R q = rec->q[d];
+                    f[d] = R(2) * q * plain + (R(1) - R(2) * q) * f[Opp(d)];
                 } else {
-                    f[d] = S::Raw(static_cast<const T*>(a.pop_dst)[SlotIndex<Q, false>(g, d, par, x, y, z)]);
+                    const R q = rec->q[d];
+                    const R inv2q = R(1) / (R(2) * q);
+                    R fi;
+                    if constexpr (!IP)
+                        fi = S::Raw(BufLoad<T>(src_layer, vo, a.st_off[d]));
+                    else
+                        fi = rec->side[d];
+                    f[d] = inv2q * plain + (R(1) - inv2q) * fi;
                 }
             });
         }
-        // Phase 2: walls and solid sources, from the cell's own previous outgoing
-        // populations (LbmReference.cpp:100-122, 336-381).
-        auto own_prev = [&](int d) -> R {
-            if constexpr (!IP) {
-                return S::Raw(BufLoad<T>(src_layer, vo, a.st_off[d]));
-            } else {
-                if constexpr (PER) {
-                    if (seam)
-                        return S::Raw(static_cast<const T*>(a.pop_dst)[SlotIndex<Q, true>(g, d, par ^ 1, x, y, z)]);
-                }
-                return S::Raw(BufLoad<T>(dst_layer, vo, a.prev_off[d]));
-            }
-        };
-        StaticFor<1, Q>([&](auto I) {
-            constexpr int d = I;
-            const int sy_ = y - Lat<Q>::ey[d], sz_ = z - Lat<Q>::ez[d];
-            const bool outY = !a.pery && (sy_ < 0 || sy_ >= a.ny);
-            const bool outZ = !a.perz && (sz_ < 0 || sz_ >= a.nz);
-            if (outY || outZ) {
-                const bool ok = (!outY || a.fsy) && (!outZ || a.fsz);
-                if (ok) {
-                    if (outY && outZ)
-                        f[d] = own_prev(MirrorZ<Q>(MirrorY<Q>(d)));
-                    else if (outY)
-                        f[d] = own_prev(MirrorY<Q>(d));
-                    else
-                        f[d] = own_prev(MirrorZ<Q>(d));
-                } else {
-                    f[d] = own_prev(Opp(d));
-                }
-                return;
-            }
-            const unsigned kd = KindOf(kinds, d);
-            if (kd == kPull) return;
-            const R plain = own_prev(Opp(d));
-            if (kd == kPlainBB) {
-                f[d] = plain;
-            } else if (kd == kBouzidiLo) {
-                const R q = rec->q[d];
-                f[d] = R(2) * q * plain + (R(1) - R(2) * q) * f[Opp(d)];
-            } else {
-                const R q = rec->q[d];
-                const R inv2q = R(1) / (R(2) * q);
-                R fi;
-                if constexpr (!IP)
-                    fi = S::Raw(BufLoad<T>(src_layer, vo, a.st_off[d]));
-                else
-                    fi = rec->side[d];
-                f[d] = inv2q * plain + (R(1) - inv2q) * fi;
-            }
-        });
+        // ------------------------------------------------------------- collide
+        const R tau = sizeof(R) == 4 ? static_cast<R>(a.tau) : static_cast<R>(a.tau_d);
+        const R cs = sizeof(R) == 4 ? static_cast<R>(a.cs) : static_cast<R>(a.cs_d);
+        bool bad;
+        if constexpr (S::kOffset)
+            bad = CollideOffset<Q, S>(f, fo, tau, cs);
+        else
+            bad = Collide<Q, R>(f, fo, tau, cs);
+        // One store per wave, from its first diverged lane: a lattice that has blown up
+        // everywhere would otherwise serialise millions of same-address atomics.
+        const uint64_t bad_lanes = __builtin_amdgcn_ballot_w64(bad);
+        if (bad_lanes != 0 && __lane_id() == static_cast<unsigned>(__builtin_ctzll(bad_lanes)))
+            __hip_atomic_store(a.diverged, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // ------------------------------------------------------------- collide
-    const R tau = sizeof(R) == 4 ? static_cast<R>(a.tau) : static_cast<R>(a.tau_d);
-    const R cs = sizeof(R) == 4 ? static_cast<R>(a.cs) : static_cast<R>(a.cs_d);
-    R fo[Q];
-    bool bad;
-    if constexpr (S::kOffset)
-        bad = CollideOffset<Q, S>(f, fo, tau, cs);
-    else
-        bad = Collide<Q, R>(f, fo, tau, cs);
-    // One store per wave, from its first diverged lane: a lattice that has blown up
-    // everywhere would otherwise serialise millions of same-address atomics.
-    const uint64_t bad_lanes = __builtin_amdgcn_ballot_w64(bad);
-    if (bad_lanes != 0 && __lane_id() == static_cast<unsigned>(__builtin_ctzll(bad_lanes)))
-        __hip_atomic_store(a.diverged, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // One store sequence for every non-solid cell: inlet and outlet lanes write together
+    // with the fluid lanes of their wave, so no cache line is written in two pieces.
     StoreCell<Q, S, Lay, PER>(a, g, dst_layer, vo, x, y, z, seam, fo, rec);
 }
 
-// One workgroup = a 64 (x) by 4*CPT (y) tile of one z layer; each thread updates CPT
-// cells 4 rows apart.  The bulk pulls of all CPT cells are issued first.
+// One workgroup = a kTileX by kTileY*CPT tile of one z layer; each thread updates CPT
+// cells kTileY rows apart.  The bulk pulls of all CPT cells are issued first.
 template <int Q, class S, class Lay, int CPT, bool PER>
 __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     using R = typename S::R;
@@ -896,7 +903,7 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     const int y0 = static_cast<int>(ty * kTileY * CPT + (threadIdx.x / kTileX));
     const int z = a.z0 + static_cast<int>(rest / tiles_y);
     if (x >= a.nx) return;
-    const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2), Q};
+    const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2), Q, PlaneStride(a.nx, a.ny, sizeof(typename S::T))};
     // workgroup bases: padded layer z (the lower neighbour of the cells' own layer z+1)
     const long long zb = static_cast<long long>(z) * a.layer_bytes;
     const BufRsrc src_layer = LayerRsrc(static_cast<const char*>(Lay::kInPlace ? a.pop_dst : a.pop_src) + zb);
