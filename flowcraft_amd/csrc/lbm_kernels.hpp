@@ -876,10 +876,13 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
             bad = CollideOffset<Q, S>(f, fo, tau, cs);
         else
             bad = Collide<Q, R>(f, fo, tau, cs);
-        // One store per wave, from its first diverged lane: a lattice that has blown up
-        // everywhere would otherwise serialise millions of same-address atomics.
+        // At most one store per wave, from its first diverged lane, and none once the
+        // sticky flag is set: a lattice that has blown up everywhere would otherwise
+        // serialise one same-address store per wave (half a million per step at 33.5M
+        // cells) in one L2 channel.
         const uint64_t bad_lanes = __builtin_amdgcn_ballot_w64(bad);
-        if (bad_lanes != 0 && __lane_id() == static_cast<unsigned>(__builtin_ctzll(bad_lanes)))
+        if (bad_lanes != 0 && __lane_id() == static_cast<unsigned>(__builtin_ctzll(bad_lanes)) &&
+            __hip_atomic_load(a.diverged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
             __hip_atomic_store(a.diverged, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     // One store sequence for every non-solid cell: inlet and outlet lanes write together

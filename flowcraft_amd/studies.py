@@ -1,0 +1,190 @@
+"""Measured studies of the step on one MI355X, beside the throughput bench.
+
+    python -m flowcraft_amd.studies precision [--steps 5000] [--every 50] [--window 1000]
+    python -m flowcraft_amd.studies large [--steps 200] [--warmup 50]
+
+precision — the 512x256x256 NACA 0012 wing (chord 128, leading edge at x = 128,
+    angle of attack 4 deg, extruded over z; D3Q19 and D3Q15 Smagorinsky Cs = 0.1,
+    u = 0.08, tau = 0.5005, free-slip y/z walls), run from the uniform rest start
+    in FP64, FP32, FP16-offset and BF16-offset storage.  The FP64 run (the GPU
+    twin of the FP64 oracle, equal to it within 1e-12 on the parity cases) is the
+    reference.  Drag and lift coefficients Cd = Fx / (0.5 u^2 A), Cl = Fy /
+    (0.5 u^2 A) with A = chord * nz (lift is along +y: the angle of attack rotates
+    the section in the x-y plane), F the FP64 momentum-exchange sum over all solid
+    cells of the current populations (LbmReference::SolidCellForce convention,
+    FlowCraft/LbmReference.cpp:127-148).  Reported: the instantaneous value at the
+    last step and the mean over the trailing `window` steps, and each storage's
+    relative error against FP64.
+
+large — the 768x512x576 bandwidth lattice (226M cells): an ellipsoid of semi-axes
+    (96, 40, 40) at (192, 256, 288) plus 64 spheres of radius uniform in [4, 12]
+    with centres uniform over the downstream half (numpy PCG64, seed 0); D3Q19
+    Smagorinsky Cs = 0.1, u = 0.06, tau = 0.501, free-slip y/z; `warmup` untimed
+    steps, then `steps` steps timed with HIP events on the solver's stream.  FP32 in
+    place (one 17 GB population copy) and FP16 offsets.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import geometry
+from .lbm import LatticeModel, Layout, LbmSolver, Params, Storage
+
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_grid():
+    return geometry.naca_wing_grid(512, 256, 256, m=0.0, p=0.0, t=0.12, chord=128.0, le_x=128.0, le_y=128.0,
+                                   aoa_deg=4.0)
+
+
+def c3_grid():
+    return geometry.ellipsoid_and_spheres_grid(768, 512, 576, semi_axes=(96, 40, 40), n_spheres=64, r_range=(4, 12),
+                                               seed=0)
+
+
+def _solver(grid, model, storage, layout):
+    s = LbmSolver()
+    if not s.Init(None, grid, model, storage=storage, layout=layout):
+        raise RuntimeError(s.LastError())
+    return s
+
+
+def force_series(solver, steps, every):
+    """Steps `steps` times, evaluating the FP64 force sums every `every` steps on the
+    device; returns an (steps // every, 4) array (Fx, Fy, Fz, maxStress)."""
+    import torch
+    n = steps // every
+    buf = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for k in range(n):
+        solver.Step(None, every)
+        solver.ForcesDevice(buf[k])
+    solver.Sync()
+    return buf.cpu().numpy()
+
+
+def precision_study(steps=5000, every=50, window=1000, models=("D3Q19", "D3Q15"), storages=("f64", "f32", "f16", "bf16"),
+                    layout=Layout.PINGPONG, log=print):
+    import torch
+    grid = c2_grid()
+    u, chord, nz = 0.08, 128.0, 256
+    q = 0.5 * u * u * chord * nz
+    params = Params(tau=0.5005, inletVelocity=u, smagorinskyCs=0.1)
+    st_map = {"f64": Storage.F64, "f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16}
+    w = max(1, window // every)
+    out = {"workload": "512x256x256 NACA 0012 chord 128 AoA 4 deg, Cs 0.1, u 0.08, tau 0.5005, free-slip y/z",
+           "steps": steps, "every": every, "window": window, "area": chord * nz, "layout": layout.name.lower(),
+           "models": {}}
+    for mname in models:
+        model = LatticeModel[mname]
+        res = {}
+        for sname in storages:
+            s = _solver(grid, model, st_map[sname], layout)
+            s.Reset(params, False)
+            t0 = time.perf_counter()
+            f = force_series(s, steps, every)
+            dt = time.perf_counter() - t0
+            div = s.HasDiverged()
+            s.Shutdown()
+            cd, cl = f[:, 0] / q, f[:, 1] / q
+            first_bad = next((k for k in range(len(cd)) if not (math.isfinite(cd[k]) and math.isfinite(cl[k]))), None)
+            res[sname] = {"cd_last": float(cd[-1]), "cl_last": float(cl[-1]), "cd_mean": float(cd[-w:].mean()),
+                          "cl_mean": float(cl[-w:].mean()), "diverged": bool(div),
+                          "first_nonfinite_step": None if first_bad is None else (first_bad + 1) * every,
+                          "wall_s": round(dt, 2)}
+            log(f"  {mname} {sname}: Cd {res[sname]['cd_mean']:.5f} Cl {res[sname]['cl_mean']:.5f} "
+                f"(trailing {window}) last Cd {cd[-1]:.5f} Cl {cl[-1]:.5f} diverged {div} {dt:.1f}s")
+            torch.cuda.empty_cache()
+        ref = res.get("f64")
+        if ref is not None:
+            for sname, r in res.items():
+                if sname == "f64":
+                    continue
+                for k in ("cd_mean", "cl_mean", "cd_last", "cl_last"):
+                    r[k + "_relerr_vs_f64"] = float(abs(r[k] - ref[k]) / max(abs(ref[k]), 1e-30))
+        out["models"][mname] = res
+    return out
+
+
+def large_lattice(steps=200, warmup=50, storages=("f32", "f16"), layout=Layout.INPLACE, log=print):
+    import torch
+    t0 = time.time()
+    grid = c3_grid()
+    log(f"  geometry {time.time() - t0:.1f}s, solid cells {int(grid.solid.sum())}")
+    params = Params(tau=0.501, inletVelocity=0.06, smagorinskyCs=0.1)
+    st_map = {"f32": (Storage.F32, 4), "f16": (Storage.F16, 2), "bf16": (Storage.BF16, 2)}
+    out = {"workload": "768x512x576 (226M cells) ellipsoid (96,40,40) + 64 spheres r 4-12 seed 0, D3Q19 Cs 0.1, "
+                       "u 0.06, tau 0.501, free-slip y/z", "steps": steps, "warmup": warmup,
+           "layout": layout.name.lower(), "runs": {}}
+    cells = grid.CellCount()
+    for sname in storages:
+        st, b = st_map[sname]
+        s = _solver(grid, LatticeModel.D3Q19, st, layout)
+        s.Reset(params, False)
+        stream = torch.cuda.ExternalStream(s.Stream())
+        s.Step(None, warmup)
+        s.Sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        s.Step(None, steps)
+        e1.record(stream)
+        s.Sync()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        mlups = cells * steps / (ms * 1e-3) / 1e6
+        gbs = mlups * 1e6 * 2 * 19 * b / 1e9
+        st_ = s.ReadbackStats()
+        r = {"mlups": round(mlups, 1), "ms_per_step": round(ms / steps, 4), "achieved_GBs": round(gbs, 1),
+             "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "population_bytes": s.PopulationBytes(),
+             "finite": not st_.hasNaN}
+        s.Shutdown()
+        torch.cuda.empty_cache()
+        out["runs"][sname] = r
+        log(f"  {sname}: {r}")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m flowcraft_amd.studies")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    p = sub.add_parser("precision")
+    p.add_argument("--steps", type=int, default=5000)
+    p.add_argument("--every", type=int, default=50)
+    p.add_argument("--window", type=int, default=1000)
+    p.add_argument("--models", nargs="+", default=["D3Q19", "D3Q15"])
+    p.add_argument("--storages", nargs="+", default=["f64", "f32", "f16", "bf16"])
+    p.add_argument("--out", default=None)
+    g = sub.add_parser("large")
+    g.add_argument("--steps", type=int, default=200)
+    g.add_argument("--warmup", type=int, default=50)
+    g.add_argument("--storages", nargs="+", default=["f32", "f16"])
+    g.add_argument("--layout", default="inplace", choices=["inplace", "pingpong"])
+    g.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    import torch
+    torch.cuda.init()
+
+    def log(m):
+        print(m, flush=True)
+
+    if a.cmd == "precision":
+        r = precision_study(a.steps, a.every, a.window, tuple(a.models), tuple(a.storages), log=log)
+    else:
+        r = large_lattice(a.steps, a.warmup, tuple(a.storages),
+                          Layout.INPLACE if a.layout == "inplace" else Layout.PINGPONG, log=log)
+    s = json.dumps(r, indent=1)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(s)
+    print(s, flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

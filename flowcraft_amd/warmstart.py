@@ -95,6 +95,7 @@ def train(net, samples, dims, u_in, epochs=20, batch=4, lr=1e-3, seed=0, time_bu
     rng = np.random.Generator(np.random.PCG64(seed))
     hist = []
     t0 = time.time()
+    t_log = t0
     for ep in range(epochs):
         order = rng.permutation(len(samples))
         losses = []
@@ -102,6 +103,12 @@ def train(net, samples, dims, u_in, epochs=20, batch=4, lr=1e-3, seed=0, time_bu
             idx = order[b:b + batch]
             x, tgt, wm = batch_tensors(samples, idx, u_in, net, dims)
             losses.append(net.train_step(x, tgt, wm, len(idx), (nz, ny, nx), lr=lr))
+            if time.time() - t_log > 45.0:
+                torch.cuda.synchronize()
+                t_log = time.time()
+                log(f"  epoch {ep + 1}: {b + batch}/{len(order)} samples ({t_log - t0:.1f}s)")
+            if time_budget_s is not None and time.time() - t0 > time_budget_s:
+                break
         torch.cuda.synchronize()
         mean = float(torch.stack(losses).mean().cpu())
         hist.append(mean)
@@ -197,7 +204,7 @@ def coarse_seed(desc, dims, div, steps, params, rng_state=None):
 
 
 def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, cs, coarse_div=10, coarse_steps=1000,
-             storage=Storage.F16, log=print):
+             storage=Storage.F16, log=print, time_budget_s=None):
     """The warm-vs-plain study on n_shapes held-out shapes.  Returns per-shape records."""
     import torch
     nx, ny, nz = dims
@@ -206,6 +213,7 @@ def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, 
     params = Params(tau=tau, inletVelocity=u_in, smagorinskyCs=cs)
     w = max(1, window // every)
     recs = []
+    t_start = time.time()
     for i in range(n_shapes):
         g, d = geometry.random_shape_grid(nx, ny, nz, rng)
         area = frontal_area(d, dims)
@@ -252,6 +260,9 @@ def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, 
         recs.append(rec)
         log(f"  eval {i + 1}/{n_shapes} {d['kind']}: Cd_ref {ref:.4f} plain {rec['plain_steps']} "
             f"warm {rec['warm_steps']} coarse {rec.get('coarse_steps')} (warm Cd {rec['warm_cd_final']:.4f})")
+        if time_budget_s is not None and time.time() - t_start > time_budget_s:
+            log(f"  evaluation time budget reached after {i + 1} shapes")
+            break
     return recs
 
 
@@ -295,3 +306,150 @@ def summarize(recs, horizon):
     if dcd:
         out["final_cd_rel_diff_warm_vs_plain"] = stats(dcd)
     return out
+
+
+# ------------------------------------------------------------------- driver
+def _logger(path):
+    f = open(path, "a") if path else None
+
+    def log(msg):
+        line = f"[{time.strftime('%H:%M:%S')}] {msg}"
+        print(line, flush=True)
+        if f is not None:
+            f.write(line + "\n")
+            f.flush()
+    return log
+
+
+def save_weights(net, path):
+    """Trained weights for inference: conv weights as their bf16 bit patterns (what the
+    kernels consume), biases and the fp32 head as float32."""
+    sd = net.state_dict()
+    out = {}
+    for k, v in sd.items():
+        if k.endswith(".w") and not k.startswith("head"):
+            b = np.ascontiguousarray(v, np.float32).view(np.uint32).astype(np.uint64)
+            out[k + ".bf16"] = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+        else:
+            out[k] = np.asarray(v, np.float32)
+    np.savez(path, **out)
+
+
+def load_weights(net, path):
+    sd = {}
+    with np.load(path, allow_pickle=False) as z:
+        for k in z.files:
+            if k.endswith(".bf16"):
+                sd[k[:-5]] = (z[k].astype(np.uint32) << 16).view(np.float32)
+            else:
+                sd[k] = z[k]
+    net.load_state_dict(sd)
+
+
+def cmd_train(args):
+    import json
+    import os
+    import torch
+    os.makedirs(args.out, exist_ok=True)
+    log = _logger(os.path.join(args.out, "train.log"))
+    dims = tuple(args.dims)
+    torch.cuda.init()
+    log(f"dataset: {args.shapes} shapes on {dims}, {args.steps} steps, u={args.u} tau={args.tau} Cs={args.cs}")
+    t0 = time.time()
+    samples, desc = generate_dataset(args.shapes, dims, args.steps, args.u, args.tau, args.cs, args.seed,
+                                     log=log, time_budget_s=args.data_budget)
+    t_data = time.time() - t0
+    log(f"dataset done: {len(samples)} shapes in {t_data:.1f}s")
+    n_val = max(1, len(samples) // 20)
+    train_set, val_set = samples[:-n_val], samples[-n_val:]
+    net = sg.Surrogate(seed=args.seed)
+    log(f"surrogate: {net.num_parameters()} parameters; training on {len(train_set)}, validating on {len(val_set)}")
+    val0 = field_l2(net, val_set, dims, args.u, range(len(val_set)))
+    t1 = time.time()
+    hist = train(net, train_set, dims, args.u, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed,
+                 time_budget_s=args.train_budget, log=log)
+    torch.cuda.synchronize()
+    t_train = time.time() - t1
+    val = field_l2(net, val_set, dims, args.u, range(len(val_set)))
+    save_weights(net, os.path.join(args.out, "surrogate.npz"))
+    rep = {"dims": dims, "shapes": len(samples), "lbm_steps": args.steps, "u_in": args.u, "tau": args.tau,
+           "cs": args.cs, "seed": args.seed, "data_s": t_data, "train_s": t_train, "epochs": len(hist),
+           "batch": args.batch, "lr": args.lr, "loss_history": hist, "parameters": net.num_parameters(),
+           "val_rel_l2_untrained": float(np.mean(val0)), "val_rel_l2": float(np.mean(val)),
+           "val_rel_l2_p95": float(np.percentile(val, 95))}
+    with open(os.path.join(args.out, "train.json"), "w") as f:
+        json.dump(rep, f, indent=1)
+    log(f"trained {len(hist)} epochs in {t_train:.1f}s; validation field rel-L2 {rep['val_rel_l2']:.4f} "
+        f"(untrained {rep['val_rel_l2_untrained']:.4f})")
+
+
+def cmd_evaluate(args):
+    import json
+    import os
+    import torch
+    os.makedirs(args.out, exist_ok=True)
+    log = _logger(os.path.join(args.out, "evaluate.log"))
+    dims = tuple(args.dims)
+    torch.cuda.init()
+    net = sg.Surrogate(seed=0)
+    load_weights(net, args.weights)
+    storage = {"f16": Storage.F16, "bf16": Storage.BF16, "f32": Storage.F32}[args.storage]
+    log(f"evaluate: {args.shapes} held-out shapes (seed {args.seed}) on {dims}, {args.storage} storage, "
+        f"horizon {args.horizon}, Cd every {args.every} averaged over {args.window} steps, tol {args.tol}")
+    recs = evaluate(net, dims, args.shapes, args.seed, args.horizon, args.every, args.window, args.tol, args.u,
+                    args.tau, args.cs, coarse_div=args.coarse_div, coarse_steps=args.coarse_steps, storage=storage,
+                    log=log, time_budget_s=args.budget)
+    rep = {"dims": dims, "storage": args.storage, "u_in": args.u, "tau": args.tau, "cs": args.cs, "seed": args.seed,
+           "horizon": args.horizon, "every": args.every, "window": args.window, "tol": args.tol,
+           "weights": os.path.basename(args.weights), "summary": summarize(recs, args.horizon), "records": recs}
+    with open(os.path.join(args.out, "evaluate.json"), "w") as f:
+        json.dump(rep, f, indent=1)
+    log("summary: " + json.dumps(rep["summary"]))
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m flowcraft_amd.warmstart",
+                                 description="Learned warm start: train the surrogate on fields this engine "
+                                             "generates, then compare warm and plain starts.")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+
+    def common(p):
+        p.add_argument("--dims", type=int, nargs=3, default=[256, 128, 128], metavar=("NX", "NY", "NZ"))
+        p.add_argument("--u", type=float, default=0.06)
+        p.add_argument("--tau", type=float, default=0.503)
+        p.add_argument("--cs", type=float, default=0.1)
+        p.add_argument("--out", required=True)
+
+    tr = sub.add_parser("train", help="generate the dataset and train")
+    common(tr)
+    tr.add_argument("--shapes", type=int, default=600)
+    tr.add_argument("--steps", type=int, default=3000)
+    tr.add_argument("--seed", type=int, default=1)
+    tr.add_argument("--epochs", type=int, default=20)
+    tr.add_argument("--batch", type=int, default=4)
+    tr.add_argument("--lr", type=float, default=1e-3)
+    tr.add_argument("--data-budget", type=float, default=None, help="seconds")
+    tr.add_argument("--train-budget", type=float, default=None, help="seconds")
+    tr.set_defaults(fn=cmd_train)
+
+    ev = sub.add_parser("evaluate", help="warm vs plain (and coarse-grid) starts on held-out shapes")
+    common(ev)
+    ev.add_argument("--weights", required=True)
+    ev.add_argument("--shapes", type=int, default=64)
+    ev.add_argument("--seed", type=int, default=1001)
+    ev.add_argument("--horizon", type=int, default=6000)
+    ev.add_argument("--every", type=int, default=10)
+    ev.add_argument("--window", type=int, default=1000)
+    ev.add_argument("--tol", type=float, default=0.01)
+    ev.add_argument("--storage", default="f16", choices=["f16", "bf16", "f32"])
+    ev.add_argument("--coarse-div", type=int, default=10)
+    ev.add_argument("--coarse-steps", type=int, default=1000)
+    ev.add_argument("--budget", type=float, default=None, help="seconds")
+    ev.set_defaults(fn=cmd_evaluate)
+    args = ap.parse_args(argv)
+    args.fn(args)
+
+
+if __name__ == "__main__":
+    main()

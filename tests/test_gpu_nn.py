@@ -3,8 +3,10 @@ oracle (oracle/nn_oracle.py), on bf16-representable inputs.
 
 Bars: convolution outputs in fp32 within 1e-4 of the FP64 result relative to the
 output scale (fp32 accumulation of exact bf16 products); bf16 outputs within 2^-7
-relative; whole-network parameter gradients (bf16 activations and activation
-gradients) within 5% relative norm of the FP64 gradients."""
+relative; each network layer, fed the GPU's own inputs, within one or two bf16
+roundings (activations) and 1e-3 relative norm (parameter gradients) of the FP64
+oracle; the free-running network's parameter gradients within 5% relative norm of
+the oracle with bf16 rounding emulated."""
 from __future__ import annotations
 
 import ctypes
@@ -197,14 +199,115 @@ def test_head_loss_colsum_adam():
     assert t.equal(pb.cpu(), p.cpu().to(t.bfloat16))
 
 
-def test_network_gradients_match_fp64_oracle():
-    """Production layer list at a small resolution (nb=2, 16x8x8).  Against the FP64
-    oracle with the GPU's bf16 rounding points emulated (quant=True): every parameter
-    gradient within 2e-2 relative norm (what remains is fp32-vs-fp64 accumulation order
-    flipping an occasional bf16 rounding); against the unquantised FP64 oracle, within
-    25% — a sanity bound only: at this tiny size the deepest levels hold 4-16 voxels and
-    bf16 activations alone move their gradients by ~10% — and the loss within 1%."""
-    from flowcraft_amd.surrogate import LAYERS, HEAD_IN, Surrogate
+def _host(tensor, shape):
+    return tensor.float().cpu().numpy().astype(np.float64).reshape(shape)
+
+
+def _within_bf16(got, ref, ulps, mag=None):
+    """Largest |got - ref| / tol, tol = ulps * 2^-8 * mag + 1e-4 * max|ref| elementwise:
+    `ulps` bf16 roundings (unit roundoff 2^-8) of values no larger than `mag` (default
+    |ref|), on top of fp32-vs-fp64 accumulation noise."""
+    mag = np.abs(ref) if mag is None else mag
+    tol = ulps * 2.0 ** -8 * mag + 1e-4 * max(np.abs(ref).max(), 1e-30)
+    return float((np.abs(got - ref) / tol).max())
+
+
+def test_network_layers_match_fp64_oracle_teacher_forced():
+    """Production layer list at a small resolution (nb=2, 16x8x8).  Every layer of the
+    GPU forward and backward is checked against the FP64 oracle applied to the GPU's own
+    inputs for that layer (its input activations, and its upstream activation gradient),
+    so each kernel's error is measured without the rounding history of the layers
+    before it.  Forward activations: within one bf16 rounding (2^-8 relative) of the
+    FP64 conv + bias + leaky ReLU.  Parameter gradients: within 1e-3 relative norm
+    (fp32 accumulation of exact bf16 products).  Activation gradients: within two bf16
+    roundings of the FP64 adjoint conv, summed over consumers in the GPU's order."""
+    from flowcraft_amd.surrogate import HEAD_IN, LAYERS, Surrogate
+    t = torch_mod()
+    net = Surrogate(seed=3)
+    nb, dims = 2, (16, 8, 8)
+    rng = np.random.default_rng(7)
+    x = bf(rng.normal(size=(nb, *dims, 8)))
+    tgt = rng.normal(size=(nb, *dims, 4)).astype(np.float32).astype(np.float64)
+    wm = (rng.random((nb, *dims)) > 0.2).astype(np.float64)
+    sd = net.state_dict()
+    params = {name: (bf(sd[f"{name}.w"]), sd[f"{name}.b"].astype(np.float64)) for name, *_ in LAYERS}
+    hw, hb = sd["head.w"].astype(np.float64), sd["head.b"].astype(np.float64)
+
+    xa = net.forward(dev_bf16(x), nb, dims)
+    V = nb * dims[0] * dims[1] * dims[2]
+    pred_dev = t.empty(V * 4, dtype=t.float32, device="cuda")
+    loss, dh = net.head(xa, nb, dims, tgt=dev_f32(tgt), wmask=dev_f32(wm), want_grad=True, pred_out=pred_dev)
+    grads = net.backward(xa, dh, nb, dims)
+    t.cuda.synchronize()
+
+    shapes = {"x": (nb, *dims, 8)}
+    for name, mode, ins, kin, c1, cout, stride, lvl in LAYERS:
+        shapes[name] = (nb, *Surrogate.level_dims(dims, lvl), cout)
+    ga = {k: _host(v, shapes[k]) for k, v in xa.items()}
+    gg = {k: _host(v, shapes[k]) for k, v in grads.items()}
+
+    # forward, layer by layer
+    for name, mode, ins, kin, c1, cout, stride, lvl in LAYERS:
+        w, b = params[name]
+        if mode == 0:
+            ref = no.leaky(no.conv_fwd(ga[ins[0]], w, b, stride))
+        else:
+            ref = no.leaky(no.conv_fwd(np.concatenate([no.up2(ga[ins[0]]), ga[ins[1]]], axis=-1), w, b, 1))
+        r = _within_bf16(ga[name], ref, 1.01)
+        print(name, f"forward {r:.3f} of tolerance")
+        assert r <= 1.0, (name, r)
+    # head, loss and the gradient it feeds back
+    pred = np.einsum("ndhwc,oc->ndhwo", ga["d0"], hw) + hb
+    assert rel(pred_dev.cpu().numpy().reshape(pred.shape), pred) < 1e-5
+    ref_loss = no.loss_fn(pred, tgt, wm)
+    assert abs(float(loss.cpu()[0]) - ref_loss) < 1e-5 * ref_loss
+    g = 2.0 * wm[..., None] * (pred - tgt) / (4.0 * V)
+    hg = net.head_gw.cpu().numpy().astype(np.float64)
+    assert rel(hg[:4 * HEAD_IN].reshape(4, HEAD_IN), np.einsum("ndhwo,ndhwc->oc", g, ga["d0"])) < 1e-4
+    assert rel(hg[4 * HEAD_IN:4 * HEAD_IN + 4], g.sum(axis=(0, 1, 2, 3))) < 1e-4
+    assert _within_bf16(gg["d0"], np.einsum("ndhwo,oc->ndhwc", g, hw), 1.01) <= 1.0
+
+    # backward, layer by layer, from the GPU's upstream gradient of each layer's output
+    contrib = {}
+    for name, mode, ins, kin, c1, cout, stride, lvl in reversed(LAYERS):
+        w, _ = params[name]
+        dz = no.bf16(no.leaky_grad(ga[name], gg[name]))
+        p = net.params[name]
+        gb = p["gb"].cpu().numpy().astype(np.float64)
+        e_b = np.linalg.norm(gb - dz.sum(axis=(0, 1, 2, 3))) / max(np.linalg.norm(dz.sum(axis=(0, 1, 2, 3))), 1e-30)
+        src = ga[ins[0]] if mode == 0 else np.concatenate([no.up2(ga[ins[0]]), ga[ins[1]]], axis=-1)
+        rw = no.conv_wgrad(src, dz, stride if mode == 0 else 1)
+        gw = p["gw"].cpu().numpy().astype(np.float64).reshape(rw.shape)
+        e_w = np.linalg.norm(gw - rw) / max(np.linalg.norm(rw), 1e-30)
+        print(name, f"wgrad {e_w:.2e}  bgrad {e_b:.2e}")
+        assert e_w < 1e-3 and e_b < 1e-3, (name, e_w, e_b)
+        if name == "e0":
+            continue
+        if mode == 0:
+            contrib.setdefault(ins[0], []).append(no.bf16(no.conv_adj(dz, w, src.shape, stride)))
+        else:
+            dcat = no.bf16(no.conv_adj(dz, w, src.shape, 1))
+            contrib.setdefault(ins[1], []).append(dcat[..., c1:])
+            contrib.setdefault(ins[0], []).append(no.bf16(no.down2_sum(dcat[..., :c1])))
+    for key, parts in contrib.items():
+        # summed in the GPU's order; each rounding is bounded by the operands' magnitude
+        acc, mag = parts[0], np.abs(parts[0])
+        for v in parts[1:]:
+            acc = no.bf16(acc + v)
+            mag = mag + np.abs(v)
+        r = _within_bf16(gg[key], acc, 2.02, mag)
+        print(key, f"data gradient {r:.3f} of tolerance")
+        assert r <= 1.0, (key, r)
+
+
+def test_network_gradients_end_to_end_sanity():
+    """The same network run freely (no teacher forcing) against the FP64 oracle, exact
+    and with the GPU's bf16 rounding points emulated.  A sanity bound only: rounding
+    flips compound through the 11 layers and, at this tiny size, the deepest levels hold
+    4-16 voxels, so the free-running gradients differ by up to a few percent from the
+    emulation (2.0e-2 measured at e3b) and ~10% from exact arithmetic.  Mis-wired
+    layers (a transposed weight, a dropped skip gradient) give O(1) errors."""
+    from flowcraft_amd.surrogate import LAYERS, Surrogate
     t = torch_mod()
     net = Surrogate(seed=3)
     nb, dims = 2, (16, 8, 8)
@@ -234,7 +337,5 @@ def test_network_gradients_match_fp64_oracle():
             ex = exact[f"{name}.{kind}"]
             e2 = np.linalg.norm(gv - ex) / max(np.linalg.norm(ex), 1e-30)
             print(name, kind, f"vs emulated {e:.3e}  vs exact {e2:.3e}")
-            assert e < 2e-2, (name, kind, e)
+            assert e < 5e-2, (name, kind, e)
             assert e2 < 2.5e-1, (name, kind, e2)
-    hg = net.head_gw.cpu().numpy()
-    assert np.linalg.norm(hg[:4 * HEAD_IN].reshape(4, HEAD_IN) - ref["head.w"]) < 1e-2 * np.linalg.norm(ref["head.w"])
