@@ -16,6 +16,7 @@
 #include "../../include/flowcraft/fc_lbm.h"
 #include "lbm_aux.hpp"
 #include "lbm_kernels.hpp"
+#include "step_launch.hpp"
 
 using namespace fc;
 
@@ -536,22 +537,12 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     const int cpt = h->cells_per_thread;
     const long long tiles = static_cast<long long>((h->nx + kTileX - 1) / kTileX) *
                             ((h->ny + kTileY * cpt - 1) / (kTileY * cpt)) * zcount;
+    hipError_t e = hipSuccess;
     Dispatch(h, [&]<int Q, class S, class Lay>() {
         InletPops<Q, S>(h, a);
-        const dim3 grid(static_cast<unsigned>(tiles));
-        if (h->pery | h->perz) {
-            if (cpt == 2)
-                hipLaunchKernelGGL((KStep<Q, S, Lay, 2, true>), grid, dim3(256), 0, h->stream, a);
-            else
-                hipLaunchKernelGGL((KStep<Q, S, Lay, 1, true>), grid, dim3(256), 0, h->stream, a);
-        } else {
-            if (cpt == 2)
-                hipLaunchKernelGGL((KStep<Q, S, Lay, 2, false>), grid, dim3(256), 0, h->stream, a);
-            else
-                hipLaunchKernelGGL((KStep<Q, S, Lay, 1, false>), grid, dim3(256), 0, h->stream, a);
-        }
+        e = LaunchStep<Q, S, Lay>(a, static_cast<unsigned>(tiles), cpt, (h->pery | h->perz) != 0, h->stream);
     });
-    FC_HIP(h, hipGetLastError());
+    FC_HIP(h, e);
     return FC_LBM_OK;
 }
 
