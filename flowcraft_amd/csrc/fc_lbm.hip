@@ -60,11 +60,13 @@ namespace {
 const char* kNoHandle = "null fc_lbm handle";
 constexpr size_t kPopMargin = 256;
 
-// Elements of one population copy: Q planes per padded z layer, nz+2 layers, and
-// kPopMargin elements past the last plane (the step's speculative bulk pull of a cell
-// at the far corner may address up to one element beyond it).
+// Elements of one population copy: the front margin, nz+2 padded layers of ny+2 rows of
+// population blocks, then one block and kPopMargin elements (the in-place anchors and the
+// step's speculative bulk pull of a cell at the far corner reach up to one block beyond
+// the last row).
 size_t PopElements(const fc_lbm* h) {
-    return static_cast<size_t>(h->Q) * PlaneStride(h->nx, h->ny, h->tbytes) * (h->nz + 2) + kPopMargin;
+    const long long rs = PopRowStride(h->nx, h->Q);
+    return static_cast<size_t>(PopFront(rs, h->Q) + rs * (h->ny + 2) * (h->nz + 2) + h->Q * kBlk + kPopMargin);
 }
 
 void DevFree(void* p) {
@@ -275,7 +277,7 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     h->tbytes = h->storage == FC_LBM_STORE_F64 ? 8 : (h->storage == FC_LBM_STORE_F32 ? 4 : 2);
     // The step kernel addresses three padded z layers of all planes with 32-bit byte
     // offsets from one base.
-    if (3LL * h->Q * PlaneStride(h->nx, h->ny, h->tbytes) * h->tbytes >= (1LL << 31))
+    if ((3LL * (h->ny + 2) + 2) * PopRowStride(h->nx, h->Q) * h->tbytes >= (1LL << 31))
         return Fail(h, FC_LBM_EINVAL, "lattice cross-section too large (3*Q*nx*(ny+2) population bytes must be < 2 GiB)");
     h->rbytes = h->storage == FC_LBM_STORE_F64 ? 8 : 4;
     fc_lbm_params_default(&h->params);
@@ -436,15 +438,20 @@ int fc_lbm_reset_from_field(fc_lbm* h, const float* field4) {
     return FC_LBM_OK;
 }
 
-// Byte offsets of the step kernel's bulk pull and stores from the base of the padded
-// z layer below the cells' own (layer-major populations, see lbm_kernels.hpp): for
-// direction d of an interior cell at padded (x, y, 1) relative to that base.
+// Byte offsets of the step kernel's bulk pull and stores from the workgroup base (the
+// padded z layer below the cells' own, lowered by StepBias elements; blocked populations,
+// see lbm_kernels.hpp): for direction d of a cell at padded (x, y, 1) relative to that
+// layer, with the x shift applied inside the cell's block (the kernel moves the two edge
+// lanes of a block across the boundary).  The bias makes every offset non-negative.
 static void StepOffsets(const fc_lbm* h, int par, StepArgs& a) {
-    const long long PS = PlaneStride(h->nx, h->ny, h->tbytes);
-    const long long sy = h->nx;
+    const long long rs = PopRowStride(h->nx, h->Q);
     const long long tb = h->tbytes;
-    a.layer_bytes = h->Q * PS * tb;
-    auto elem = [&](int layer, int plane, int dy, int dx) { return (layer * h->Q + plane) * PS + dy * sy + dx; };
+    const long long bias = StepBias(rs);
+    a.layer_bytes = (h->ny + 2) * rs * tb;
+    a.base_bytes = (PopFront(rs, h->Q) - bias) * tb;
+    auto elem = [&](int layer, int plane, int dy, int dx) {
+        return bias + layer * (h->ny + 2) * rs + dy * rs + plane * kBlk + dx;
+    };
     auto lat = [&](int d, int& ex, int& ey, int& ez) {
         if (h->Q == 19) {
             ex = Lat<19>::ex[d]; ey = Lat<19>::ey[d]; ez = Lat<19>::ez[d];

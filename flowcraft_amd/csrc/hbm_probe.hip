@@ -12,6 +12,9 @@
 //   pull19_2   same with 2-byte values
 //   pull19a_4  pull19_4 without the x shift (every load 256-byte aligned)
 //   pull19x4_4 pull19_4 with four consecutive cells per lane (16-byte loads and stores)
+//   pull19nt_4 pull19_4 with non-temporal stores
+//   blkB_4     pull19_4 on a blocked layout (runs of B cells keep their 19 populations
+//              together); blk64nt_4 with non-temporal stores
 // Prints one line per kernel: bytes moved per launch and GB/s (HIP events).
 #include <hip/hip_runtime.h>
 
@@ -105,6 +108,55 @@ __global__ void KPull19x4(const float* __restrict__ a, float* __restrict__ b, in
     }
 }
 
+// The D3Q19 pull pattern on a blocked layout: runs of BX consecutive x cells keep their
+// 19 populations together, element (d, x, y, z) at
+//   ((((z+1)*(ny+2) + y+1) * (nx/BX) + x/BX) * 19 + d) * BX + x%BX,
+// so one wave's 19 loads of its own run fall in one 19*BX*4-byte region instead of 19
+// planes.  NT: non-temporal stores.
+template <int BX, bool NT>
+__global__ void KPull19Blk(const float* __restrict__ a, float* __restrict__ b, int nx, int ny, int nz) {
+    const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lin >= static_cast<long long>(nx) * ny * nz) return;
+    const int x = static_cast<int>(lin % nx);
+    const int y = static_cast<int>((lin / nx) % ny);
+    const int z = static_cast<int>(lin / (static_cast<long long>(nx) * ny));
+    const int nb = nx / BX;
+    auto idx = [&](int d, int xx, int yy, int zz) -> long long {
+        return ((static_cast<long long>(zz + 1) * (ny + 2) + (yy + 1)) * nb + xx / BX) * (19 * BX) + d * BX + xx % BX;
+    };
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        int xs = x - kEx[d];
+        xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+        acc += a[idx(d, xs, y - kEy[d], z - kEz[d])];
+        if (NT)
+            __builtin_nontemporal_store(acc, &b[idx(d, x, y, z)]);
+        else
+            b[idx(d, x, y, z)] = acc;
+    }
+}
+
+// KPull19<float> with non-temporal stores.
+__global__ void KPull19Nt(const float* __restrict__ a, float* __restrict__ b, int nx, int ny, int nz, long long P) {
+    const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lin >= static_cast<long long>(nx) * ny * nz) return;
+    const int x = static_cast<int>(lin % nx);
+    const int y = static_cast<int>((lin / nx) % ny);
+    const int z = static_cast<int>(lin / (static_cast<long long>(nx) * ny));
+    const int sy = nx, sz = nx * (ny + 2);
+    const int self = (z + 1) * sz + (y + 1) * sy + x;
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        int xs = x - kEx[d];
+        xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+        const int src = (z + 1 - kEz[d]) * sz + (y + 1 - kEy[d]) * sy + xs;
+        acc += a[d * P + src];
+        __builtin_nontemporal_store(acc, &b[d * P + self]);
+    }
+}
+
 template <class F>
 float TimeMs(F launch, int reps) {
     hipEvent_t e0, e1;
@@ -175,6 +227,22 @@ int main(int argc, char** argv) {
             ms = TimeMs([&] { hipLaunchKernelGGL(KPull19x4, dim3((cells / 4 + 255) / 256), dim3(256), 0, 0,
                                                  (const float*)a, (float*)b, nx, ny, nz, P); }, reps);
             report("pull19x4_4", 2.0 * 19 * s * cells, ms);
+            ms = TimeMs([&] { hipLaunchKernelGGL(KPull19Nt, dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const float*)a, (float*)b, nx, ny, nz, P); }, reps);
+            report("pull19nt_4", 2.0 * 19 * s * cells, ms);
+            // blocked layouts: the same element count (nx*(ny+2)*(nz+2)*19) as the planes
+            ms = TimeMs([&] { hipLaunchKernelGGL((KPull19Blk<64, false>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const float*)a, (float*)b, nx, ny, nz); }, reps);
+            report("blk64_4", 2.0 * 19 * s * cells, ms);
+            ms = TimeMs([&] { hipLaunchKernelGGL((KPull19Blk<64, true>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const float*)a, (float*)b, nx, ny, nz); }, reps);
+            report("blk64nt_4", 2.0 * 19 * s * cells, ms);
+            ms = TimeMs([&] { hipLaunchKernelGGL((KPull19Blk<256, false>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const float*)a, (float*)b, nx, ny, nz); }, reps);
+            report("blk256_4", 2.0 * 19 * s * cells, ms);
+            ms = TimeMs([&] { hipLaunchKernelGGL((KPull19Blk<512, false>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const float*)a, (float*)b, nx, ny, nz); }, reps);
+            report("blk512_4", 2.0 * 19 * s * cells, ms);
         }
     }
     CHECK(hipFree(a));

@@ -193,7 +193,7 @@ __global__ void KSeed(void* pops0, void* pops1, long long P, long long t, int nx
     if (c >= N) return;
     int x, y, z;
     Coords(c, nx, ny, x, y, z);
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PlaneStride(nx, ny, sizeof(typename S::T))};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     R rho, ux, uy, uz;
     if (field4 == nullptr) {
         rho = static_cast<R>(rho0);
@@ -229,7 +229,7 @@ __global__ void KSetPops(void* pops0, void* pops1, long long P, long long t, int
     if (c >= N) return;
     int x, y, z;
     Coords(c, nx, ny, x, y, z);
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PlaneStride(nx, ny, sizeof(typename S::T))};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     T* f = static_cast<T*>(const_cast<void*>(CurrentCopy<Lay>(pops0, pops1, t)));
 #pragma unroll
     for (int d = 0; d < Q; ++d) f[CurrentIndex<Q, Lay>(g, t, x, y, z, d)] = S::Store(in[d * N + c], d);
@@ -246,7 +246,7 @@ __global__ void KGetPops(const void* pops0, const void* pops1, long long P, long
     if (c >= N) return;
     int x, y, z;
     Coords(c, nx, ny, x, y, z);
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PlaneStride(nx, ny, sizeof(typename S::T))};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     const bool stale = Lay::kInPlace && t > 0 && solid[c];
 #pragma unroll
     for (int d = 0; d < Q; ++d)
@@ -260,7 +260,7 @@ __global__ void KRefreshSide(const void* pops0, const void* pops1, long long P, 
                              int pery, int perz, const int* __restrict__ list, int count, BRec<typename S::R>* recs,
                              typename S::R* side_mom) {
     using R = typename S::R;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PlaneStride(nx, ny, sizeof(typename S::T))};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = 2 * ny * nz;
     if (k < count) {
@@ -304,7 +304,7 @@ __global__ void KSurfaceForce(const void* pops0, const void* pops1, long long P,
     using R = typename S::R;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsurf) return;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PlaneStride(nx, ny, sizeof(typename S::T))};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     const int n = surf[k];
     const int x = n % nx;
     const int y = (n / nx) % (ny + 2) - 1;
@@ -357,7 +357,7 @@ __global__ void KField(const void* pops0, const void* pops1, long long P, long l
     int x, y, zr;
     Coords(j, nx, ny, x, y, zr);
     const int z = z0 + zr;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PlaneStride(nx, ny, sizeof(typename S::T))};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (!(flags[g.Idx(x, y, z)] & kSolid)) {
         R rho = 0, ux = 0, uy = 0, uz = 0;
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(256) void KStatsPartial(const void* pops0, const vo
                                                      const uint8_t* __restrict__ flags, double* __restrict__ part) {
     using R = typename S::R;
     const long long N = static_cast<long long>(nx) * ny * nz;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PlaneStride(nx, ny, sizeof(typename S::T))};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     double dsum = 0, ssum = 0, cnt = 0, nan = 0;
     double mn = 1e30, mx = -1e30, ms = 0;
     for (long long c = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; c < N;
@@ -537,7 +537,7 @@ __global__ void KProbe(const void* pops0, const void* pops1, long long P, long l
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nx * ny) return;
     const int y = j / nx, x = j - y * nx;
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PlaneStride(nx, ny, sizeof(typename S::T))};
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     if (flags[g.Idx(x, y, z)] & kSolid) return;
     R rho = 0, ux = 0, uy = 0, uz = 0;
 #pragma unroll

@@ -15,17 +15,18 @@
 // reference's rounding.  The macro field is NOT written every step (the GLSL
 // kernel's macroOut); it is derived on demand from the current populations.
 //
-// HBM layout.  Layer-major SoA with a one-cell ghost shell in y and z only: the
-// Q population planes of one padded z layer follow each other,
-//   f[((z+1)*Q + d)*PS + (y+1)*nx + x],   PS = nx*(ny+2) + 16 KiB / element size,
-// for z in -1..nz (Q*(nz+2)*PS elements in all).  Rows stay nx long, so a
-// 64-lane wave reading 64 consecutive x of one plane is one contiguous 256 B
-// (FP32) / 128 B (16-bit) segment; and everything a z layer's update touches
-// lies within three consecutive layers (3*Q*PS elements), so the step kernel
-// addresses it with 32-bit offsets from one per-workgroup base at any lattice
-// size.  Ghost cells are never updated; in the in-place layout they hold the
-// populations that leave the domain through a y/z wall, which is what makes
-// wall reads race-free (see InPlace below).
+// HBM layout.  Blocked SoA with a one-cell ghost shell in y and z only: each padded
+// row of a z layer is a run of blocks of 64 consecutive x cells, and a block holds the
+// Q population planes of its 64 cells one after the other,
+//   f[front + ((z+1)*(ny+2) + (y+1))*rs + (x/64)*Q*64 + d*64 + x%64],
+//   rs = ceil(nx/64)*Q*64,
+// for z in -1..nz (see Geo::Pop).  A 64-lane wave reading 64 consecutive x of one
+// plane is one contiguous 256 B (FP32) / 128 B (16-bit) segment, and its Q planes lie
+// in one Q*256 B (Q*128 B) region; everything a z layer's update touches lies within
+// three consecutive layers, so the step kernel addresses it with 32-bit offsets from one
+// per-workgroup base at any lattice size.  Ghost cells are never updated; in the
+// in-place layout they hold the populations that leave the domain through a y/z wall,
+// which is what makes wall reads race-free (see InPlace below).
 
 #pragma once
 
@@ -204,7 +205,8 @@ struct StepArgs {
     void* side_mom;  // R[2 parity][2 column][ny*nz][4]: moments of columns 1 and nx-2
     int* diverged;
     long long P;     // padded cells per plane set: nx*(ny+2)*(nz+2)
-    long long layer_bytes;  // Q*PS*sizeof(T): one padded z layer of all planes
+    long long layer_bytes;  // (ny+2)*rs*sizeof(T): one padded z layer of population blocks
+    long long base_bytes;   // (PopFront - StepBias)*sizeof(T): workgroup base of layer z = 0
     uint32_t ld_off[19];    // bulk-pull byte offset of direction d from the lower layer's base
     uint32_t st_off[19];    // store byte offset of direction d (interior anchors in place)
     uint32_t prev_off[19];  // byte offset of the cell's own outgoing f_d of the previous step
@@ -218,27 +220,45 @@ struct StepArgs {
     double tau_d, u_in_d, cs_d;
 };
 
-// Population plane stride: one padded z layer's cells plus a 16 KiB gap.  The gap
-// staggers the planes of a layer against each other in the HBM address map; without it
-// the step kernel's concurrent plane streams lose about 15% of bandwidth (measured on
-// the 512x256x256 lattice, FP32 ping-pong: gaps of 0 / 4 / 8 / 12-32 KiB gave 29.6 /
-// 32.0 / 34.2 / 34.5-34.7 GLUPS, 64 KiB 33.5).
-constexpr long long kPlaneGapBytes = 16384;
+// Population blocks.  Runs of kBlk = 64 consecutive x cells (one wavefront) keep their Q
+// populations together: element (plane pl, padded cell x, y, z) is at
+//   front + ((z+1)*(ny+2) + (y+1)) * rs + (x >> 6) * Q*64 + pl*64 + (x & 63),
+// rs = ceil(nx/64) * Q*64 (one padded row of blocks).  A wave's 19 (15) plane accesses
+// of its own run then fall in one 19*64-element region instead of 19 planes a layer
+// apart, which keeps HBM pages open: the bare D3Q19 pull pattern moves 5.89 TB/s with
+// this layout against 5.57 TB/s on plain planes (both with non-temporal stores, MI355X).
+// x may be -1 or nx (the in-place anchors of the inlet and outlet columns): x = -1 is
+// the last slot of the previous block run, x = nx the next one — cells of the
+// neighbouring row when nx is a multiple of 64, unused padding slots otherwise; either
+// way slots that no cell owns during a step.  `front` (one block) and an equal margin
+// after the last row keep every such slot, and every speculative load of the step
+// kernel, inside the allocation.
+constexpr int kBlk = 64;
 
-__host__ __device__ inline long long PlaneStride(int nx, int ny, int elem_bytes) {
-    return static_cast<long long>(nx) * (ny + 2) + kPlaneGapBytes / elem_bytes;
+__host__ __device__ inline long long PopRowStride(int nx, int Q) {
+    return static_cast<long long>((nx + kBlk - 1) / kBlk) * Q * kBlk;
 }
+
+// Elements before row 0 of the padded layer z = -1: one padded row plus one block, so
+// that the step kernel's per-workgroup base (see StepBias) stays inside the allocation.
+__host__ __device__ inline long long PopFront(long long rs, int Q) { return rs + static_cast<long long>(Q) * kBlk; }
+
+// The step kernel's scalar offsets reach one row and one cell below a cell's own row in
+// the layer below; they are stored biased by StepBias elements (and the workgroup base
+// lowered by as much) so that every offset is non-negative.
+__host__ __device__ inline long long StepBias(long long rs) { return rs + kBlk; }
 
 struct Geo {
     int nx, ny, nz, pery, perz;
     int sy, sz;  // padded strides of the per-cell arrays: sy = nx, sz = LS = nx*(ny+2)
     int Q;
-    long long ps;  // population plane stride, PlaneStride(nx, ny, element bytes)
+    long long rs;  // population row stride, PopRowStride(nx, Q)
     // index of a padded cell in per-cell arrays (flags, boundary-record index)
     __device__ __forceinline__ int Idx(int x, int y, int z) const { return (z + 1) * sz + (y + 1) * sy + x; }
     // element index of population plane `pl` of padded cell (x, y, z) (x may be -1 or nx)
     __device__ __forceinline__ long long Pop(int pl, int x, int y, int z) const {
-        return (static_cast<long long>(z + 1) * Q + pl) * ps + (y + 1) * sy + x;
+        return PopFront(rs, Q) + (static_cast<long long>(z + 1) * (ny + 2) + (y + 1)) * rs +
+               static_cast<long long>(x >> 6) * (Q * kBlk) + pl * kBlk + (x & (kBlk - 1));
     }
     // neighbour coordinates with the wrap rules of Nbr
     __device__ __forceinline__ void NbrXYZ(int x, int y, int z, int dx, int dy, int dz, int& ax, int& ay,
@@ -261,6 +281,35 @@ struct Geo {
         return Idx(ax, ay, az);
     }
 };
+
+// Per-lane byte offsets of a cell within its z layer of population blocks (plane 0):
+// c for the cell itself; m and p for the slot one cell to the left / right when that
+// slot is in the neighbouring block (lanes 0 and 63 of a run), else equal to c.  The
+// step kernel's scalar per-direction offsets include the in-block x shift; adding m or p
+// in place of c moves the two edge lanes across the block boundary.
+struct PopLane {
+    uint32_t c, m, p;
+};
+
+template <int Q, class T>
+__device__ __forceinline__ PopLane MakePopLane(int x, int y, long long rs) {
+    const uint32_t c = static_cast<uint32_t>(((y + 1) * rs + (x >> 6) * (Q * kBlk) + (x & (kBlk - 1))) * sizeof(T));
+    constexpr uint32_t jump = static_cast<uint32_t>((Q - 1) * kBlk * sizeof(T));
+    const int xi = x & (kBlk - 1);
+    return PopLane{c, xi == 0 ? c - jump : c, xi == kBlk - 1 ? c + jump : c};
+}
+
+template <int DX>
+__device__ __forceinline__ uint32_t LaneSel(const PopLane& l) {
+    if constexpr (DX < 0)
+        return l.m;
+    else if constexpr (DX > 0)
+        return l.p;
+    else
+        return l.c;
+}
+
+__device__ __forceinline__ uint32_t LaneSelRT(const PopLane& l, int dx) { return dx < 0 ? l.m : (dx > 0 ? l.p : l.c); }
 
 // -------------------------------------------------------------- layouts
 // Ping-pong pull: read f_d(x - e_d) from the source copy, write f_d(x) into the
@@ -615,14 +664,23 @@ __device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
     return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
+// Stored form of a cell's post-collision working values.
 template <int Q, class S>
-__device__ __forceinline__ void WriteMoments(typename S::R* side_mom, const typename S::R (&fo)[Q], int par,
+__device__ __forceinline__ void EncodeCell(const typename S::R (&fo)[Q], typename S::T (&enc)[Q]) {
+    StaticFor<0, Q>([&](auto I) {
+        constexpr int d = I;
+        enc[d] = S::Enc(fo[d]);
+    });
+}
+
+template <int Q, class S>
+__device__ __forceinline__ void WriteMoments(typename S::R* side_mom, const typename S::T (&enc)[Q], int par,
                                              int col, int plane_yz, int zy) {
     using R = typename S::R;
     R v[Q];
     StaticFor<0, Q>([&](auto I) {
         constexpr int d = I;
-        v[d] = S::Raw(S::Enc(fo[d]));
+        v[d] = S::Raw(enc[d]);
     });
     R m0, m1, m2, m3;
     MomentSums<Q, R>(v, m0, m1, m2, m3);
@@ -633,17 +691,37 @@ __device__ __forceinline__ void WriteMoments(typename S::R* side_mom, const type
     m[3] = m3;
 }
 
+// x shift (-1, 0, +1) of the slot that population d is loaded from / stored to,
+// relative to the cell: the pull source x - e_d (ping-pong), or the pair anchor (in
+// place: loads of even d and stores of odd d sit at the anchor x + e_{pair}).  The
+// own-previous slot (prev_off) of the in-place layout is a store slot.
+template <int Q, class Lay>
+__host__ __device__ constexpr int LoadDx(int d) {
+    if constexpr (!Lay::kInPlace)
+        return -Lat<Q>::ex[d];
+    else
+        return (d != 0 && (d & 1) == 0) ? -Lat<Q>::ex[d] : 0;
+}
+
+template <int Q, class Lay>
+__host__ __device__ constexpr int StoreDx(int d) {
+    if constexpr (!Lay::kInPlace)
+        return 0;
+    else
+        return (d & 1) ? Lat<Q>::ex[d] : 0;
+}
+
 // Bulk pull of one cell, issued before its flag byte is known.  `src` addresses the
-// workgroup's lower z layer; `vo` is the cell's byte offset within a layer, shared by
+// workgroup's lower z layer; `ln` holds the cell's byte offsets within a layer, shared by
 // every direction.  The offsets are those of an interior cell and stay inside the
 // padded allocation for every cell of the lattice; values for cells that take another
 // branch are discarded.
-template <int Q, class S>
-__device__ __forceinline__ void SpecLoad(const StepArgs& a, BufRsrc src, uint32_t vo, typename S::R (&fs)[Q]) {
+template <int Q, class S, class Lay>
+__device__ __forceinline__ void SpecLoad(const StepArgs& a, BufRsrc src, const PopLane& ln, typename S::R (&fs)[Q]) {
     using T = typename S::T;
     StaticFor<0, Q>([&](auto I) {
         constexpr int d = I;
-        fs[d] = S::Raw(BufLoad<T>(src, vo, a.ld_off[d]));
+        fs[d] = S::Raw(BufLoad<T>(src, LaneSel<LoadDx<Q, Lay>(d)>(ln), a.ld_off[d]));
     });
 }
 
@@ -651,27 +729,29 @@ __device__ __forceinline__ void SpecLoad(const StepArgs& a, BufRsrc src, uint32_
 // other cells read next step).  `seam`: a cell on a periodic y/z edge, whose anchors
 // wrap (only instantiated for periodic lattices).
 template <int Q, class S, class Lay, bool PER>
-__device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRsrc dst_layer, uint32_t vo, int x,
+__device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRsrc dst_layer, const PopLane& ln, int x,
                                           int y, int z, bool seam, const typename S::R (&fo)[Q], BRec<typename S::R>* rec) {
     using T = typename S::T;
     using R = typename S::R;
+    T enc[Q];
+    EncodeCell<Q, S>(fo, enc);
     if constexpr (PER && Lay::kInPlace) {
         if (seam) {
             T* dst = static_cast<T*>(a.pop_dst);
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                PutPop(&dst[SlotIndex<Q, true>(g, d, a.parity, x, y, z)], S::Enc(fo[d]));
+                PutPop(&dst[SlotIndex<Q, true>(g, d, a.parity, x, y, z)], enc[d]);
             });
         } else {
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                BufStore<T>(dst_layer, vo, a.st_off[d], S::Enc(fo[d]));
+                BufStore<T>(dst_layer, LaneSel<StoreDx<Q, Lay>(d)>(ln), a.st_off[d], enc[d]);
             });
         }
     } else {
         StaticFor<0, Q>([&](auto I) {
             constexpr int d = I;
-            BufStore<T>(dst_layer, vo, a.st_off[d], S::Enc(fo[d]));
+            BufStore<T>(dst_layer, LaneSel<StoreDx<Q, Lay>(d)>(ln), a.st_off[d], enc[d]);
         });
     }
     if constexpr (Lay::kInPlace) {
@@ -679,11 +759,11 @@ __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRs
             const uint32_t kinds[3] = {rec->kind[0], rec->kind[1], rec->kind[2]};
             StaticFor<1, Q>([&](auto I) {
                 constexpr int d = I;
-                if (KindOf(kinds, d) == kBouzidiHi) rec->side[d] = S::Raw(S::Enc(fo[d]));
+                if (KindOf(kinds, d) == kBouzidiHi) rec->side[d] = S::Raw(enc[d]);
             });
         }
         if (x == 1 || x == a.nx - 2)
-            WriteMoments<Q, S>(static_cast<R*>(a.side_mom), fo, a.parity, x == 1 ? 0 : 1, a.ny * a.nz, z * a.ny + y);
+            WriteMoments<Q, S>(static_cast<R*>(a.side_mom), enc, a.parity, x == 1 ? 0 : 1, a.ny * a.nz, z * a.ny + y);
     }
 }
 
@@ -696,7 +776,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
     using R = typename S::R;
     constexpr bool IP = Lay::kInPlace;
     const int par = a.parity;
-    const uint32_t vo = static_cast<uint32_t>((y + 1) * g.sy + x) * static_cast<uint32_t>(sizeof(T));
+    const PopLane ln = MakePopLane<Q, T>(x, y, g.rs);
     const bool seam = PER && ((a.pery && (y == 0 || y == a.ny - 1)) || (a.perz && (z == 0 || z == a.nz - 1)));
     const int inletX = a.reverse ? a.nx - 1 : 0;
     const int outletX = a.reverse ? 0 : a.nx - 1;
@@ -715,11 +795,14 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
         if constexpr (!IP) {
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                BufStore<T>(dst_layer, vo, a.st_off[d], S::Enc(fo[d]));
+                BufStore<T>(dst_layer, ln.c, a.st_off[d], S::Enc(fo[d]));
             });
         } else {
-            if (x == 1 || x == a.nx - 2)
-                WriteMoments<Q, S>(static_cast<R*>(a.side_mom), fo, par, x == 1 ? 0 : 1, a.ny * a.nz, z * a.ny + y);
+            if (x == 1 || x == a.nx - 2) {
+                T enc[Q];
+                EncodeCell<Q, S>(fo, enc);
+                WriteMoments<Q, S>(static_cast<R*>(a.side_mom), enc, par, x == 1 ? 0 : 1, a.ny * a.nz, z * a.ny + y);
+            }
         }
         return;
     }
@@ -742,7 +825,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
         R nr, nu, nv, nw;
         if constexpr (!IP) {
             R v[Q];
-            const uint32_t nvo = nbx < x ? vo - static_cast<uint32_t>(sizeof(T)) : vo + static_cast<uint32_t>(sizeof(T));
+            const uint32_t nvo = nbx < x ? ln.m - static_cast<uint32_t>(sizeof(T)) : ln.p + static_cast<uint32_t>(sizeof(T));
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
                 v[d] = S::Raw(BufLoad<T>(src_layer, nvo, a.st_off[d]));
@@ -820,13 +903,13 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
             // populations (LbmReference.cpp:100-122, 336-381).
             auto own_prev = [&](int d) -> R {
                 if constexpr (!IP) {
-                    return S::Raw(BufLoad<T>(src_layer, vo, a.st_off[d]));
+                    return S::Raw(BufLoad<T>(src_layer, ln.c, a.st_off[d]));
                 } else {
                     if constexpr (PER) {
                         if (seam)
                             return S::Raw(static_cast<const T*>(a.pop_dst)[SlotIndex<Q, true>(g, d, par ^ 1, x, y, z)]);
                     }
-                    return S::Raw(BufLoad<T>(dst_layer, vo, a.prev_off[d]));
+                    return S::Raw(BufLoad<T>(dst_layer, LaneSelRT(ln, (d & 1) ? Lat<Q>::ex[d] : 0), a.prev_off[d]));
                 }
             };
             StaticFor<1, Q>([&](auto I) {
@@ -861,7 +944,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
                     const R inv2q = R(1) / (R(2) * q);
                     R fi;
                     if constexpr (!IP)
-                        fi = S::Raw(BufLoad<T>(src_layer, vo, a.st_off[d]));
+                        fi = S::Raw(BufLoad<T>(src_layer, ln.c, a.st_off[d]));
                     else
                         fi = rec->side[d];
                     f[d] = inv2q * plain + (R(1) - inv2q) * fi;
@@ -887,7 +970,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
     }
     // One store sequence for every non-solid cell: inlet and outlet lanes write together
     // with the fluid lanes of their wave, so no cache line is written in two pieces.
-    StoreCell<Q, S, Lay, PER>(a, g, dst_layer, vo, x, y, z, seam, fo, rec);
+    StoreCell<Q, S, Lay, PER>(a, g, dst_layer, ln, x, y, z, seam, fo, rec);
 }
 
 // One workgroup = a kTileX by kTileY*CPT tile of one z layer; each thread updates CPT
@@ -906,9 +989,9 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     const int y0 = static_cast<int>(ty * kTileY * CPT + (threadIdx.x / kTileX));
     const int z = a.z0 + static_cast<int>(rest / tiles_y);
     if (x >= a.nx) return;
-    const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2), Q, PlaneStride(a.nx, a.ny, sizeof(typename S::T))};
+    const Geo g{a.nx, a.ny, a.nz, a.pery, a.perz, a.nx, a.nx * (a.ny + 2), Q, PopRowStride(a.nx, Q)};
     // workgroup bases: padded layer z (the lower neighbour of the cells' own layer z+1)
-    const long long zb = static_cast<long long>(z) * a.layer_bytes;
+    const long long zb = a.base_bytes + static_cast<long long>(z) * a.layer_bytes;
     const BufRsrc src_layer = LayerRsrc(static_cast<const char*>(Lay::kInPlace ? a.pop_dst : a.pop_src) + zb);
     const BufRsrc dst_layer = LayerRsrc(static_cast<char*>(a.pop_dst) + zb);
     R fs[CPT][Q];
@@ -918,8 +1001,7 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int y = min(y0 + k * kTileY, a.ny - 1);
-        SpecLoad<Q, S>(a, src_layer, static_cast<uint32_t>((y + 1) * g.sy + x) * static_cast<uint32_t>(sizeof(T)),
-                       fs[k]);
+        SpecLoad<Q, S, Lay>(a, src_layer, MakePopLane<Q, T>(x, y, g.rs), fs[k]);
     }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {

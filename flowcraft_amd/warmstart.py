@@ -39,9 +39,38 @@ def shape_stream(seed):
     return np.random.Generator(np.random.PCG64(seed))
 
 
-def generate_dataset(n, dims, steps, u_in, tau, cs, seed, storage=Storage.F16, log=print, time_budget_s=None):
-    """Runs n random shapes; returns a list of device samples (solid, sdf, tgt, wmask) and
-    their descriptions.  dims = (nx, ny, nz)."""
+def run_to_mean_field(s, steps, avg_window, avg_every, V):
+    """Steps `steps` times from the solver's current state; returns the (rho, u) field
+    averaged over the last `avg_window` steps (sampled every `avg_every` steps; the
+    final field when avg_window is 0) as a device tensor."""
+    import torch
+    field = torch.zeros(V * 4, dtype=torch.float32, device="cuda")
+    tmp = torch.empty(V * 4, dtype=torch.float32, device="cuda")
+    torch.cuda.current_stream().synchronize()
+    start = steps - avg_window if avg_window > 0 else steps
+    s.Step(None, start)
+    n = 0
+    if avg_window > 0:
+        for _ in range(avg_window // avg_every):
+            s.Step(None, avg_every)
+            s.FieldDevice(tmp)
+            s.Sync()
+            field += tmp
+            n += 1
+        field /= n
+    else:
+        s.FieldDevice(field)
+        s.Sync()
+    return field
+
+
+def generate_dataset(n, dims, steps, u_in, tau, cs, seed, storage=Storage.F16, log=print, time_budget_s=None,
+                     avg_window=1000, avg_every=50):
+    """Runs n random shapes from the uniform rest start; returns a list of device samples
+    (solid, sdf, tgt, wmask) and their descriptions.  dims = (nx, ny, nz).  The target is
+    the (rho, u) field averaged over the last `avg_window` of the `steps` steps: the
+    acoustic waves the impulsive start leaves in the domain average out, the part of the
+    field the geometry determines remains."""
     import torch
     nx, ny, nz = dims
     rng = shape_stream(seed)
@@ -53,11 +82,7 @@ def generate_dataset(n, dims, steps, u_in, tau, cs, seed, storage=Storage.F16, l
         g, d = geometry.random_shape_grid(nx, ny, nz, rng)
         s = make_solver(g, storage)
         s.Reset(params, False)
-        s.Step(None, steps)
-        field = torch.empty(V * 4, dtype=torch.float32, device="cuda")
-        torch.cuda.current_stream().synchronize()
-        s.FieldDevice(field)
-        s.Sync()
+        field = run_to_mean_field(s, steps, avg_window, avg_every, V)
         solid = torch.from_numpy(np.ascontiguousarray(g.solid.reshape(-1))).cuda()
         sdf = torch.from_numpy(np.ascontiguousarray(g.signedDistance.reshape(-1))).cuda()
         tgt, wm = sg.targets_from_field(field, solid, u_in, V)
@@ -164,13 +189,24 @@ def trailing(x, w):
     return out
 
 
-def converged_index(cd_avg, ref, tol):
-    """First index after which |cd_avg - ref| <= tol*|ref| holds to the end, or None."""
+def converged_index(cd_avg, ref, tol, hold=None):
+    """First index from which |cd_avg - ref| <= tol*|ref| holds for the next `hold`
+    samples (to the end of the series when hold is None, or when fewer remain), or None."""
     ok = np.abs(cd_avg - ref) <= tol * abs(ref)
-    if not ok[-1]:
-        return None
-    bad = np.nonzero(~ok)[0]
-    return 0 if bad.size == 0 else int(bad[-1]) + 1
+    n = len(ok)
+    if hold is None:
+        if not ok[-1]:
+            return None
+        bad = np.nonzero(~ok)[0]
+        return 0 if bad.size == 0 else int(bad[-1]) + 1
+    # run length of consecutive in-band samples starting at each index
+    run = np.zeros(n + 1, int)
+    for i in range(n - 1, -1, -1):
+        run[i] = run[i + 1] + 1 if ok[i] else 0
+    for i in range(n):
+        if ok[i] and (run[i] >= hold or i + run[i] == n):
+            return i
+    return None
 
 
 def coarse_seed(desc, dims, div, steps, params, rng_state=None):
@@ -203,30 +239,53 @@ def coarse_seed(desc, dims, div, steps, params, rng_state=None):
     return fine.reshape(-1)
 
 
+RULES = ("hold", "horizon")
+
+
+def _arm_steps(cd, ref, tol, w, every, hold):
+    """Steps to convergence of one arm under both rules: 'hold' — the trailing-mean Cd
+    enters the band |Cd - ref| <= tol*|ref| and stays in it for `hold` steps (or to the
+    end of the run); 'horizon' — it stays in the band to the end of the run."""
+    if not np.all(np.isfinite(cd)):
+        return {r: None for r in RULES}, None
+    cda = trailing(cd, w)
+    out = {}
+    for rule in RULES:
+        k = converged_index(cda, ref, tol, hold=max(1, hold // every) if rule == "hold" else None)
+        out[rule] = None if k is None else (k + 1) * every
+    return out, cda
+
+
 def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, cs, coarse_div=10, coarse_steps=1000,
-             storage=Storage.F16, log=print, time_budget_s=None):
-    """The warm-vs-plain study on n_shapes held-out shapes.  Returns per-shape records."""
+             storage=Storage.F16, log=print, time_budget_s=None, hold=1000, series_every=50):
+    """The warm-vs-plain study on n_shapes held-out shapes.  Each shape is run from the
+    uniform rest start (plain), from the surrogate's field (warm) and from the
+    reference's coarse-grid warm start (coarse), `horizon` steps each.  The reference
+    value is the plain run's trailing-mean Cd at the horizon.  Returns per-shape records
+    with the steps to convergence under both rules and the trailing-mean Cd series."""
     import torch
     nx, ny, nz = dims
     V = nx * ny * nz
     rng = shape_stream(seed)
     params = Params(tau=tau, inletVelocity=u_in, smagorinskyCs=cs)
     w = max(1, window // every)
+    ds = max(1, series_every // every)
     recs = []
     t_start = time.time()
     for i in range(n_shapes):
         g, d = geometry.random_shape_grid(nx, ny, nz, rng)
         area = frontal_area(d, dims)
-        rec = {"kind": d["kind"], "area": area}
+        rec = {"kind": d["kind"], "area": area,
+               "shape": {k: (list(v) if isinstance(v, tuple) else v) for k, v in d.items() if k not in ("kind", "area")}}
         s = make_solver(g, storage)
         # plain arm
         s.Reset(params, False)
         cd, cl, ms = run_cd_series(s, horizon, every, area, u_in)
         cda = trailing(cd, w)
-        ref = cda[-1]
-        k = converged_index(cda, ref, tol)
-        rec.update(cd_ref=float(ref), cl_ref=float(trailing(cl, w)[-1]), ms_per_step=ms,
-                   plain_steps=None if k is None else (k + 1) * every)
+        ref = float(cda[-1])
+        steps_p, _ = _arm_steps(cd, ref, tol, w, every, hold)
+        rec.update(cd_ref=ref, cl_ref=float(trailing(cl, w)[-1]), ms_per_step=ms, plain=steps_p,
+                   plain_series=[float(v) for v in cda[ds - 1::ds]])
         # learned arm
         solid = torch.from_numpy(np.ascontiguousarray(g.solid.reshape(-1))).cuda()
         sdf = torch.from_numpy(np.ascontiguousarray(g.signedDistance.reshape(-1))).cuda()
@@ -240,11 +299,12 @@ def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, 
         s.SetParams(params)
         s.ResetFromField(seed_field)
         cdw, clw, _ = run_cd_series(s, horizon, every, area, u_in)
-        cdwa = trailing(cdw, w)
-        kw = None if not np.all(np.isfinite(cdw)) else converged_index(cdwa, ref, tol)
-        rec.update(infer_ms=infer_ms, warm_steps=None if kw is None else (kw + 1) * every,
-                   warm_cd_final=float(cdwa[-1]), warm_cl_final=float(trailing(clw, w)[-1]))
-        # reference coarse-grid arm
+        steps_w, cdwa = _arm_steps(cdw, ref, tol, w, every, hold)
+        rec.update(infer_ms=infer_ms, warm=steps_w,
+                   warm_cd_final=None if cdwa is None else float(cdwa[-1]),
+                   warm_cl_final=float(trailing(clw, w)[-1]) if np.all(np.isfinite(clw)) else None,
+                   warm_series=None if cdwa is None else [float(v) for v in cdwa[ds - 1::ds]])
+        # the reference's coarse-grid warm start
         try:
             t0 = time.perf_counter()
             cseed = coarse_seed(d, dims, coarse_div, coarse_steps, params)
@@ -252,14 +312,16 @@ def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, 
             s.SetParams(params)
             s.ResetFromField(cseed)
             cdc, _, _ = run_cd_series(s, horizon, every, area, u_in)
-            kc = None if not np.all(np.isfinite(cdc)) else converged_index(trailing(cdc, w), ref, tol)
-            rec.update(coarse_ms=coarse_ms, coarse_steps=None if kc is None else (kc + 1) * every)
+            steps_c, cdca = _arm_steps(cdc, ref, tol, w, every, hold)
+            rec.update(coarse_ms=coarse_ms, coarse=steps_c,
+                       coarse_series=None if cdca is None else [float(v) for v in cdca[ds - 1::ds]])
         except Exception as e:  # the comparison arm must not stop the study
-            rec.update(coarse_error=str(e))
+            rec.update(coarse_error=str(e), coarse={r: None for r in RULES})
         s.Shutdown()
         recs.append(rec)
-        log(f"  eval {i + 1}/{n_shapes} {d['kind']}: Cd_ref {ref:.4f} plain {rec['plain_steps']} "
-            f"warm {rec['warm_steps']} coarse {rec.get('coarse_steps')} (warm Cd {rec['warm_cd_final']:.4f})")
+        log(f"  eval {i + 1}/{n_shapes} {d['kind']}: Cd_ref {ref:.4f}  hold-rule steps plain {steps_p['hold']} "
+            f"warm {steps_w['hold']} coarse {rec['coarse']['hold']}  horizon-rule plain {steps_p['horizon']} "
+            f"warm {steps_w['horizon']} coarse {rec['coarse']['horizon']}  (warm final Cd {rec['warm_cd_final']})")
         if time_budget_s is not None and time.time() - t_start > time_budget_s:
             log(f"  evaluation time budget reached after {i + 1} shapes")
             break
@@ -267,44 +329,58 @@ def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, 
 
 
 def summarize(recs, horizon):
-    """Steps/wall time to convergence (mean, p50, p95) per arm, speedups, fallbacks."""
+    """Per rule: steps and wall time to convergence (mean, p50, p95) of the plain and
+    warm arms, the wall-time speedup, the number of warm runs that fell back to the plain
+    start (no convergence within the horizon: charged the plain run's time), and the
+    coarse-grid arm; plus the final Cd difference between warm and plain runs."""
     def stats(v):
         v = np.asarray(v, float)
         return {"mean": float(v.mean()), "p50": float(np.percentile(v, 50)), "p95": float(np.percentile(v, 95))}
 
-    plain_s, warm_s, plain_t, warm_t, coarse_s = [], [], [], [], []
-    fallbacks = 0
-    unconverged_plain = 0
-    for r in recs:
-        ps = r["plain_steps"]
-        if ps is None:
-            unconverged_plain += 1
-            continue
-        ws = r["warm_steps"]
-        ms = r["ms_per_step"]
-        if ws is None:
-            fallbacks += 1
-            ws_eff, wt = ps, ps * ms + r["infer_ms"]
-        else:
-            ws_eff, wt = ws, ws * ms + r["infer_ms"]
-        plain_s.append(ps)
-        warm_s.append(ws_eff)
-        plain_t.append(ps * ms)
-        warm_t.append(wt)
-        if r.get("coarse_steps") is not None:
-            coarse_s.append(r["coarse_steps"])
-    out = {"shapes": len(recs), "plain_unconverged_at_horizon": unconverged_plain, "fallbacks": fallbacks,
-           "horizon_steps": horizon}
-    if plain_s:
-        sp = np.asarray(plain_t) / np.asarray(warm_t)
-        out.update(plain_steps=stats(plain_s), warm_steps=stats(warm_s), plain_ms=stats(plain_t),
-                   warm_ms=stats(warm_t), speedup_wall=stats(sp),
-                   speedup_wall_of_means=float(np.mean(plain_t) / np.mean(warm_t)))
-    if coarse_s:
-        out["coarse_steps"] = stats(coarse_s)
-    dcd = [abs(r["warm_cd_final"] - r["cd_ref"]) / abs(r["cd_ref"]) for r in recs if r["cd_ref"] != 0]
+    out = {"shapes": len(recs), "horizon_steps": horizon}
+    for rule in RULES:
+        plain_s, warm_s, plain_t, warm_t, coarse_s, coarse_t = [], [], [], [], [], []
+        fallbacks = unconverged_plain = coarse_fallbacks = 0
+        for r in recs:
+            ps = r["plain"][rule]
+            if ps is None:
+                unconverged_plain += 1
+                continue
+            ms = r["ms_per_step"]
+            ws = r["warm"][rule]
+            if ws is None:
+                fallbacks += 1
+                ws_eff, wt = ps, ps * ms + r["infer_ms"]
+            else:
+                ws_eff, wt = ws, ws * ms + r["infer_ms"]
+            plain_s.append(ps)
+            warm_s.append(ws_eff)
+            plain_t.append(ps * ms)
+            warm_t.append(wt)
+            cs_ = r["coarse"][rule]
+            if cs_ is None:
+                coarse_fallbacks += 1
+                coarse_s.append(ps)
+                coarse_t.append(ps * ms + r.get("coarse_ms", 0.0))
+            else:
+                coarse_s.append(cs_)
+                coarse_t.append(cs_ * ms + r.get("coarse_ms", 0.0))
+        o = {"plain_unconverged": unconverged_plain, "warm_fallbacks": fallbacks,
+             "coarse_fallbacks": coarse_fallbacks}
+        if plain_s:
+            o.update(plain_steps=stats(plain_s), warm_steps=stats(warm_s), plain_ms=stats(plain_t),
+                     warm_ms=stats(warm_t), speedup_wall=stats(np.asarray(plain_t) / np.asarray(warm_t)),
+                     speedup_wall_of_means=float(np.mean(plain_t) / np.mean(warm_t)),
+                     coarse_steps=stats(coarse_s), coarse_ms=stats(coarse_t),
+                     coarse_speedup_wall_of_means=float(np.mean(plain_t) / np.mean(coarse_t)))
+        out[rule] = o
+    dcd = [abs(r["warm_cd_final"] - r["cd_ref"]) / abs(r["cd_ref"]) for r in recs
+           if r["cd_ref"] != 0 and r["warm_cd_final"] is not None]
     if dcd:
         out["final_cd_rel_diff_warm_vs_plain"] = stats(dcd)
+    dcl = [abs(r["warm_cl_final"] - r["cl_ref"]) for r in recs if r["warm_cl_final"] is not None]
+    if dcl:
+        out["final_cl_abs_diff_warm_vs_plain"] = stats(dcl)
     return out
 
 
@@ -357,7 +433,7 @@ def cmd_train(args):
     log(f"dataset: {args.shapes} shapes on {dims}, {args.steps} steps, u={args.u} tau={args.tau} Cs={args.cs}")
     t0 = time.time()
     samples, desc = generate_dataset(args.shapes, dims, args.steps, args.u, args.tau, args.cs, args.seed,
-                                     log=log, time_budget_s=args.data_budget)
+                                     log=log, time_budget_s=args.data_budget, avg_window=args.target_window)
     t_data = time.time() - t0
     log(f"dataset done: {len(samples)} shapes in {t_data:.1f}s")
     n_val = max(1, len(samples) // 20)
@@ -373,7 +449,7 @@ def cmd_train(args):
     val = field_l2(net, val_set, dims, args.u, range(len(val_set)))
     save_weights(net, os.path.join(args.out, "surrogate.npz"))
     rep = {"dims": dims, "shapes": len(samples), "lbm_steps": args.steps, "u_in": args.u, "tau": args.tau,
-           "cs": args.cs, "seed": args.seed, "data_s": t_data, "train_s": t_train, "epochs": len(hist),
+           "cs": args.cs, "seed": args.seed, "target_window": args.target_window, "data_s": t_data, "train_s": t_train, "epochs": len(hist),
            "batch": args.batch, "lr": args.lr, "loss_history": hist, "parameters": net.num_parameters(),
            "val_rel_l2_untrained": float(np.mean(val0)), "val_rel_l2": float(np.mean(val)),
            "val_rel_l2_p95": float(np.percentile(val, 95))}
@@ -398,9 +474,9 @@ def cmd_evaluate(args):
         f"horizon {args.horizon}, Cd every {args.every} averaged over {args.window} steps, tol {args.tol}")
     recs = evaluate(net, dims, args.shapes, args.seed, args.horizon, args.every, args.window, args.tol, args.u,
                     args.tau, args.cs, coarse_div=args.coarse_div, coarse_steps=args.coarse_steps, storage=storage,
-                    log=log, time_budget_s=args.budget)
+                    log=log, time_budget_s=args.budget, hold=args.hold)
     rep = {"dims": dims, "storage": args.storage, "u_in": args.u, "tau": args.tau, "cs": args.cs, "seed": args.seed,
-           "horizon": args.horizon, "every": args.every, "window": args.window, "tol": args.tol,
+           "horizon": args.horizon, "every": args.every, "window": args.window, "tol": args.tol, "hold": args.hold,
            "weights": os.path.basename(args.weights), "summary": summarize(recs, args.horizon), "records": recs}
     with open(os.path.join(args.out, "evaluate.json"), "w") as f:
         json.dump(rep, f, indent=1)
@@ -429,6 +505,8 @@ def main(argv=None):
     tr.add_argument("--epochs", type=int, default=20)
     tr.add_argument("--batch", type=int, default=4)
     tr.add_argument("--lr", type=float, default=1e-3)
+    tr.add_argument("--target-window", type=int, default=1000,
+                    help="steps at the end of each run the target field is averaged over (0: last field)")
     tr.add_argument("--data-budget", type=float, default=None, help="seconds")
     tr.add_argument("--train-budget", type=float, default=None, help="seconds")
     tr.set_defaults(fn=cmd_train)
@@ -442,6 +520,7 @@ def main(argv=None):
     ev.add_argument("--every", type=int, default=10)
     ev.add_argument("--window", type=int, default=1000)
     ev.add_argument("--tol", type=float, default=0.01)
+    ev.add_argument("--hold", type=int, default=1000, help="steps a run must stay in the band (hold rule)")
     ev.add_argument("--storage", default="f16", choices=["f16", "bf16", "f32"])
     ev.add_argument("--coarse-div", type=int, default=10)
     ev.add_argument("--coarse-steps", type=int, default=1000)

@@ -316,3 +316,55 @@ def test_cells_per_thread_variants_agree(storage, layout, monkeypatch):
         assert rel_err(out["2"][0][:, fluid], orc.pops()[:, fluid]) <= 1e-12
     else:
         _fields_close(out["2"][1], orc.field(), 1e-5, grid.solid)
+
+
+def _sphere_c1_grid():
+    """128x64x64 lattice, sphere of radius 8 cells centred at (32, 32, 32), SDF as in the
+    reference test (tests/gpu_vs_cpu_test.cpp:29-41)."""
+    from flowcraft_amd import geometry
+    return geometry.sphere_grid(128, 64, 64, (32, 32, 32), 8)
+
+
+C1_PARAMS = Params(tau=0.56, inletVelocity=0.05, smagorinskyCs=0.0)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: l.name)
+def test_sphere_128x64x64_1000_steps_matches_reference(layout):
+    """The plain regularized BGK sphere workload (D3Q19, Cs = 0, tau = 0.56, u = 0.05,
+    rest start) run 1000 steps in FP32 against the oracle — bit-exact with FlowCraft's
+    LbmReference — with the reference's free-slip y/z walls: field within 1e-5 per
+    channel, force sums and total mass within 1e-5; the drag coefficient's error
+    against the FP64 oracle is printed."""
+    grid = _sphere_c1_grid()
+    eng, orc = _run_pair(grid, C1_PARAMS, LatticeModel.D3Q19, Storage.F32, layout, steps=1000, seed_mid=False)
+    g, o = eng.ReadbackField(), orc.field()
+    err = _fields_close(g, o, 1e-5, grid.solid)
+    s, so = eng.ReadbackStats(), orc.stats()
+    assert abs(s.forceX - so["forceX"]) <= 1e-5 * abs(so["forceX"])
+    assert abs(s.totalMass - so["totalMass"]) <= 1e-5 * so["totalMass"]
+    o64 = make_oracle(grid, LatticeModel.D3Q19, Storage.F64)
+    oracle_params(o64, C1_PARAMS)
+    o64.reset(False)
+    o64.step(1000)
+    s64 = o64.stats()
+    q = 0.5 * 0.05 ** 2 * np.pi * 8 ** 2
+    print(f"field err vs FP32 oracle {err:.2e}, max ulp {ulp_diff32(g, o)}; Cd {s.forceX / q:.6f} vs FP64 oracle "
+          f"{s64['forceX'] / q:.6f} (rel {abs(s.forceX - s64['forceX']) / abs(s64['forceX']):.2e})")
+    assert abs(s.forceX - s64["forceX"]) <= 1e-3 * abs(s64["forceX"])
+
+
+def test_sphere_128x64x64_periodic_1000_steps():
+    """The same workload with periodic y/z (an extension: the reference has only walls),
+    in place, against the oracle's periodic wrap in FP32 (within 1e-5) and FP64 (drag
+    within 1e-3)."""
+    grid = _sphere_c1_grid()
+    eng, orc = _run_pair(grid, C1_PARAMS, LatticeModel.D3Q19, Storage.F32, Layout.INPLACE, steps=1000,
+                         seed_mid=False, periodic=(True, True))
+    _fields_close(eng.ReadbackField(), orc.field(), 1e-5, grid.solid)
+    o64 = make_oracle(grid, LatticeModel.D3Q19, Storage.F64, periodic=(True, True))
+    oracle_params(o64, C1_PARAMS)
+    o64.reset(False)
+    o64.step(1000)
+    f, f64 = eng.Forces()[0], o64.stats()["forceX"]
+    print(f"periodic Fx {f:.6e} vs FP64 oracle {f64:.6e}")
+    assert abs(f - f64) <= 1e-3 * abs(f64)
