@@ -17,6 +17,13 @@ both sides.  The roofline uses the algorithmic 2*Q*s bytes per cell update.
 The CPU baseline times FlowCraft's LbmReference::Step (oracle/_ref) — or the
 oracle's restatement where that build is absent — on the host cores, over a
 bounded number of steps of the same lattice.
+
+`warm_start` is the metric's second half, measured live: on --warm-shapes held-out
+shapes of the warm-start distribution (256x128x128, u = 0.06, tau = 0.503, FP16
+storage), steps and wall time until the trailing-mean drag coefficient is within 1%
+of the plain start's value at 6000 steps, for the plain start, the learned warm
+start (committed surrogate weights, inference included) and FlowCraft's coarse-grid
+warm start.
 """
 from __future__ import annotations
 
@@ -25,6 +32,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -105,6 +114,37 @@ def _cpu_baseline_run(oracle, grid, p, cores, budget_s, max_steps):
                       f"process pinned to {cores} host cores"}
 
 
+def warm_start_block(torch, n_shapes):
+    """Live warm-vs-plain comparison on held-out shapes of the warm-start distribution
+    (256x128x128 D3Q19 Smagorinsky Cs = 0.1, u = 0.06, tau = 0.503, FP16 storage, 6000-step
+    horizon), with the committed surrogate weights: steps and wall time until the
+    trailing-1000-step mean Cd is within 1% of the plain start's value at the horizon,
+    under the 'reach', 'hold' (1000 steps) and 'horizon' (stays to the end) rules."""
+    from flowcraft_amd import surrogate as sg
+    from flowcraft_amd import warmstart as ws
+    net = sg.Surrogate(seed=0)
+    ws.load_weights(net, ws.DEFAULT_WEIGHTS)
+    t0 = time.time()
+    recs = ws.evaluate(net, (256, 128, 128), n_shapes, seed=1001, horizon=6000, every=10, window=1000, tol=0.01,
+                       u_in=0.06, tau=0.503, cs=0.1, log=lambda m: print("#" + m, file=sys.stderr, flush=True))
+    summ = ws.summarize(recs, 6000)
+    out = {"workload": "256x128x128 D3Q19 Cs 0.1, u 0.06, tau 0.503, FP16 storage; held-out shapes seed 1001; "
+                       "surrogate trained on 600 engine-generated fields (3000 steps, mean of the last 1000)",
+           "shapes": len(recs), "wall_s": round(time.time() - t0, 1),
+           "infer_ms_mean": round(float(np.mean([r["infer_ms"] for r in recs])), 2)}
+    for rule in ws.RULES:
+        o = summ[rule]
+        if "plain_steps" not in o:
+            out[rule] = o
+            continue
+        out[rule] = {"plain_steps_p50": o["plain_steps"]["p50"], "warm_steps_p50": o["warm_steps"]["p50"],
+                     "coarse_steps_p50": o["coarse_steps"]["p50"], "speedup_wall_p50": round(o["speedup_wall"]["p50"], 3),
+                     "speedup_wall_of_means": round(o["speedup_wall_of_means"], 3),
+                     "coarse_speedup_wall_of_means": round(o["coarse_speedup_wall_of_means"], 3),
+                     "warm_fallbacks": o["warm_fallbacks"], "coarse_fallbacks": o["coarse_fallbacks"]}
+    return out
+
+
 def load_traffic(name):
     """Per-launch HBM bytes of the step kernel from the committed rocprofv3 --pmc
     summaries (profiles/), if they exist for this workload."""
@@ -123,6 +163,8 @@ def main():
     ap.add_argument("--layout", default="pingpong", choices=["inplace", "pingpong"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--warm-shapes", type=int, default=8,
+                    help="held-out shapes of the live warm-vs-plain comparison (0 skips it)")
     ap.add_argument("--profile", action="store_true",
                     help="only the FP32 D3Q19 step kernel for --steps steps (for rocprofv3 runs); "
                          "prints the HIP-event average step time")
@@ -179,6 +221,7 @@ def main():
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "algorithmic_bytes_per_launch": bytes_per_cell * grid.CellCount(),
                 "kernel": f"fc::KStep<19, StoreF32, {layout.name.title().replace('place', 'Place')}>"}
+    warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(grid)
     line = {
         "metric": METRIC, "value": round(main_r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
@@ -189,7 +232,8 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "finite": main_r["finite"],
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "warm_start": warm,
+        "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)
 

@@ -19,6 +19,7 @@ fine grid (FlowCraft/main.cpp:1413-1564) — is the third arm.
 from __future__ import annotations
 
 import math
+import os
 import time
 
 import numpy as np
@@ -26,6 +27,11 @@ import numpy as np
 from . import geometry
 from .lbm import LatticeModel, Layout, LbmSolver, Params, Storage
 from . import surrogate as sg
+
+
+# Surrogate weights trained on the 256x128x128 warm-start distribution (600 engine-generated
+# fields, 3000 steps each, targets the mean of the last 1000 steps; `train` below).
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "surrogate_256x128x128.npz")
 
 
 def make_solver(grid, storage=Storage.F16, layout=Layout.PINGPONG):
@@ -239,19 +245,21 @@ def coarse_seed(desc, dims, div, steps, params, rng_state=None):
     return fine.reshape(-1)
 
 
-RULES = ("hold", "horizon")
+RULES = ("reach", "hold", "horizon")
 
 
 def _arm_steps(cd, ref, tol, w, every, hold):
-    """Steps to convergence of one arm under both rules: 'hold' — the trailing-mean Cd
-    enters the band |Cd - ref| <= tol*|ref| and stays in it for `hold` steps (or to the
-    end of the run); 'horizon' — it stays in the band to the end of the run."""
+    """Steps to convergence of one arm under three rules, with the band
+    |Cd - ref| <= tol*|ref| on the trailing-mean Cd: 'horizon' — it stays in the band to
+    the end of the run (the literal rule); 'hold' — it enters the band and stays for
+    `hold` steps (or to the end of the run); 'reach' — it first enters the band."""
     if not np.all(np.isfinite(cd)):
         return {r: None for r in RULES}, None
     cda = trailing(cd, w)
     out = {}
     for rule in RULES:
-        k = converged_index(cda, ref, tol, hold=max(1, hold // every) if rule == "hold" else None)
+        h = {"reach": 1, "hold": max(1, hold // every), "horizon": None}[rule]
+        k = converged_index(cda, ref, tol, hold=h)
         out[rule] = None if k is None else (k + 1) * every
     return out, cda
 
@@ -319,9 +327,10 @@ def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, 
             rec.update(coarse_error=str(e), coarse={r: None for r in RULES})
         s.Shutdown()
         recs.append(rec)
-        log(f"  eval {i + 1}/{n_shapes} {d['kind']}: Cd_ref {ref:.4f}  hold-rule steps plain {steps_p['hold']} "
-            f"warm {steps_w['hold']} coarse {rec['coarse']['hold']}  horizon-rule plain {steps_p['horizon']} "
-            f"warm {steps_w['horizon']} coarse {rec['coarse']['horizon']}  (warm final Cd {rec['warm_cd_final']})")
+        log(f"  eval {i + 1}/{n_shapes} {d['kind']}: Cd_ref {ref:.4f}  steps plain/warm/coarse: reach "
+            f"{steps_p['reach']}/{steps_w['reach']}/{rec['coarse']['reach']}  hold {steps_p['hold']}/{steps_w['hold']}/"
+            f"{rec['coarse']['hold']}  horizon {steps_p['horizon']}/{steps_w['horizon']}/{rec['coarse']['horizon']}  "
+            f"(warm final Cd {rec['warm_cd_final']})")
         if time_budget_s is not None and time.time() - t_start > time_budget_s:
             log(f"  evaluation time budget reached after {i + 1} shapes")
             break
