@@ -314,3 +314,74 @@ def field_from_prediction(pred, u_in, V):
     _chk(nn_lib().fc_nn_pred_to_field(_p(pred), u_in, V, _p(field),
                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "pred_to_field")
     return field
+
+
+def time_layers(nb=4, dims=(128, 128, 256), reps=10, train_reps=5, log=print):
+    """Times each forward convolution of the network (HIP events over `reps` launches)
+    and a whole training step on random inputs of nb samples of dims = (D, H, W);
+    returns {"layers": [(name, voxels, kin, cout, ms, TFLOP/s)], "train_step_ms", "fwd_flops"}."""
+    import torch
+    L = nn_lib()
+    net = Surrogate(seed=0)
+    V0 = nb * dims[0] * dims[1] * dims[2]
+    x = torch.randn(V0 * IN_CH, device=net.dev).to(torch.bfloat16)
+    acts = net.forward(x, nb, dims)
+    torch.cuda.synchronize()
+    s = net._stream()
+    rows, tot_f = [], 0.0
+    for name, mode, ins, kin, c1, cout, stride, lvl in LAYERS:
+        dd, hd, wd = net.level_dims(dims, lvl)
+        V = nb * dd * hd * wd
+        y = torch.empty(V * cout, dtype=torch.bfloat16, device=net.dev)
+        p = net.params[name]
+        if mode == FWD:
+            si = net.level_dims(dims, lvl - (1 if stride == 2 else 0))
+            args = (FWD, _p(acts[ins[0]]), None, _p(p["wbf"]), _p(p["b"]), _p(y), None, nb, si[0], si[1], si[2], dd,
+                    hd, wd, kin, 0, cout, stride, 1, s)
+        else:
+            args = (UPCAT, _p(acts[ins[1]]), _p(acts[ins[0]]), _p(p["wbf"]), _p(p["b"]), _p(y), None, nb, dd, hd, wd,
+                    dd, hd, wd, kin, c1, cout, 1, 1, s)
+        _chk(L.fc_nn_conv(*args), name)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            L.fc_nn_conv(*args)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        flops = 2.0 * V * 27 * kin * cout
+        tot_f += flops
+        rows.append((name, V, kin, cout, ms, flops / ms / 1e9))
+        log(f"{name:4s} V={V:9d} K={27 * kin:5d} N={cout:4d}  {ms:8.3f} ms  {flops / ms / 1e9:8.1f} TFLOP/s")
+    tgt = torch.zeros(V0 * 4, device=net.dev)
+    wm = torch.ones(V0, device=net.dev)
+    for _ in range(2):
+        net.train_step(x, tgt, wm, nb, dims)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(train_reps):
+        net.train_step(x, tgt, wm, nb, dims)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / train_reps
+    log(f"train step (batch {nb}): {ms:.2f} ms; 3 x forward FLOPs -> {3 * tot_f / ms / 1e9:.1f} TFLOP/s")
+    return {"layers": rows, "train_step_ms": ms, "fwd_flops": tot_f}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m flowcraft_amd.surrogate",
+                                 description="Time the surrogate's convolution kernels and a training step.")
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--dims", type=int, nargs=3, default=[128, 128, 256], metavar=("D", "H", "W"))
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    import torch
+    torch.cuda.init()
+    r = time_layers(a.batch, tuple(a.dims))
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(r, f)
