@@ -71,15 +71,20 @@ constexpr int kStoreAux = 2;  // non-temporal
 #else
 constexpr int kStoreAux = 0;
 #endif
+// Cache policy of the population loads (0: default).
+#ifndef FC_LBM_LOAD_AUX
+#define FC_LBM_LOAD_AUX 0
+#endif
+constexpr int kLoadAux = FC_LBM_LOAD_AUX;
 
 template <class T>
 __device__ __forceinline__ T BufLoad(BufRsrc r, uint32_t vo, uint32_t so) {
     if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, kLoadAux));
     else if constexpr (sizeof(T) == 4)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, kLoadAux));
     else
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, kLoadAux));
 }
 
 template <class T>
@@ -657,7 +662,11 @@ __device__ __forceinline__ bool CollideOffset(const float (&g)[Q], float (&go)[Q
 // Workgroups are dealt to the 8 XCDs round-robin by hardware id; the id is remapped so
 // that each XCD processes one contiguous range of tiles, keeping the cache lines shared
 // by neighbouring tiles (x-shifted pulls, y/z neighbour rows) inside one XCD's L2.
-constexpr int kTileX = 256, kTileY = 1;
+#ifndef FC_LBM_TILE_X
+#define FC_LBM_TILE_X 256
+#endif
+constexpr int kTileX = FC_LBM_TILE_X, kTileY = 256 / FC_LBM_TILE_X;
+static_assert(kTileX % 64 == 0 && 256 % kTileX == 0, "a tile row is whole waves");
 
 __device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
     const unsigned q = n >> 3, r = n & 7u, xcd = hw & 7u, slot = hw >> 3;
@@ -979,7 +988,10 @@ template <int Q, class S, class Lay, int CPT, bool PER>
 __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     using R = typename S::R;
     using T = typename S::T;
-    const unsigned tile = XcdContiguous(blockIdx.x, gridDim.x);
+#ifndef FC_LBM_XCD_REMAP
+#define FC_LBM_XCD_REMAP 1
+#endif
+    const unsigned tile = FC_LBM_XCD_REMAP ? XcdContiguous(blockIdx.x, gridDim.x) : blockIdx.x;
     const unsigned tiles_x = static_cast<unsigned>((a.nx + kTileX - 1) / kTileX);
     const unsigned tiles_y = static_cast<unsigned>((a.ny + kTileY * CPT - 1) / (kTileY * CPT));
     const unsigned tx = tile % tiles_x;
