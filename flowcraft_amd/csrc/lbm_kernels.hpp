@@ -123,6 +123,10 @@ __host__ __device__ constexpr R W(int d) {
     return sizeof(R) == 4 ? static_cast<R>(Lat<Q>::w[d]) : static_cast<R>(Lat<Q>::wd[d]);
 }
 
+// s_setreg operand for bits 6..7 of the MODE register (half/double denormal handling):
+// hwreg id 1 (MODE) | offset 6 << 6 | (size 2 - 1) << 11.
+constexpr int kHwRegModeFp16Denorm = 1 | (6 << 6) | (1 << 11);
+
 // -------------------------------------------------------- storage formats
 // Raw/Enc move between the stored element and the step's working value: the
 // population f itself for FP32/FP64, the scaled offset g = (f - w) * S for the
@@ -138,11 +142,19 @@ struct StoreF32 {
     static __device__ __forceinline__ T Store(R f, int) { return f; }
     static __device__ __forceinline__ R Raw(T v) { return v; }
     static __device__ __forceinline__ T Enc(R v) { return v; }
+    static __device__ __forceinline__ void StepMode() {}
+    static __device__ __forceinline__ T EncStep(R v) { return v; }
 };
 
 // FP16 offset: g = (f - w) * 4096 as IEEE binary16 (RNE); arithmetic in FP32.  Offsets
-// below the smallest normal half (|g| < 2^-14, i.e. |f - w| < 1.5e-8) are stored as 0,
-// so the format never depends on the device's half-precision denormal mode.
+// that are tiny in the IEEE sense — below the smallest normal half 2^-14 after rounding
+// to 11 significant bits with an unbounded exponent ("tininess after rounding") — are
+// stored as a zero of the same sign: |f - w| < 1.5e-8, never a subnormal half, so the
+// format does not depend on a device's half-precision denormal mode.  That is exactly
+// what the hardware conversion produces with half denormals flushed (checked on
+// MI355X over the values bracketing 2^-14, ties included): the step kernel sets that
+// mode (StepMode) and converts with one instruction (EncStep); Enc spells the rule out
+// for kernels that run in the default mode.
 template <int Q>
 struct StoreF16 {
     using T = _Float16;
@@ -150,7 +162,20 @@ struct StoreF16 {
     static constexpr bool kOffset = true;
     static constexpr float kScale = 4096.0f;
     static __device__ __forceinline__ R Raw(T v) { return static_cast<float>(v); }
-    static __device__ __forceinline__ T Enc(R g) { return static_cast<T>(fabsf(g) < 6.103515625e-05f ? 0.0f : g); }
+    static __device__ __forceinline__ T Enc(R g) {
+        const uint32_t b = __float_as_uint(g);
+        const uint32_t a = b & 0x7FFFFFFFu;
+        if (a < 0x38800000u) {  // |g| < 2^-14: round to 11 significant bits, unbounded exponent
+            uint32_t hb = a >> 13;
+            const uint32_t rest = a & 0x1FFFu;
+            if (rest > 0x1000u || (rest == 0x1000u && (hb & 1u))) ++hb;
+            if ((hb << 13) < 0x38800000u) return __builtin_bit_cast(T, static_cast<uint16_t>((b >> 16) & 0x8000u));
+        }
+        return static_cast<T>(g);
+    }
+    // Half denormals flushed (MODE.FP_DENORM for f16/f64 = 0) for the rest of the wave.
+    static __device__ __forceinline__ void StepMode() { __builtin_amdgcn_s_setreg(kHwRegModeFp16Denorm, 0); }
+    static __device__ __forceinline__ T EncStep(R g) { return static_cast<T>(g); }
     static __device__ __forceinline__ R Load(T v, int d) { return Lat<Q>::w[d] + Raw(v) * (1.0f / 4096.0f); }
     static __device__ __forceinline__ T Store(R f, int d) { return Enc((f - Lat<Q>::w[d]) * 4096.0f); }
 };
@@ -171,6 +196,8 @@ struct StoreBF16 {
     }
     static __device__ __forceinline__ R Load(T v, int d) { return Lat<Q>::w[d] + Raw(v); }
     static __device__ __forceinline__ T Store(R f, int d) { return Enc(f - Lat<Q>::w[d]); }
+    static __device__ __forceinline__ void StepMode() {}
+    static __device__ __forceinline__ T EncStep(R g) { return Enc(g); }
 };
 
 // FP64: the whole step in double (GPU twin of the FP64 oracle).
@@ -183,6 +210,8 @@ struct StoreF64 {
     static __device__ __forceinline__ T Store(R f, int) { return f; }
     static __device__ __forceinline__ R Raw(T v) { return v; }
     static __device__ __forceinline__ T Enc(R v) { return v; }
+    static __device__ __forceinline__ void StepMode() {}
+    static __device__ __forceinline__ T EncStep(R v) { return v; }
 };
 
 // ------------------------------------------------------ boundary records
@@ -673,12 +702,13 @@ __device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
     return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
-// Stored form of a cell's post-collision working values.
+// Stored form of a cell's post-collision working values (inside the step kernel, which
+// has set the storage format's StepMode).
 template <int Q, class S>
 __device__ __forceinline__ void EncodeCell(const typename S::R (&fo)[Q], typename S::T (&enc)[Q]) {
     StaticFor<0, Q>([&](auto I) {
         constexpr int d = I;
-        enc[d] = S::Enc(fo[d]);
+        enc[d] = S::EncStep(fo[d]);
     });
 }
 
@@ -988,6 +1018,7 @@ template <int Q, class S, class Lay, int CPT, bool PER>
 __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
     using R = typename S::R;
     using T = typename S::T;
+    S::StepMode();
 #ifndef FC_LBM_XCD_REMAP
 #define FC_LBM_XCD_REMAP 1
 #endif

@@ -71,6 +71,8 @@ def lib():
         L.orc_set_pops.argtypes = [ctypes.c_void_p, _c_double_p]
         L.orc_f32_to_f16.argtypes = [ctypes.c_float]
         L.orc_f32_to_f16.restype = ctypes.c_uint16
+        L.orc_f32_to_f16_flush.argtypes = [ctypes.c_float]
+        L.orc_f32_to_f16_flush.restype = ctypes.c_uint16
         L.orc_f16_to_f32.argtypes = [ctypes.c_uint16]
         L.orc_f16_to_f32.restype = ctypes.c_float
         L.orc_f32_to_bf16.argtypes = [ctypes.c_float]

@@ -108,6 +108,23 @@ uint16_t F32ToF16(float v) {
     return static_cast<uint16_t>(sign | hm);
 }
 
+// The engine's FP16 storage rule: RNE conversion, except that values tiny in the IEEE
+// "tininess after rounding" sense — below 2^-14 once rounded to 11 significant bits with
+// an unbounded exponent — become a zero of the same sign (the device's conversion with
+// half denormals flushed).
+uint16_t F32ToF16Flush(float v) {
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    const uint32_t a = b & 0x7FFFFFFFu;
+    if (a < 0x38800000u) {
+        uint32_t hb = a >> 13;
+        const uint32_t rest = a & 0x1FFFu;
+        if (rest > 0x1000u || (rest == 0x1000u && (hb & 1u))) ++hb;
+        if ((hb << 13) < 0x38800000u) return static_cast<uint16_t>((b >> 16) & 0x8000u);
+    }
+    return F32ToF16(v);
+}
+
 float F16ToF32(uint16_t h) {
     uint32_t sign = static_cast<uint32_t>(h & 0x8000u) << 16;
     uint32_t e = (h >> 10) & 0x1Fu;
@@ -250,7 +267,7 @@ struct Oracle {
     float Scale() const { return storage == 1 ? kF16Scale : 1.0f; }
     // FP16: offsets below the smallest normal half (2^-14) are stored as 0.
     float EncDec(float g) const {
-        if (storage == 1) return std::fabs(g) < 6.103515625e-05f ? 0.0f : F16ToF32(F32ToF16(g));
+        if (storage == 1) return F16ToF32(F32ToF16Flush(g));
         return BF16ToF32(F32ToBF16(g));
     }
     Real ToF(Real v, int d) const {
@@ -911,6 +928,7 @@ void orc_set_pops(void* p, const double* in) {
 
 // Storage-format helpers, exported so tests can pin them against numpy.
 uint16_t orc_f32_to_f16(float v) { return F32ToF16(v); }
+uint16_t orc_f32_to_f16_flush(float v) { return F32ToF16Flush(v); }
 float orc_f16_to_f32(uint16_t h) { return F16ToF32(h); }
 uint16_t orc_f32_to_bf16(float v) { return F32ToBF16(v); }
 float orc_bf16_to_f32(uint16_t h) { return BF16ToF32(h); }

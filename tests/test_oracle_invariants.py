@@ -5,6 +5,8 @@ that reduce to simpler schemes.  Each is chosen so a plausible mistake in the
 oracle (a dropped term, a sign, a transposed index) fails it."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -294,3 +296,23 @@ def test_expected_links_cover_every_solid_source():
         if k in (3, 4):
             qv = np.uint32(qb).view(np.float32)
             assert 0 <= qv <= 1 and ((qv < 0.5) == (k == 3))
+
+
+def test_f16_storage_flush_matches_device_conversion():
+    """The FP16 storage rule of the oracle (tiny values, in the IEEE 'tininess after
+    rounding' sense, become a signed zero) equals what MI355X's conversion produces with
+    half denormals flushed, on 167 values bracketing 2^-14 including the ties
+    (tests/golden/f16_flush_probe.hip wrote the fixture on the device)."""
+    import oracle
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "f16_flush_mi355x.npz"))
+    L = oracle.lib()
+    got = np.array([L.orc_f32_to_f16_flush(float(v)) for v in z["inputs"]], np.uint16)
+    assert np.array_equal(got, z["device_bits"])
+    # away from 2^-14 the rule is plain RNE, and it never produces a subnormal half
+    rng = np.random.default_rng(0)
+    v = (rng.standard_normal(20000) * 10.0 ** rng.uniform(-9, 3, 20000)).astype(np.float32)
+    f = np.array([L.orc_f32_to_f16_flush(float(x)) for x in v], np.uint16)
+    plain = v.astype(np.float16).view(np.uint16)
+    normal = np.abs(v) >= 2.0 ** -14
+    assert np.array_equal(f[normal], plain[normal])
+    assert not np.any(((f & 0x7C00) == 0) & ((f & 0x3FF) != 0))
