@@ -15,6 +15,9 @@
 //   pull19nt_4 pull19_4 with non-temporal stores
 //   blkB_4     pull19_4 on a blocked layout (runs of B cells keep their 19 populations
 //              together); blk64nt_4 with non-temporal stores
+//   ip_blk64_4 in place on the blocked layout: each plane read and written at the cell
+//   ip_esoteric_4  in place with the esoteric-pull shifts (half the planes read, half
+//              written, one cell to the right)
 // Prints one line per kernel: bytes moved per launch and GB/s (HIP events).
 #include <hip/hip_runtime.h>
 
@@ -137,6 +140,34 @@ __global__ void KPull19Blk(const float* __restrict__ a, float* __restrict__ b, i
     }
 }
 
+// In-place variants on the blocked layout (one buffer, non-temporal stores): plane d of
+// the run is read and written at the cell itself (SHIFT = false), or — the esoteric-pull
+// pattern — the odd planes are read at the cell and written one cell to the right and the
+// even planes read one cell to the right and written at the cell (SHIFT = true).  Every
+// slot is read and then written by one lane; values are meaningless (timing only).
+template <bool SHIFT>
+__global__ void KInPlaceBlk(float* __restrict__ a, int nx, int ny, int nz) {
+    const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lin >= static_cast<long long>(nx) * ny * nz) return;
+    const int x = static_cast<int>(lin % nx);
+    const int y = static_cast<int>((lin / nx) % ny);
+    const int z = static_cast<int>(lin / (static_cast<long long>(nx) * ny));
+    const int nb = nx / 64;
+    auto idx = [&](int d, int xx) -> long long {
+        return ((static_cast<long long>(z + 1) * (ny + 2) + (y + 1)) * nb + xx / 64) * (19 * 64) + d * 64 + xx % 64;
+    };
+    const int xr = x + 1 < nx ? x + 1 : 0;
+    float v[19];
+#pragma unroll
+    for (int d = 0; d < 19; ++d) v[d] = a[idx(d, SHIFT && d != 0 && (d & 1) == 0 ? xr : x)];
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        acc += v[d];
+        __builtin_nontemporal_store(acc, &a[idx(d, SHIFT && (d & 1) ? xr : x)]);
+    }
+}
+
 // KPull19<float> with non-temporal stores.
 __global__ void KPull19Nt(const float* __restrict__ a, float* __restrict__ b, int nx, int ny, int nz, long long P) {
     const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -243,6 +274,12 @@ int main(int argc, char** argv) {
             ms = TimeMs([&] { hipLaunchKernelGGL((KPull19Blk<512, false>), dim3((cells + 255) / 256), dim3(256), 0, 0,
                                                  (const float*)a, (float*)b, nx, ny, nz); }, reps);
             report("blk512_4", 2.0 * 19 * s * cells, ms);
+            ms = TimeMs([&] { hipLaunchKernelGGL((KInPlaceBlk<false>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (float*)a, nx, ny, nz); }, reps);
+            report("ip_blk64_4", 2.0 * 19 * s * cells, ms);
+            ms = TimeMs([&] { hipLaunchKernelGGL((KInPlaceBlk<true>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (float*)a, nx, ny, nz); }, reps);
+            report("ip_esoteric_4", 2.0 * 19 * s * cells, ms);
         }
     }
     CHECK(hipFree(a));
