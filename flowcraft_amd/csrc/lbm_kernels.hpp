@@ -685,12 +685,12 @@ __device__ __forceinline__ bool CollideOffset(const float (&g)[Q], float (&go)[Q
 
 // ----------------------------------------------------------- step kernel
 // Work decomposition: one workgroup = a 256-cell run of one row (kTileX by kTileY),
-// 256 threads, no integer division per thread.  A workgroup's access to each plane is
-// one contiguous 1 KiB (FP32) / 512 B (16-bit) run, which measured faster than square
-// 64x4 tiles (FP32 ping-pong 35.3 vs 34.8 GLUPS, FP16 62.3 vs 58.4 on 512x256x256).
-// Workgroups are dealt to the 8 XCDs round-robin by hardware id; the id is remapped so
-// that each XCD processes one contiguous range of tiles, keeping the cache lines shared
-// by neighbouring tiles (x-shifted pulls, y/z neighbour rows) inside one XCD's L2.
+// 256 threads = 4 waves = 4 population blocks, no integer division per thread.  On the
+// 512x256x256 lattice, 256x1 tiles measured 36.7 (FP32) / 65.5 (FP16) GLUPS, 128x2 36.96 /
+// 64.3, 64x4 36.7 / 62.8 (ping-pong).  Workgroups are dealt to the 8 XCDs round-robin by
+// hardware id; the id is remapped so that each XCD processes one contiguous range of
+// tiles, keeping the cache lines shared by neighbouring tiles (x-shifted pulls, y/z
+// neighbour rows) inside one XCD's L2 (without the remap: 36.4 / 61.2 GLUPS).
 #ifndef FC_LBM_TILE_X
 #define FC_LBM_TILE_X 256
 #endif
