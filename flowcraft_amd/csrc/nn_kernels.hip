@@ -26,6 +26,7 @@
 #include <stdint.h>
 
 #include <cstring>
+#include <type_traits>
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -178,8 +179,8 @@ __global__ __launch_bounds__(256) void KConv(const ConvArgs a) {
 }
 
 // Weight gradient of a kFwd/kUpcat conv: dW[co][k] = sum_v dZ[v][co] * A[v][k], split over
-// voxel ranges (blockIdx.z) into fp32 partials part[split][co][K].  Block tile 64 co x 64 k,
-// 32 voxels per step, staged transposed in LDS so fragments are single 16-byte reads.
+// voxel ranges (blockIdx.z) into fp32 partials part[split][co][K].  32 voxels per step,
+// staged transposed in LDS so fragments are single 16-byte reads.
 struct WgradArgs {
     ConvArgs g;       // gather description of the forward conv (MODE, dims, kin, c1, stride)
     const u16* dz;    // [V][cout]
@@ -188,37 +189,47 @@ struct WgradArgs {
     float* part;      // [splits][cout][27*kin]
 };
 
-template <int MODE>
+// Block tile TCO output channels x TK = 4096 / TCO k-columns (16 x 256, 32 x 128 or
+// 64 x 64, matched to the layer's channel count so narrow layers waste no MFMA work), 32
+// voxels per step; each of the 4 waves owns TCO/16 x (TK/4)/16 MFMA tiles (4 in all).
+template <int MODE, int TCO>
 __global__ __launch_bounds__(256) void KWgrad(const WgradArgs w) {
     constexpr int VS = 32, LD = 40;  // voxels per step, padded LDS row (bf16 elements)
-    __shared__ __attribute__((aligned(16))) u16 sA[64 * LD];  // [co][v]
-    __shared__ __attribute__((aligned(16))) u16 sB[64 * LD];  // [k][v]
+    constexpr int TK = 4096 / TCO;
+    constexpr int WI = TCO == 16 ? 1 : 2;  // MFMA tiles per wave along co
+    constexpr int WJ = 4 / WI;             // ... along k
+    static_assert(TCO == 16 || TCO == 32 || TCO == 64, "co tile");
+    __shared__ __attribute__((aligned(16))) u16 sA[TCO * LD];  // [co][v]
+    __shared__ __attribute__((aligned(16))) u16 sB[TK * LD];   // [k][v]
     const ConvArgs& a = w.g;
     const int K = 27 * a.kin;
-    const int co0 = blockIdx.y * 64, k0b = blockIdx.x * 64;
+    const int co0 = blockIdx.y * TCO, k0b = blockIdx.x * TK;
     const long long V = static_cast<long long>(a.nb) * a.dd * a.hd * a.wd;
     const long long vbeg = static_cast<long long>(blockIdx.z) * w.vper;
     const long long vend = vbeg + w.vper < V ? vbeg + w.vper : V;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;  // wave tile 32 co x 32 k
-    f32x4 acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // wave tile: rows [wm, wm + 16*WI) of co, columns [wn, wn + 16*WJ) of k
+    const int wm = TCO == 64 ? (wave & 1) * 32 : 0;
+    const int wn = TCO == 64 ? (wave >> 1) * 32 : wave * 16 * WJ;
+    f32x4 acc[WI][WJ];
+    for (int i = 0; i < WI; ++i)
+        for (int j = 0; j < WJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     for (long long v0 = vbeg; v0 < vend; v0 += VS) {
-        // stage dZ[v0..v0+32][co0..co0+64] transposed: 2048 values, 8 per thread
-        for (int e = threadIdx.x; e < VS * 64; e += 256) {
-            const int vv = e >> 6, cc = e & 63;
+        // stage dZ[v0..v0+32][co0..co0+TCO] transposed
+        for (int e = threadIdx.x; e < VS * TCO; e += 256) {
+            const int vv = e / TCO, cc = e % TCO;
             const long long v = v0 + vv;
             u16 val = 0;
             if (v < vend && co0 + cc < w.cout) val = w.dz[v * w.cout + co0 + cc];
             sA[cc * LD + vv] = val;
         }
-        // stage A[v][k0b..k0b+64] (gathered input) transposed: 8-runs per thread
-        for (int e = threadIdx.x; e < VS * 8; e += 256) {
-            const int vv = e >> 3, kk = (e & 7) * 8;
+        // stage A[v][k0b..k0b+TK] (gathered input) transposed, voxel-fastest: lanes
+        // 32 apart in e share a k-run, consecutive lanes write consecutive voxels of one
+        // LDS row (no bank conflicts), and each thread keeps one voxel for all its runs.
+        {
+            const int vv = threadIdx.x % VS;
             const long long v = v0 + vv;
-            bf16x8 g8;
             const bool vok = v < vend;
             long long vc = vok ? v : 0;
             const int x = static_cast<int>(vc % a.wd);
@@ -227,23 +238,25 @@ __global__ __launch_bounds__(256) void KWgrad(const WgradArgs w) {
             r /= a.hd;
             const int z = static_cast<int>(r % a.dd);
             const int n = static_cast<int>(r / a.dd);
-            g8 = GatherA<MODE>(a, n, z, y, x, k0b + kk, vok);
-            u16 tmp[8];
-            __builtin_memcpy(tmp, &g8, 16);
-            for (int q = 0; q < 8; ++q) sB[(kk + q) * LD + vv] = tmp[q];
+            for (int kk = (threadIdx.x / VS) * 8; kk < TK; kk += (256 / VS) * 8) {
+                const bf16x8 g8 = GatherA<MODE>(a, n, z, y, x, k0b + kk, vok);
+                u16 tmp[8];
+                __builtin_memcpy(tmp, &g8, 16);
+                for (int q = 0; q < 8; ++q) sB[(kk + q) * LD + vv] = tmp[q];
+            }
         }
         __syncthreads();
         const int row = lane & 15, kq = (lane >> 4) * 8;
-        bf16x8 af[2], bf[2];
-        for (int i = 0; i < 2; ++i) __builtin_memcpy(&af[i], &sA[(wm + i * 16 + row) * LD + kq], 16);
-        for (int j = 0; j < 2; ++j) __builtin_memcpy(&bf[j], &sB[(wn + j * 16 + row) * LD + kq], 16);
-        for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
+        bf16x8 af[WI], bf[WJ];
+        for (int i = 0; i < WI; ++i) __builtin_memcpy(&af[i], &sA[(wm + i * 16 + row) * LD + kq], 16);
+        for (int j = 0; j < WJ; ++j) __builtin_memcpy(&bf[j], &sB[(wn + j * 16 + row) * LD + kq], 16);
+        for (int i = 0; i < WI; ++i)
+            for (int j = 0; j < WJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
     float* part = w.part + static_cast<long long>(blockIdx.z) * w.cout * K;
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) {
+    for (int i = 0; i < WI; ++i)
+        for (int j = 0; j < WJ; ++j) {
             const int k = k0b + wn + j * 16 + (lane & 15);
             for (int r = 0; r < 4; ++r) {
                 const int co = co0 + wm + i * 16 + (lane >> 4) * 4 + r;
@@ -527,11 +540,20 @@ int fc_nn_wgrad(int mode, const void* in, const void* in2, const void* dz, float
     w.part = ws;
     const int K = 27 * kin;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    dim3 grid(Blk(K, 64), Blk(cout, 64), splits);
-    if (mode == kUpcat)
-        hipLaunchKernelGGL(KWgrad<kUpcat>, grid, dim3(256), 0, s, w);
+    auto launch = [&](auto tco) {
+        constexpr int TCO = decltype(tco)::value;
+        const dim3 grid(Blk(K, 4096 / TCO), Blk(cout, TCO), splits);
+        if (mode == kUpcat)
+            hipLaunchKernelGGL((KWgrad<kUpcat, TCO>), grid, dim3(256), 0, s, w);
+        else
+            hipLaunchKernelGGL((KWgrad<kFwd, TCO>), grid, dim3(256), 0, s, w);
+    };
+    if (cout <= 16)
+        launch(std::integral_constant<int, 16>{});
+    else if (cout <= 32)
+        launch(std::integral_constant<int, 32>{});
     else
-        hipLaunchKernelGGL(KWgrad<kFwd>, grid, dim3(256), 0, s, w);
+        launch(std::integral_constant<int, 64>{});
     const long long n = static_cast<long long>(cout) * K;
     hipLaunchKernelGGL(KReduceSplits, dim3(Blk(n, 256)), dim3(256), 0, s, ws, splits, n, dw, accumulate);
     return static_cast<int>(hipGetLastError());

@@ -79,6 +79,16 @@ def nn_lib() -> ctypes.CDLL:
     return L
 
 
+def wgrad_splits(V: int, K: int, cout: int) -> int:
+    """Voxel splits of a weight-gradient launch: enough workgroups (>= 8192, or the 256
+    splits a 64x64 tile gets) to hide the gather latency, at least 512 voxels per split.
+    The kernel's output tile is 16x256, 32x128 or 64x64 (co x k) by the channel count."""
+    tco = 16 if cout <= 16 else (32 if cout <= 32 else 64)
+    tiles = -(-K // (4096 // tco)) * -(-cout // tco)
+    want = max(min(256, V // 2048), -(-8192 // tiles))
+    return int(max(1, min(want, V // 512)))
+
+
 def _chk(rc, what):
     if rc != 0:
         raise RuntimeError(f"{what} failed with code {rc}")
@@ -227,7 +237,7 @@ class Surrogate:
             _chk(L.fc_nn_leaky_backward(_p(y), _p(grads[name]), _p(dz), V * cout, s), "leaky")
             # weight and bias gradients
             K = 27 * kin
-            splits = max(1, min(256, V // 2048))
+            splits = wgrad_splits(V, K, cout)
             ws = t.empty(splits * cout * K, dtype=t.float32, device=self.dev)
             if mode == FWD:
                 si = self.level_dims(dims, lvl - (1 if stride == 2 else 0))
