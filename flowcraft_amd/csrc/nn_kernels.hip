@@ -265,34 +265,52 @@ __global__ __launch_bounds__(256) void KWgrad(const WgradArgs w) {
         }
 }
 
-// out[i] = sum_s part[s][i] (fixed order), optionally += into out.
-__global__ void KReduceSplits(const float* __restrict__ part, int splits, long long n, float* __restrict__ out,
-                              int accumulate) {
-    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s = accumulate ? out[i] : 0.f;
-    for (int k = 0; k < splits; ++k) s += part[static_cast<long long>(k) * n + i];
-    out[i] = s;
+// out[i] = sum_s part[s][i], optionally += into out.  A block owns 16 outputs; its 16
+// thread groups each sum every 16th split, then the 16 group sums are added in group
+// order — a fixed order, so results are reproducible run to run.
+constexpr int kRedOut = 16, kRedGroups = 16;
+__global__ __launch_bounds__(256) void KReduceSplits(const float* __restrict__ part, int splits, long long n,
+                                                     float* __restrict__ out, int accumulate) {
+    const int io = threadIdx.x % kRedOut, g = threadIdx.x / kRedOut;
+    const long long i = static_cast<long long>(blockIdx.x) * kRedOut + io;
+    float s = 0.f;
+    if (i < n)
+        for (int k = g; k < splits; k += kRedGroups) s += part[static_cast<long long>(k) * n + i];
+    __shared__ float red[kRedGroups][kRedOut];
+    red[g][io] = s;
+    __syncthreads();
+    if (g == 0 && i < n) {
+        float t = accumulate ? out[i] : 0.f;
+        for (int k = 0; k < kRedGroups; ++k) t += red[k][io];
+        out[i] = t;
+    }
 }
 
-// Per-channel sums over voxels of a [V][C] bf16 tensor: part[block][c] (fixed grid).
+// Per-channel sums over voxels of a [V][C] bf16 tensor: part[block][c] (fixed grid).  A
+// thread reads 8 consecutive channels of a voxel with one 16-byte load.
 constexpr int kColBlocks = 256;
 __global__ __launch_bounds__(256) void KColSumPartial(const u16* __restrict__ x, long long V, int C,
                                                       float* __restrict__ part) {
-    // thread handles channel c = tid % C over voxels strided by (256 / C) per block
-    const int lanes_per_c = 256 / C;
-    const int c = threadIdx.x % C, sub = threadIdx.x / C;
-    float s = 0.f;
-    if (sub < lanes_per_c)
-        for (long long v = static_cast<long long>(blockIdx.x) * lanes_per_c + sub; v < V;
-             v += static_cast<long long>(kColBlocks) * lanes_per_c)
-            s += Bf2F(x[v * C + c]);
-    __shared__ float red[256];
-    red[threadIdx.x] = sub < lanes_per_c ? s : 0.f;
+    const int octs = C / 8;                  // 16-byte groups per voxel
+    const int lanes = 256 / octs;            // voxels in flight per block
+    const int o = threadIdx.x % octs, sub = threadIdx.x / octs;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (long long v = static_cast<long long>(blockIdx.x) * lanes + sub; v < V;
+         v += static_cast<long long>(kColBlocks) * lanes) {
+        const uint4 raw = *reinterpret_cast<const uint4*>(x + v * C + o * 8);
+        const uint32_t w4[4] = {raw.x, raw.y, raw.z, raw.w};
+        for (int q = 0; q < 4; ++q) {
+            s[2 * q] += __uint_as_float(w4[q] << 16);
+            s[2 * q + 1] += __uint_as_float(w4[q] & 0xFFFF0000u);
+        }
+    }
+    __shared__ float red[256 * 8];
+    for (int q = 0; q < 8; ++q) red[threadIdx.x * 8 + q] = s[q];
     __syncthreads();
     if (threadIdx.x < C) {
+        const int oc = threadIdx.x / 8, q = threadIdx.x % 8;
         float t = 0.f;
-        for (int k = 0; k < lanes_per_c; ++k) t += red[k * C + threadIdx.x];
+        for (int k = 0; k < lanes; ++k) t += red[(k * octs + oc) * 8 + q];
         part[blockIdx.x * C + threadIdx.x] = t;
     }
 }
@@ -555,17 +573,17 @@ int fc_nn_wgrad(int mode, const void* in, const void* in2, const void* dz, float
     else
         launch(std::integral_constant<int, 64>{});
     const long long n = static_cast<long long>(cout) * K;
-    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(n, 256)), dim3(256), 0, s, ws, splits, n, dw, accumulate);
+    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(n, kRedOut)), dim3(256), 0, s, ws, splits, n, dw, accumulate);
     return static_cast<int>(hipGetLastError());
 }
 
 // Per-channel sums (bias gradient) of a [V][C] bf16 tensor into out[C] (fp32); ws holds
-// 256*C floats.  C must divide 256.
+// 256*C floats.  C is a multiple of 8 and C/8 divides 256.
 int fc_nn_colsum(const void* x, long long V, int C, float* out, float* ws, int accumulate, void* stream) {
-    if (C <= 0 || 256 % C != 0) return -1;
+    if (C <= 0 || C % 8 != 0 || 256 % (C / 8) != 0) return -1;
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(KColSumPartial, dim3(kColBlocks), dim3(256), 0, s, static_cast<const u16*>(x), V, C, ws);
-    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(C, 256)), dim3(256), 0, s, ws, kColBlocks, static_cast<long long>(C),
+    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(C, kRedOut)), dim3(256), 0, s, ws, kColBlocks, static_cast<long long>(C),
                        out, accumulate);
     return static_cast<int>(hipGetLastError());
 }
@@ -607,7 +625,7 @@ int fc_nn_head_loss(const void* h, int cin, const float* wh, const float* bh, co
     else
         return -1;
     const long long P = 4LL * cin + 5;
-    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(P, 256)), dim3(256), 0, s, ws, kHeadBlocks, P, out_grads_and_loss, 0);
+    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(P, kRedOut)), dim3(256), 0, s, ws, kHeadBlocks, P, out_grads_and_loss, 0);
     return static_cast<int>(hipGetLastError());
 }
 
