@@ -506,9 +506,145 @@ __global__ void KPredToField(const float* __restrict__ pred, float u_in, long lo
 
 unsigned Blk(long long n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
 
+// LDS-tiled convolution for the stride-1 modes (forward, upsample-concat forward, and
+// the adjoint of a stride-1 convolution).  A workgroup computes a 16 (x) x 4 (y) x 4 (z)
+// block of output voxels (wave w = z-slice w, 4 MFMA row tiles of 16 x-voxels each) for
+// 16*TN output channels.  For each chunk of up to 32 input channels it stages the
+// 18 x 6 x 6 halo of the input in LDS once (zeros outside the volume; for upsample-concat
+// the nearest-upsampled coarse channels and the skip channels side by side), then runs
+// the implicit GEMM over (tap, 8-channel run) pairs, four runs per 32-wide MFMA K step:
+// every input voxel is read from global memory once per chunk instead of once per tap.
+#ifndef FC_NN_CONV_TILED
+#define FC_NN_CONV_TILED 1
+#endif
+constexpr bool kConvTiled = FC_NN_CONV_TILED;
+constexpr int kTX = 16, kTY = 4, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHV = kHX * kHY * kHZ;  // 648 halo voxels
+constexpr int kCC = 32, kCS = kCC + 8;  // channels per chunk; LDS row stride (bf16), padded against bank conflicts
+
+template <int MODE, int TN>
+__global__ __launch_bounds__(256) void KConvTile(const ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) u16 sX[kHV * kCS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + kTZ - 1) / kTZ;
+    int b = blockIdx.x;
+    const int tx = b % tiles_x;
+    b /= tiles_x;
+    const int ty = b % tiles_y;
+    b /= tiles_y;
+    const int tz = b % tiles_z;
+    const int n = b / tiles_z;
+    const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+    const int nbase = blockIdx.y * (16 * TN);
+    const int row = lane & 15, grp = lane >> 4;
+    f32x4 acc[4][TN];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int K = 27 * a.kin;
+    for (int cb = 0; cb < a.kin; cb += kCC) {
+        const int cc = a.kin - cb < kCC ? a.kin - cb : kCC;  // channels in this chunk (multiple of 8)
+        const int runs8 = cc / 8;
+        // ---- stage the halo of channels [cb, cb+cc): one 16-byte run per thread iteration
+        for (int e = threadIdx.x; e < kHV * runs8; e += 256) {
+            const int hv = e / runs8, r8 = e - hv * runs8;
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const int ix = x0 - 1 + hx, iy = y0 - 1 + hy, iz = z0 - 1 + hz;
+            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+            if (ix >= 0 && iy >= 0 && iz >= 0 && ix < a.wi && iy < a.hi && iz < a.di) {
+                const int ci = cb + r8 * 8;
+                if (MODE == kUpcat && ci < a.c1) {
+                    const int h2 = a.hi >> 1, w2 = a.wi >> 1, d2 = a.di >> 1;
+                    const long long vi = ((static_cast<long long>(n) * d2 + (iz >> 1)) * h2 + (iy >> 1)) * w2 + (ix >> 1);
+                    raw = *reinterpret_cast<const uint4*>(a.in2 + vi * a.c1 + ci);
+                } else {
+                    const int cs = MODE == kUpcat ? a.kin - a.c1 : a.kin;
+                    const int c0 = MODE == kUpcat ? ci - a.c1 : ci;
+                    const long long vi = ((static_cast<long long>(n) * a.di + iz) * a.hi + iy) * a.wi + ix;
+                    raw = *reinterpret_cast<const uint4*>(a.in + vi * cs + c0);
+                }
+            }
+            *reinterpret_cast<uint4*>(&sX[hv * kCS + r8 * 8]) = raw;
+        }
+        __syncthreads();
+        // ---- implicit GEMM over the (tap, run) pairs of this chunk, 4 runs per K step
+        const int nruns = 27 * runs8;
+        for (int r0 = 0; r0 < nruns; r0 += 4) {
+            const int r = r0 + grp;  // this lane group's run
+            bf16x8 bfr[TN];
+            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+            int t = 0, c8 = 0;
+            if (r < nruns) {
+                t = r / runs8;
+                c8 = r - t * runs8;
+            }
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const int nn = nbase + j * 16 + row;
+                uint4 wv = make_uint4(0u, 0u, 0u, 0u);
+                if (r < nruns && nn < a.nout)
+                    wv = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(nn) * K + t * a.kin + cb + c8 * 8);
+                __builtin_memcpy(&bfr[j], &wv, 16);
+            }
+            const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+            // halo offsets of the tap: forward/upcat read o + k - 1, the adjoint o + 1 - k
+            const int oz = MODE == kAdj ? 2 - kz : kz, oy = MODE == kAdj ? 2 - ky : ky, ox = MODE == kAdj ? 2 - kx : kx;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                if (r < nruns) {
+                    const int hv = ((wave + oz) * kHY + (m + oy)) * kHX + (row + ox);
+                    raw = *reinterpret_cast<const uint4*>(&sX[hv * kCS + c8 * 8]);
+                } else {
+                    raw = make_uint4(0u, 0u, 0u, 0u);
+                }
+                bf16x8 afr;
+                __builtin_memcpy(&afr, &raw, 16);
+#pragma unroll
+                for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // epilogue: lane holds output rows 4*(lane/16)+r (x within the tile row), column lane%16
+    const int z = z0 + wave;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int nn = nbase + j * 16 + row;
+            const int y = y0 + m;
+            if (nn >= a.nout || y >= a.hd || z >= a.dd) continue;
+            const float bb = a.bias ? a.bias[nn] : 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int x = x0 + grp * 4 + q;
+                if (x >= a.wd) continue;
+                const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
+                float val = acc[m][j][q] + bb;
+                if (a.act == 1) val = val > 0.f ? val : kLeak * val;
+                if (a.outf)
+                    a.outf[v * a.nout + nn] = val;
+                else
+                    a.out[v * a.nout + nn] = F2Bf(val);
+            }
+        }
+}
+
 template <int MODE>
 void LaunchConv(const ConvArgs& a, hipStream_t s) {
     const long long V = static_cast<long long>(a.nb) * a.dd * a.hd * a.wd;
+    if ((MODE == kUpcat || a.stride == 1) && kConvTiled) {
+        const unsigned tiles = static_cast<unsigned>(a.nb) * ((a.dd + kTZ - 1) / kTZ) * ((a.hd + kTY - 1) / kTY) *
+                               ((a.wd + kTX - 1) / kTX);
+        if (a.nout <= 16)
+            hipLaunchKernelGGL((KConvTile<MODE, 1>), dim3(tiles, 1), dim3(256), 0, s, a);
+        else if (a.nout <= 32)
+            hipLaunchKernelGGL((KConvTile<MODE, 2>), dim3(tiles, 1), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((KConvTile<MODE, 4>), dim3(tiles, Blk(a.nout, 64)), dim3(256), 0, s, a);
+        return;
+    }
     if (a.nout <= 16) {
         hipLaunchKernelGGL((KConv<MODE, 4, 1>), dim3(Blk(V, 256), 1), dim3(256), 0, s, a);
     } else if (a.nout <= 32) {
