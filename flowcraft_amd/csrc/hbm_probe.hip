@@ -15,6 +15,8 @@
 //   pull19nt_4 pull19_4 with non-temporal stores
 //   blkB_4     pull19_4 on a blocked layout (runs of B cells keep their 19 populations
 //              together); blk64nt_4 with non-temporal stores
+//   blkflag_4  blk64nt_4 plus a one-byte-per-cell flag stream; blksep_4 all loads before
+//              the stores; blkflagsep_4 both (the step kernel's access pattern)
 //   ip_blk64_4 in place on the blocked layout: each plane read and written at the cell
 //   ip_esoteric_4  in place with the esoteric-pull shifts (half the planes read, half
 //              written, one cell to the right)
@@ -168,6 +170,41 @@ __global__ void KInPlaceBlk(float* __restrict__ a, int nx, int ny, int nz) {
     }
 }
 
+// KPull19Blk<64, true> variants closer to the step kernel: FLAG also reads one byte per
+// cell from a separate array (the step's cell-type flags); SEP issues all 19 loads before
+// the 19 stores (the step's order: pull, collide, store).
+template <bool FLAG, bool SEP>
+__global__ void KPull19BlkX(const float* __restrict__ a, float* __restrict__ b, const unsigned char* __restrict__ fl,
+                            int nx, int ny, int nz) {
+    const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lin >= static_cast<long long>(nx) * ny * nz) return;
+    const int x = static_cast<int>(lin % nx);
+    const int y = static_cast<int>((lin / nx) % ny);
+    const int z = static_cast<int>(lin / (static_cast<long long>(nx) * ny));
+    const int nb = nx / 64;
+    auto idx = [&](int d, int xx, int yy, int zz) -> long long {
+        return ((static_cast<long long>(zz + 1) * (ny + 2) + (yy + 1)) * nb + xx / 64) * (19 * 64) + d * 64 + xx % 64;
+    };
+    float v[19];
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        int xs = x - kEx[d];
+        xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+        v[d] = a[idx(d, xs, y - kEy[d], z - kEz[d])];
+        if (!SEP) __builtin_nontemporal_store(v[d] * 0.5f, &b[idx(d, x, y, z)]);
+    }
+    float f = FLAG ? static_cast<float>(fl[lin]) : 0.0f;
+    if (SEP) {
+        float acc = f;
+#pragma unroll
+        for (int d = 0; d < 19; ++d) acc += v[d];
+#pragma unroll
+        for (int d = 0; d < 19; ++d) __builtin_nontemporal_store(v[d] - acc * 0.05f, &b[idx(d, x, y, z)]);
+    } else if (FLAG && f != 0.0f) {
+        b[idx(0, x, y, z)] = f;
+    }
+}
+
 // KPull19<float> with non-temporal stores.
 __global__ void KPull19Nt(const float* __restrict__ a, float* __restrict__ b, int nx, int ny, int nz, long long P) {
     const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -274,6 +311,21 @@ int main(int argc, char** argv) {
             ms = TimeMs([&] { hipLaunchKernelGGL((KPull19Blk<512, false>), dim3((cells + 255) / 256), dim3(256), 0, 0,
                                                  (const float*)a, (float*)b, nx, ny, nz); }, reps);
             report("blk512_4", 2.0 * 19 * s * cells, ms);
+            {
+                unsigned char* fl;
+                CHECK(hipMalloc(&fl, cells));
+                CHECK(hipMemset(fl, 0, cells));
+                ms = TimeMs([&] { hipLaunchKernelGGL((KPull19BlkX<true, false>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                     (const float*)a, (float*)b, fl, nx, ny, nz); }, reps);
+                report("blkflag_4", 2.0 * 19 * s * cells, ms);
+                ms = TimeMs([&] { hipLaunchKernelGGL((KPull19BlkX<false, true>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                     (const float*)a, (float*)b, fl, nx, ny, nz); }, reps);
+                report("blksep_4", 2.0 * 19 * s * cells, ms);
+                ms = TimeMs([&] { hipLaunchKernelGGL((KPull19BlkX<true, true>), dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                     (const float*)a, (float*)b, fl, nx, ny, nz); }, reps);
+                report("blkflagsep_4", 2.0 * 19 * s * cells, ms);
+                CHECK(hipFree(fl));
+            }
             ms = TimeMs([&] { hipLaunchKernelGGL((KInPlaceBlk<false>), dim3((cells + 255) / 256), dim3(256), 0, 0,
                                                  (float*)a, nx, ny, nz); }, reps);
             report("ip_blk64_4", 2.0 * 19 * s * cells, ms);
