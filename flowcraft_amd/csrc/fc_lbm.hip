@@ -281,10 +281,11 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
         return Fail(h, FC_LBM_EINVAL, "lattice cross-section too large (3*Q*nx*(ny+2) population bytes must be < 2 GiB)");
     h->rbytes = h->storage == FC_LBM_STORE_F64 ? 8 : 4;
     fc_lbm_params_default(&h->params);
-    // Two cells per thread keeps twice the bytes in flight per wave: +2% for the 16-bit
-    // formats in ping-pong, a loss for FP32 and in place (512x256x256 measurements).
-    // FC_LBM_CELLS_PER_THREAD=1|2 overrides.
-    h->cells_per_thread = (h->tbytes == 2 && h->layout == FC_LBM_LAYOUT_PINGPONG) ? 2 : 1;
+    // One cell per thread: with every pull of a thread issued before the flag branch,
+    // two cells per thread (twice the bytes in flight per wave, at a lower occupancy)
+    // lose 3-8% in every format and layout (512x256x256: FP16 ping-pong 69.7 vs 66.7
+    // GLUPS, FP32 36.7 vs 35.6).  FC_LBM_CELLS_PER_THREAD=2 selects the two-cell variant.
+    h->cells_per_thread = 1;
     if (const char* e = std::getenv("FC_LBM_CELLS_PER_THREAD")) h->cells_per_thread = std::atoi(e) == 2 ? 2 : 1;
     FC_HIP(h, hipGetDevice(&h->device));
     if (desc->stream) {
@@ -542,8 +543,8 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     a.u_in_d = h->params.inlet_velocity;
     a.cs_d = h->params.smagorinsky_cs;
     const int cpt = h->cells_per_thread;
-    const long long tiles = static_cast<long long>((h->nx + kTileX - 1) / kTileX) *
-                            ((h->ny + kTileY * cpt - 1) / (kTileY * cpt)) * zcount;
+    const int tx = TileX(static_cast<int>(h->tbytes)), ty = TileY(static_cast<int>(h->tbytes));
+    const long long tiles = static_cast<long long>((h->nx + tx - 1) / tx) * ((h->ny + ty * cpt - 1) / (ty * cpt)) * zcount;
     hipError_t e = hipSuccess;
     Dispatch(h, [&]<int Q, class S, class Lay>() {
         InletPops<Q, S>(h, a);

@@ -17,6 +17,8 @@
 //              together); blk64nt_4 with non-temporal stores
 //   blkflag_4  blk64nt_4 plus a one-byte-per-cell flag stream; blksep_4 all loads before
 //              the stores; blkflagsep_4 both (the step kernel's access pattern)
+//   blk16_2    blkflagsep_4 with 2-byte values; blk16c2_2 with two cells of consecutive
+//              rows per thread (the 16-bit step kernel's decomposition)
 //   ip_blk64_4 in place on the blocked layout: each plane read and written at the cell
 //   ip_esoteric_4  in place with the esoteric-pull shifts (half the planes read, half
 //              written, one cell to the right)
@@ -205,6 +207,43 @@ __global__ void KPull19BlkX(const float* __restrict__ a, float* __restrict__ b, 
     }
 }
 
+// The step kernel's pattern with 2-byte values on the blocked layout: each thread pulls
+// CPT cells of consecutive rows (all loads first), reads their flag bytes, then stores
+// with non-temporal stores.
+template <int CPT>
+__global__ void KPull19Blk16(const unsigned short* __restrict__ a, unsigned short* __restrict__ b,
+                             const unsigned char* __restrict__ fl, int nx, int ny, int nz) {
+    const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int nyc = ny / CPT;
+    if (lin >= static_cast<long long>(nx) * nyc * nz) return;
+    const int x = static_cast<int>(lin % nx);
+    const int y0 = static_cast<int>((lin / nx) % nyc) * CPT;
+    const int z = static_cast<int>(lin / (static_cast<long long>(nx) * nyc));
+    const int nb = nx / 64;
+    auto idx = [&](int d, int xx, int yy, int zz) -> long long {
+        return ((static_cast<long long>(zz + 1) * (ny + 2) + (yy + 1)) * nb + xx / 64) * (19 * 64) + d * 64 + xx % 64;
+    };
+    unsigned short v[CPT][19];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+        for (int d = 0; d < 19; ++d) {
+            int xs = x - kEx[d];
+            xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+            v[c][d] = a[idx(d, xs, y0 + c - kEy[d], z - kEz[d])];
+        }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const unsigned f = fl[(static_cast<long long>(z) * ny + y0 + c) * nx + x];
+        unsigned acc = f;
+#pragma unroll
+        for (int d = 0; d < 19; ++d) acc += v[c][d];
+#pragma unroll
+        for (int d = 0; d < 19; ++d)
+            __builtin_nontemporal_store(static_cast<unsigned short>(v[c][d] ^ (acc & 1u)), &b[idx(d, x, y0 + c, z)]);
+    }
+}
+
 // KPull19<float> with non-temporal stores.
 __global__ void KPull19Nt(const float* __restrict__ a, float* __restrict__ b, int nx, int ny, int nz, long long P) {
     const long long lin = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -288,6 +327,18 @@ int main(int argc, char** argv) {
             ms = TimeMs([&] { hipLaunchKernelGGL(KPull19<_Float16>, dim3((cells + 255) / 256), dim3(256), 0, 0,
                                                  (const _Float16*)a, (_Float16*)b, nx, ny, nz, P); }, reps);
         report(s == 4 ? "pull19_4" : "pull19_2", 2.0 * 19 * s * cells, ms);
+        if (s == 2) {
+            unsigned char* fl;
+            CHECK(hipMalloc(&fl, cells));
+            CHECK(hipMemset(fl, 0, cells));
+            ms = TimeMs([&] { hipLaunchKernelGGL(KPull19Blk16<1>, dim3((cells + 255) / 256), dim3(256), 0, 0,
+                                                 (const unsigned short*)a, (unsigned short*)b, fl, nx, ny, nz); }, reps);
+            report("blk16_2", 2.0 * 19 * s * cells, ms);
+            ms = TimeMs([&] { hipLaunchKernelGGL(KPull19Blk16<2>, dim3((cells / 2 + 255) / 256), dim3(256), 0, 0,
+                                                 (const unsigned short*)a, (unsigned short*)b, fl, nx, ny, nz); }, reps);
+            report("blk16c2_2", 2.0 * 19 * s * cells, ms);
+            CHECK(hipFree(fl));
+        }
         if (s == 4) {
             ms = TimeMs([&] { hipLaunchKernelGGL(KPull19Var<true>, dim3((cells + 255) / 256), dim3(256), 0, 0,
                                                  (const float*)a, (float*)b, nx, ny, nz, P); }, reps);

@@ -588,48 +588,42 @@ __device__ __forceinline__ bool Collide(const R (&f)[Q], R (&fo)[Q], R tau, R cs
 // written on g_i = (f_i - w_i) * S in moment space: with rho = 1 + drho,
 //   drho = sum g / S,  j = sum e g / S,  Pi_neq = sum e e g / S - j u - (drho/3) I,
 //   T = j u + (1 - 1/tau_eff) Pi_neq,
-//   g_i' = (w_i S) (drho + 3 e.j + 4.5 Q_i : T),  Q_i = e e - I/3.
+//   g_i' = W_i (drho + 3 e.j + 4.5 Q_i : T),  Q_i = e e - I/3,  W_i = w_i S.
 // Identical to the reference's population-space form in exact arithmetic (the
 // lattice moments of f^eq are exact to second order for D3Q19 and D3Q15); it works
-// on small offsets instead of O(w) populations and needs about half the arithmetic.
-// The CPU oracle emulates exactly these operations for the offset formats.
+// on small offsets instead of O(w) populations.  Both halves run per opposite pair
+// (i, i+1), e_{i+1} = -e_i: the moments from the pair sums g_i + g_{i+1} (even
+// moments) and differences g_i - g_{i+1} (odd moments), and the reconstruction as
+// g_i' = A + B, g_{i+1}' = A - B with the even part A = W drho + 4.5 W (e e : T -
+// tr T / 3) and the odd part B = 3 W e.j, from moments scaled once per weight shell.
+// That is about half the arithmetic of a per-direction evaluation.  The CPU oracle
+// emulates exactly these operations for the offset formats.
 template <int Q, class S>
 __host__ __device__ __forceinline__ void ReconstructOffset(float drho, float jx, float jy, float jz, float txx, float tyy,
                                                   float tzz, float txy, float txz, float tyz, float (&go)[Q]) {
-    constexpr float c3 = 1.0f / 3.0f;
-    const float tr3 = (txx + tyy + tzz) * c3;
-    const float j3x = 3.0f * jx, j3y = 3.0f * jy, j3z = 3.0f * jz;
-    const float t2xy = 2.0f * txy, t2xz = 2.0f * txz, t2yz = 2.0f * tyz;
-    StaticFor<0, Q>([&](auto I) {
-        constexpr int d = I;
+    const float tr = (txx + tyy) + tzz;
+    {
+        constexpr float W = Lat<Q>::w[0] * S::kScale;
+        constexpr float W15 = 1.5f * W;
+        go[0] = W * drho - W15 * tr;
+    }
+    StaticFor<0, (Q - 1) / 2>([&](auto K) {
+        constexpr int d = 2 * K + 1;
         constexpr int cx = Lat<Q>::ex[d], cy = Lat<Q>::ey[d], cz = Lat<Q>::ez[d];
-        constexpr float ws = Lat<Q>::w[d] * S::kScale;
-        float qt;
-        if constexpr (cx == 0 && cy == 0 && cz == 0) {
-            qt = -tr3;
-        } else {
-            float diag;
-            if constexpr (cx != 0) {
-                diag = txx;
-                if constexpr (cy != 0) diag = diag + tyy;
-                if constexpr (cz != 0) diag = diag + tzz;
-            } else if constexpr (cy != 0) {
-                diag = tyy;
-                if constexpr (cz != 0) diag = diag + tzz;
-            } else {
-                diag = tzz;
-            }
-            qt = diag - tr3;
-        }
-        AccC<cx * cy>(qt, t2xy);
-        AccC<cx * cz>(qt, t2xz);
-        AccC<cy * cz>(qt, t2yz);
-        float base;
-        if constexpr (cx == 0 && cy == 0 && cz == 0)
-            base = drho;
-        else
-            base = drho + Dot3<cx, cy, cz>(j3x, j3y, j3z);
-        go[d] = ws * (base + 4.5f * qt);
+        constexpr float W = Lat<Q>::w[d] * S::kScale;
+        constexpr float W15 = 1.5f * W, W3 = 3.0f * W, W45 = 4.5f * W, W9 = 9.0f * W;
+        // even part: the terms shared by every pair of the shell are the same
+        // expressions, evaluated once
+        float a = W * drho - W15 * tr;
+        if constexpr (cx != 0) a = a + W45 * txx;
+        if constexpr (cy != 0) a = a + W45 * tyy;
+        if constexpr (cz != 0) a = a + W45 * tzz;
+        AccC<cx * cy>(a, W9 * txy);
+        AccC<cx * cz>(a, W9 * txz);
+        AccC<cy * cz>(a, W9 * tyz);
+        const float b = Dot3<cx, cy, cz>(W3 * jx, W3 * jy, W3 * jz);
+        go[d] = a + b;
+        go[d + 1] = a - b;
     });
 }
 
@@ -644,20 +638,25 @@ __host__ __device__ __forceinline__ void EqOffset(float drho, float ux, float uy
 template <int Q, class S>
 __device__ __forceinline__ bool CollideOffset(const float (&g)[Q], float (&go)[Q], float tau, float cs) {
     constexpr float invS = 1.0f / S::kScale;
-    float sg = 0, Jx = 0, Jy = 0, Jz = 0, gxx = 0, gyy = 0, gzz = 0, gxy = 0, gxz = 0, gyz = 0;
-    StaticFor<0, Q>([&](auto I) {
-        constexpr int d = I;
+    // Sums over the pairs in index order.  The accumulators start at -0, the identity
+    // of IEEE addition, so the first term costs nothing.
+    float sg = g[0], Jx = -0.0f, Jy = -0.0f, Jz = -0.0f, gxx = -0.0f, gyy = -0.0f, gzz = -0.0f, gxy = -0.0f,
+          gxz = -0.0f, gyz = -0.0f;
+    StaticFor<0, (Q - 1) / 2>([&](auto K) {
+        constexpr int d = 2 * K + 1;
         constexpr int cx = Lat<Q>::ex[d], cy = Lat<Q>::ey[d], cz = Lat<Q>::ez[d];
-        sg += g[d];
-        AccC<cx>(Jx, g[d]);
-        AccC<cy>(Jy, g[d]);
-        AccC<cz>(Jz, g[d]);
-        if constexpr (cx != 0) gxx += g[d];
-        if constexpr (cy != 0) gyy += g[d];
-        if constexpr (cz != 0) gzz += g[d];
-        AccC<cx * cy>(gxy, g[d]);
-        AccC<cx * cz>(gxz, g[d]);
-        AccC<cy * cz>(gyz, g[d]);
+        const float ps = g[d] + g[d + 1];
+        const float pd = g[d] - g[d + 1];
+        sg += ps;
+        AccC<cx>(Jx, pd);
+        AccC<cy>(Jy, pd);
+        AccC<cz>(Jz, pd);
+        if constexpr (cx != 0) gxx += ps;
+        if constexpr (cy != 0) gyy += ps;
+        if constexpr (cz != 0) gzz += ps;
+        AccC<cx * cy>(gxy, ps);
+        AccC<cx * cz>(gxz, ps);
+        AccC<cy * cz>(gyz, ps);
     });
     const float drho = sg * invS;
     const float jx = Jx * invS, jy = Jy * invS, jz = Jz * invS;
@@ -684,18 +683,26 @@ __device__ __forceinline__ bool CollideOffset(const float (&g)[Q], float (&go)[Q
 }
 
 // ----------------------------------------------------------- step kernel
-// Work decomposition: one workgroup = a 256-cell run of one row (kTileX by kTileY),
-// 256 threads = 4 waves = 4 population blocks, no integer division per thread.  On the
-// 512x256x256 lattice, 256x1 tiles measured 36.7 (FP32) / 65.5 (FP16) GLUPS, 128x2 36.96 /
-// 64.3, 64x4 36.7 / 62.8 (ping-pong).  Workgroups are dealt to the 8 XCDs round-robin by
+// Work decomposition: one workgroup = a 256-cell tile of one z layer, TileX cells
+// along x by 256 / TileX rows, 256 threads = 4 waves = 4 population blocks, no integer
+// division per thread.  On the 512x256x256 lattice (ping-pong, one cell per thread, in
+// one run) 256x1 tiles measured 36.8 (FP32) / 72.2 (FP16) GLUPS, 128x2 37.3 / 70.5, 64x4
+// 37.0 / 68.7: 128x2 for 4- and 8-byte populations, 256x1 for 2-byte ones, whose rows
+// are half as long in bytes.  Workgroups are dealt to the 8 XCDs round-robin by
 // hardware id; the id is remapped so that each XCD processes one contiguous range of
 // tiles, keeping the cache lines shared by neighbouring tiles (x-shifted pulls, y/z
-// neighbour rows) inside one XCD's L2 (without the remap: 36.4 / 61.2 GLUPS).
+// neighbour rows) inside one XCD's L2 (without the remap: 36.4 / 61.2 GLUPS, 256x1 with
+// the earlier two-cell 16-bit kernel).  FC_LBM_TILE_X=64|128|256 fixes the width for
+// every format.
 #ifndef FC_LBM_TILE_X
-#define FC_LBM_TILE_X 256
+#define FC_LBM_TILE_X 0
 #endif
-constexpr int kTileX = FC_LBM_TILE_X, kTileY = 256 / FC_LBM_TILE_X;
-static_assert(kTileX % 64 == 0 && 256 % kTileX == 0, "a tile row is whole waves");
+__host__ __device__ constexpr int TileX(int tbytes) {
+    return FC_LBM_TILE_X > 0 ? FC_LBM_TILE_X : (tbytes == 2 ? 256 : 128);
+}
+__host__ __device__ constexpr int TileY(int tbytes) { return 256 / TileX(tbytes); }
+static_assert(TileX(2) % 64 == 0 && 256 % TileX(2) == 0 && TileX(4) % 64 == 0 && 256 % TileX(4) == 0,
+              "a tile row is whole waves");
 
 __device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
     const unsigned q = n >> 3, r = n & 7u, xcd = hw & 7u, slot = hw >> 3;
@@ -1012,6 +1019,9 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
     StoreCell<Q, S, Lay, PER>(a, g, dst_layer, ln, x, y, z, seam, fo, rec);
 }
 
+#ifndef FC_LBM_PIN_PULLS
+#define FC_LBM_PIN_PULLS 1
+#endif
 // One workgroup = a kTileX by kTileY*CPT tile of one z layer; each thread updates CPT
 // cells kTileY rows apart.  The bulk pulls of all CPT cells are issued first.
 template <int Q, class S, class Lay, int CPT, bool PER>
@@ -1023,6 +1033,7 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
 #define FC_LBM_XCD_REMAP 1
 #endif
     const unsigned tile = FC_LBM_XCD_REMAP ? XcdContiguous(blockIdx.x, gridDim.x) : blockIdx.x;
+    constexpr int kTileX = TileX(sizeof(T)), kTileY = TileY(sizeof(T));
     const unsigned tiles_x = static_cast<unsigned>((a.nx + kTileX - 1) / kTileX);
     const unsigned tiles_y = static_cast<unsigned>((a.ny + kTileY * CPT - 1) / (kTileY * CPT));
     const unsigned tx = tile % tiles_x;
@@ -1051,6 +1062,17 @@ __global__ __launch_bounds__(256) void KStep(const StepArgs a) {
         const int y = min(y0 + k * kTileY, a.ny - 1);
         fl[k] = LoadU<uint8_t>(flag_layer, static_cast<uint32_t>((y + 1) * g.sy + x));
     }
+#if FC_LBM_PIN_PULLS
+    // Pin the bulk pulls ahead of the branches on the flag byte.  Their values are used
+    // only inside the fluid branch, so the compiler would otherwise sink each pull into
+    // it, behind the flag's round trip (and, with two cells per thread, behind the other
+    // cell's update).  An empty asm that takes every pulled value keeps all loads of the
+    // thread in flight together: one memory round trip per wave instead of two or more.
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+#pragma unroll
+        for (int d = 0; d < Q; ++d) asm volatile("" ::"v"(fs[k][d]));
+#endif
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int y = y0 + k * kTileY;

@@ -291,49 +291,40 @@ struct Oracle {
     }
 
     // ---- offset-space collision (engine: lbm_kernels.hpp ReconstructOffset/CollideOffset)
+    // Per opposite pair (d, d+1), e_{d+1} = -e_d: go_d = A + B, go_{d+1} = A - B with
+    // A = W drho - 1.5 W tr T + 4.5 W sum_a e_a^2 T_aa + 9 W sum_{a<b} e_a e_b T_ab and
+    // B = 3 W e.j, W = w_d S; the constant factors are rounded to float once.
     void ReconstructOffset(float drho, float jx, float jy, float jz, float txx, float tyy, float tzz, float txy,
                            float txz, float tyz, float* go) const {
-        const float c3 = 1.0f / 3.0f;
-        const float tr3 = (txx + tyy + tzz) * c3;
-        const float j3[3] = {3.0f * jx, 3.0f * jy, 3.0f * jz};
-        const float t2xy = 2.0f * txy, t2xz = 2.0f * txz, t2yz = 2.0f * tyz;
+        const float tr = (txx + tyy) + tzz;
+        {
+            const float W = wf[0] * Scale();
+            const float W15 = 1.5f * W;
+            go[0] = W * drho - W15 * tr;
+        }
         const float tdiag[3] = {txx, tyy, tzz};
-        for (int d = 0; d < Q; ++d) {
+        const float j[3] = {jx, jy, jz};
+        for (int d = 1; d < Q; d += 2) {
             const int c[3] = {ex[d], ey[d], ez[d]};
-            const bool rest = c[0] == 0 && c[1] == 0 && c[2] == 0;
-            float qt;
-            if (rest) {
-                qt = -tr3;
-            } else {
-                bool have = false;
-                float diag = 0.0f;
-                for (int a = 0; a < 3; ++a)
-                    if (c[a] != 0) {
-                        diag = have ? diag + tdiag[a] : tdiag[a];
-                        have = true;
-                    }
-                qt = diag - tr3;
-            }
-            if (c[0] * c[1] == 1) qt += t2xy;
-            if (c[0] * c[1] == -1) qt -= t2xy;
-            if (c[0] * c[2] == 1) qt += t2xz;
-            if (c[0] * c[2] == -1) qt -= t2xz;
-            if (c[1] * c[2] == 1) qt += t2yz;
-            if (c[1] * c[2] == -1) qt -= t2yz;
-            float base = drho;
-            if (!rest) {
-                bool have = false;
-                float dot = 0.0f;
-                for (int a = 0; a < 3; ++a)
-                    if (c[a] != 0) {
-                        const float term = c[a] == 1 ? j3[a] : -j3[a];
-                        dot = have ? dot + term : term;
-                        have = true;
-                    }
-                base = drho + dot;
-            }
-            const float ws = wf[d] * Scale();
-            go[d] = ws * (base + 4.5f * qt);
+            const float W = wf[d] * Scale();
+            const float W15 = 1.5f * W, W3 = 3.0f * W, W45 = 4.5f * W, W9 = 9.0f * W;
+            float a = W * drho - W15 * tr;
+            for (int k = 0; k < 3; ++k)
+                if (c[k] != 0) a = a + W45 * tdiag[k];
+            const int sxy = c[0] * c[1], sxz = c[0] * c[2], syz = c[1] * c[2];
+            if (sxy != 0) a = sxy > 0 ? a + W9 * txy : a - W9 * txy;
+            if (sxz != 0) a = sxz > 0 ? a + W9 * txz : a - W9 * txz;
+            if (syz != 0) a = syz > 0 ? a + W9 * tyz : a - W9 * tyz;
+            bool have = false;
+            float b = 0.0f;
+            for (int k = 0; k < 3; ++k)
+                if (c[k] != 0) {
+                    const float term = c[k] > 0 ? W3 * j[k] : -(W3 * j[k]);
+                    b = have ? b + term : term;
+                    have = true;
+                }
+            go[d] = a + b;
+            go[d + 1] = a - b;
         }
     }
 
@@ -359,20 +350,30 @@ struct Oracle {
 
     bool CollideOffset(const float* g, float* go) const {
         const float invS = 1.0f / Scale();
-        float sg, Jx, Jy, Jz;
-        RawSums(g, sg, Jx, Jy, Jz);
-        float gxx = 0, gyy = 0, gzz = 0, gxy = 0, gxz = 0, gyz = 0;
-        for (int d = 0; d < Q; ++d) {
-            if (ex[d] != 0) gxx += g[d];
-            if (ey[d] != 0) gyy += g[d];
-            if (ez[d] != 0) gzz += g[d];
+        // Moments from the pair sums ps = g_d + g_{d+1} and differences pd = g_d - g_{d+1},
+        // accumulated in pair order from -0 (the additive identity).
+        float sg = g[0], Jx = -0.0f, Jy = -0.0f, Jz = -0.0f;
+        float gxx = -0.0f, gyy = -0.0f, gzz = -0.0f, gxy = -0.0f, gxz = -0.0f, gyz = -0.0f;
+        for (int d = 1; d < Q; d += 2) {
+            const float ps = g[d] + g[d + 1];
+            const float pd = g[d] - g[d + 1];
+            sg += ps;
+            if (ex[d] == 1) Jx += pd;
+            if (ex[d] == -1) Jx -= pd;
+            if (ey[d] == 1) Jy += pd;
+            if (ey[d] == -1) Jy -= pd;
+            if (ez[d] == 1) Jz += pd;
+            if (ez[d] == -1) Jz -= pd;
+            if (ex[d] != 0) gxx += ps;
+            if (ey[d] != 0) gyy += ps;
+            if (ez[d] != 0) gzz += ps;
             const int sxy = ex[d] * ey[d], sxz = ex[d] * ez[d], syz = ey[d] * ez[d];
-            if (sxy == 1) gxy += g[d];
-            if (sxy == -1) gxy -= g[d];
-            if (sxz == 1) gxz += g[d];
-            if (sxz == -1) gxz -= g[d];
-            if (syz == 1) gyz += g[d];
-            if (syz == -1) gyz -= g[d];
+            if (sxy == 1) gxy += ps;
+            if (sxy == -1) gxy -= ps;
+            if (sxz == 1) gxz += ps;
+            if (sxz == -1) gxz -= ps;
+            if (syz == 1) gyz += ps;
+            if (syz == -1) gyz -= ps;
         }
         const float drho = sg * invS;
         const float jx = Jx * invS, jy = Jy * invS, jz = Jz * invS;
