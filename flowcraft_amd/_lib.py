@@ -26,6 +26,7 @@ EXPORTS = (
     "fc_lbm_forces", "fc_lbm_forces_device", "fc_lbm_probe_health", "fc_lbm_has_diverged",
     "fc_lbm_get_populations", "fc_lbm_set_populations", "fc_lbm_sync", "fc_lbm_info", "fc_lbm_step_count",
     "fc_lbm_population_bytes", "fc_lbm_stream", "fc_lbm_boundary_links", "fc_lbm_surface_cells",
+    "fc_lbm_aero_forces",
 )
 
 
@@ -46,6 +47,13 @@ class FcLbmStats(ctypes.Structure):
                 ("force_x", ctypes.c_double), ("force_y", ctypes.c_double), ("force_z", ctypes.c_double),
                 ("max_stress", ctypes.c_double), ("total_mass", ctypes.c_double),
                 ("fluid_cells", ctypes.c_longlong), ("has_nan", ctypes.c_int)]
+
+
+class FcLbmAero(ctypes.Structure):
+    _fields_ = [("force_x", ctypes.c_float), ("force_y", ctypes.c_float), ("force_z", ctypes.c_float),
+                ("min_stress", ctypes.c_float), ("max_stress", ctypes.c_float),
+                ("center_of_pressure", ctypes.c_float * 3), ("lift_arrow_base", ctypes.c_float * 3),
+                ("has_nan", ctypes.c_int)]
 
 
 _lib = None
@@ -97,6 +105,7 @@ def load() -> ctypes.CDLL:
     L.fc_lbm_stream.argtypes = [h]
     L.fc_lbm_stream.restype = ctypes.c_void_p
     L.fc_lbm_boundary_links.argtypes = [h, ctypes.c_longlong, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_longlong)]
+    L.fc_lbm_aero_forces.argtypes = [h, ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(FcLbmAero)]
     L.fc_lbm_surface_cells.argtypes = [h]
     L.fc_lbm_surface_cells.restype = ctypes.c_longlong
     _lib = L

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -308,7 +309,7 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     FC_HIP(h, hipMemsetAsync(h->side_mom, 0, 16ull * h->ny * h->nz * h->rbytes, h->stream));
     FC_HIP(h, hipMalloc(&h->dforces, 4 * sizeof(double)));
     FC_HIP(h, hipMalloc(&h->stat_part, kStatBlocks * 8 * sizeof(double)));
-    FC_HIP(h, hipMalloc(&h->stat_out, 8 * sizeof(double)));
+    FC_HIP(h, hipMalloc(&h->stat_out, 20 * sizeof(double)));  // also the aerodynamic-force sums
     FC_HIP(h, hipMalloc(&h->dflag, sizeof(int)));
     FC_HIP(h, hipMalloc(&h->dprobe, sizeof(int)));
     FC_HIP(h, hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
@@ -619,6 +620,69 @@ int fc_lbm_forces(fc_lbm* h, double out[4]) {
     if (rc) return rc;
     FC_HIP(h, hipMemcpyAsync(out, h->dforces, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
+}
+
+int fc_lbm_aero_forces(fc_lbm* h, float cell_size, const float origin[3], fc_lbm_aero* out) {
+    if (!h || !out || !origin) return FC_LBM_EINVAL;
+    *out = fc_lbm_aero{};
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    static_assert(kAeroBlocks * kAeroVals <= kStatBlocks * 8, "aero partials reuse the statistics buffer");
+    LaunchSurfaceForce(h);
+    const float ox = origin[0], oy = origin[1], oz = origin[2];
+    if (h->rbytes == 8)
+        hipLaunchKernelGGL(KAeroPartial<double>, dim3(kAeroBlocks), dim3(256), 0, h->stream, h->surf, h->nsurf,
+                           static_cast<const double*>(h->sforce), h->flags, h->nx, h->ny, h->nz, cell_size, ox, oy, oz,
+                           h->stat_part);
+    else
+        hipLaunchKernelGGL(KAeroPartial<float>, dim3(kAeroBlocks), dim3(256), 0, h->stream, h->surf, h->nsurf,
+                           static_cast<const float*>(h->sforce), h->flags, h->nx, h->ny, h->nz, cell_size, ox, oy, oz,
+                           h->stat_part);
+    hipLaunchKernelGGL(KAeroFinal, dim3(1), dim3(64), 0, h->stream, h->stat_part, h->stat_out);
+    FC_HIP(h, hipGetLastError());
+    double a[18];
+    FC_HIP(h, hipMemcpyAsync(a, h->stat_out, sizeof(a), hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    // Host epilogue, as FieldOverlay.cpp:553-618: stress range anchored at 0, centre of
+    // pressure by |Fz|, else by |F|, else the solid centroid, and the lift-arrow base on
+    // the top solid cell of the 5x5 columns around it.
+    out->force_x = static_cast<float>(a[0]);
+    out->force_y = static_cast<float>(a[1]);
+    out->force_z = static_cast<float>(a[2]);
+    out->has_nan = a[12] > 0.0 ? 1 : 0;
+    out->min_stress = 0.0f;
+    out->max_stress = a[11] > 0.0 ? static_cast<float>(a[11]) : 1e-6f;
+    float cp[3] = {0.0f, 0.0f, 0.0f};
+    const double solid_cells = a[13];
+    if (a[3] > 1e-9) {
+        for (int i = 0; i < 3; ++i) cp[i] = static_cast<float>(a[4 + i] / a[3]);
+    } else if (a[7] > 1e-9) {
+        for (int i = 0; i < 3; ++i) cp[i] = static_cast<float>(a[8 + i] / a[7]);
+    } else if (solid_cells > 0.0) {
+        const double inv = 1.0 / solid_cells;
+        for (int i = 0; i < 3; ++i) cp[i] = static_cast<float>(a[14 + i] * inv);
+    }
+    const int gx0 = std::clamp(static_cast<int>(std::floor((cp[1] - oy) / cell_size)), 0, std::max(h->nx - 1, 0));
+    const int gy0 = std::clamp(static_cast<int>(std::floor((cp[0] - ox) / cell_size)), 0, std::max(h->ny - 1, 0));
+    int col_top = -1;
+    for (int dy = -2; dy <= 2; ++dy) {
+        const int gy = std::clamp(gy0 + dy, 0, std::max(h->ny - 1, 0));
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int gx = std::clamp(gx0 + dx, 0, std::max(h->nx - 1, 0));
+            for (int z = h->nz - 1; z >= 0; --z) {
+                if (h->solid_host[(static_cast<size_t>(z) * h->ny + gy) * h->nx + gx]) {
+                    col_top = std::max(col_top, z);
+                    break;
+                }
+            }
+        }
+    }
+    const int top = col_top >= 0 ? col_top : std::max(static_cast<int>(a[17]), 0);
+    for (int i = 0; i < 3; ++i) out->center_of_pressure[i] = cp[i];
+    out->lift_arrow_base[0] = cp[0];
+    out->lift_arrow_base[1] = cp[1];
+    out->lift_arrow_base[2] = oz + (static_cast<float>(top) + 1.0f) * cell_size;
     return FC_LBM_OK;
 }
 

@@ -433,6 +433,121 @@ __global__ __launch_bounds__(1024) void KReduceForces(const R* __restrict__ sfor
     }
 }
 
+// ------------------------------------------------------- aerodynamic forces
+// FieldOverlay::ComputeAerodynamicForces (FieldOverlay.cpp:469-621) over the solid cells
+// of the current state, without the field readback.  The field value of a solid cell is
+// (stress, Fx, Fy, Fz) of KSurfaceForce rounded to float for surface cells and exactly
+// zero for the others, so the force and weighted-position sums run over the surface
+// list and only the geometric sums (count, centroid, top layer) visit every cell.
+// Cell centres follow VoxelGrid::CellCenter (VoxelGrid.h:49-51): grid-y along world X,
+// grid-x along world Y, in float as there; every sum is FP64.
+constexpr int kAeroBlocks = 256;
+constexpr int kAeroVals = 20;  // per-block partial: see KAeroPartial
+
+__device__ __forceinline__ void CellCenterWorld(int x, int y, int z, float cs, float ox, float oy, float oz, double& px,
+                                                double& py, double& pz) {
+    px = static_cast<double>(ox + (static_cast<float>(y) + 0.5f) * cs);
+    py = static_cast<double>(oy + (static_cast<float>(x) + 0.5f) * cs);
+    pz = static_cast<double>(oz + (static_cast<float>(z) + 0.5f) * cs);
+}
+
+// Pass 1, a fixed grid of kAeroBlocks blocks of 256 threads.  part[b*kAeroVals + i]:
+//   0-2 sum F, 3 sum |Fz|, 4-6 sum |Fz| pos, 7 sum |F|, 8-10 sum |F| pos,
+//   11 max positive stress (-1 if none), 12 non-finite seen,
+//   13 solid cells, 14-16 sum pos over solid cells, 17 top solid layer (-1 if none).
+template <class R>
+__global__ __launch_bounds__(256) void KAeroPartial(const int* __restrict__ surf, int nsurf,
+                                                    const R* __restrict__ sforce, const uint8_t* __restrict__ flags,
+                                                    int nx, int ny, int nz, float cs, float ox, float oy, float oz,
+                                                    double* __restrict__ part) {
+    double a[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) a[i] = 0.0;
+    a[11] = -1.0;
+    a[17] = -1.0;
+    const int stride = kAeroBlocks * 256;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < nsurf; k += stride) {
+        const int n = surf[k];
+        const int x = n % nx;
+        const int y = (n / nx) % (ny + 2) - 1;
+        const int z = n / (nx * (ny + 2)) - 1;
+        const float st = static_cast<float>(sforce[k * 4 + 0]);
+        const float fx = static_cast<float>(sforce[k * 4 + 1]);
+        const float fy = static_cast<float>(sforce[k * 4 + 2]);
+        const float fz = static_cast<float>(sforce[k * 4 + 3]);
+        if (!__builtin_isfinite(st) || !__builtin_isfinite(fx) || !__builtin_isfinite(fy) || !__builtin_isfinite(fz)) {
+            a[12] = 1.0;
+            continue;
+        }
+        double px, py, pz;
+        CellCenterWorld(x, y, z, cs, ox, oy, oz, px, py, pz);
+        const double dx = fx, dy = fy, dz = fz;
+        a[0] += dx;
+        a[1] += dy;
+        a[2] += dz;
+        const double wl = fabs(dz);
+        a[3] += wl;
+        a[4] += wl * px;
+        a[5] += wl * py;
+        a[6] += wl * pz;
+        const double wf = sqrt(dx * dx + dy * dy + dz * dz);
+        a[7] += wf;
+        a[8] += wf * px;
+        a[9] += wf * py;
+        a[10] += wf * pz;
+        if (st > 0.0f && static_cast<double>(st) > a[11]) a[11] = st;
+    }
+    const long long N = static_cast<long long>(nx) * ny * nz;
+    for (long long c = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; c < N; c += stride) {
+        int x, y, z;
+        Coords(c, nx, ny, x, y, z);
+        if (!(flags[(static_cast<long long>(z + 1) * (ny + 2) + (y + 1)) * nx + x] & kSolid)) continue;
+        double px, py, pz;
+        CellCenterWorld(x, y, z, cs, ox, oy, oz, px, py, pz);
+        a[13] += 1.0;
+        a[14] += px;
+        a[15] += py;
+        a[16] += pz;
+        a[17] = static_cast<double>(z) > a[17] ? static_cast<double>(z) : a[17];
+    }
+    // fixed-order tree reduction: sums, except the maxima (11, 12, 17)
+    __shared__ double v[18][256];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) v[i][threadIdx.x] = a[i];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const int o = threadIdx.x + s;
+#pragma unroll
+            for (int i = 0; i < 18; ++i) {
+                if (i == 11 || i == 12 || i == 17)
+                    v[i][threadIdx.x] = v[i][threadIdx.x] > v[i][o] ? v[i][threadIdx.x] : v[i][o];
+                else
+                    v[i][threadIdx.x] += v[i][o];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 18) part[blockIdx.x * kAeroVals + threadIdx.x] = v[threadIdx.x][0];
+}
+
+// Pass 2: one thread folds the kAeroBlocks partials in block order into out[0..17].
+__global__ void KAeroFinal(const double* __restrict__ part, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a[18];
+    for (int i = 0; i < 18; ++i) a[i] = part[i];
+    for (int b = 1; b < kAeroBlocks; ++b) {
+        const double* p = part + b * kAeroVals;
+        for (int i = 0; i < 18; ++i) {
+            if (i == 11 || i == 12 || i == 17)
+                a[i] = a[i] > p[i] ? a[i] : p[i];
+            else
+                a[i] += p[i];
+        }
+    }
+    for (int i = 0; i < 18; ++i) out[i] = a[i];
+}
+
 // Fluid statistics, pass 1: a fixed grid of kStatBlocks blocks, grid-stride over
 // cells, fixed-order block reduction.  part[b*8 + ...] = dsum, ssum, count, nan,
 // minD, maxD, maxS, unused.

@@ -15,7 +15,9 @@ The SDL handles of the reference signatures (``SDL_GPUDevice*``,
 ``SDL_GPUCommandBuffer*``) are accepted as ignored optional arguments.
 Extensions beyond the reference: storage precision and population layout at
 ``Init``, periodic y/z, FP64 force sums (``Forces``), exact state
-dump/restore (``Populations``/``SetPopulations``), on-device field export.
+dump/restore (``Populations``/``SetPopulations``), on-device field export, and
+FieldOverlay's aerodynamic-force summary computed on the device
+(``ComputeAerodynamicForces``).
 """
 from __future__ import annotations
 
@@ -60,6 +62,20 @@ class Params:
     def _c(self) -> L.FcLbmParams:
         return L.FcLbmParams(self.tau, self.inletVelocity, int(self.reverseFlow), int(self.freeSlipY),
                              int(self.freeSlipZ), self.smagorinskyCs)
+
+
+@dataclass
+class AerodynamicForces:
+    """FieldOverlay::AerodynamicForces (FlowCraft/FieldOverlay.h:186-195); vectors in the
+    VoxelGrid's world frame (grid-y along world X, grid-x along world Y)."""
+    forceX: float = 0.0
+    forceY: float = 0.0
+    forceZ: float = 0.0
+    minStress: float = 0.0
+    maxStress: float = 0.0
+    centerOfPressure: tuple = (0.0, 0.0, 0.0)
+    liftArrowBase: tuple = (0.0, 0.0, 0.0)
+    hasNaN: bool = False
 
 
 @dataclass
@@ -304,6 +320,18 @@ class LbmSolver:
         """Same sums into a float64 device tensor of 4, without synchronising."""
         if self.IsValid():
             self._check(self._lib.fc_lbm_forces_device(self._h, out_tensor.data_ptr()))
+
+    def ComputeAerodynamicForces(self, cellSize: float = 1.0, origin=(0.0, 0.0, 0.0)) -> AerodynamicForces:
+        """FieldOverlay::ComputeAerodynamicForces (FlowCraft/FieldOverlay.cpp:469-621) of the
+        current state, on the device (the reference takes the ReadbackField() result).
+        cellSize/origin: the VoxelGrid's world frame."""
+        if not self.IsValid():
+            return AerodynamicForces()
+        a = L.FcLbmAero()
+        o = (ctypes.c_float * 3)(*[float(v) for v in origin])
+        self._check(self._lib.fc_lbm_aero_forces(self._h, float(cellSize), o, ctypes.byref(a)))
+        return AerodynamicForces(a.force_x, a.force_y, a.force_z, a.min_stress, a.max_stress,
+                                 tuple(a.center_of_pressure), tuple(a.lift_arrow_base), bool(a.has_nan))
 
     def ProbeSliceHealth(self, zLayer: int) -> bool:
         """LbmSolver::ProbeSliceHealth (FlowCraft/LbmSolver.cpp:653-720): True = diverged."""

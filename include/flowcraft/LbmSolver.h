@@ -177,6 +177,35 @@ public:
         Check(fc_lbm_readback_field_range(h_, zStart, zCount, outField.data()));
     }
 
+    // Extension: FieldOverlay::ComputeAerodynamicForces (FlowCraft/FieldOverlay.cpp:469-621)
+    // of the current state, on the device; the reference computes it on the host from
+    // ReadbackField().  Members as FieldOverlay::AerodynamicForces
+    // (FlowCraft/FieldOverlay.h:186-195), world frame from the grid's cellSize/origin.
+    struct AerodynamicForces {
+        float forceX = 0.0f, forceY = 0.0f, forceZ = 0.0f;
+        float minStress = 0.0f, maxStress = 0.0f;
+        Vec3 centerOfPressure{0.0f, 0.0f, 0.0f};
+        Vec3 liftArrowBase{0.0f, 0.0f, 0.0f};
+        bool hasNaN = false;
+    };
+
+    AerodynamicForces ComputeAerodynamicForces(const VoxelGrid& grid) {
+        AerodynamicForces a;
+        if (!h_) return a;
+        const float origin[3] = {grid.origin.x, grid.origin.y, grid.origin.z};
+        fc_lbm_aero r{};
+        if (!Check(fc_lbm_aero_forces(h_, grid.cellSize, origin, &r))) return a;
+        a.forceX = r.force_x;
+        a.forceY = r.force_y;
+        a.forceZ = r.force_z;
+        a.minStress = r.min_stress;
+        a.maxStress = r.max_stress;
+        a.centerOfPressure = Vec3{r.center_of_pressure[0], r.center_of_pressure[1], r.center_of_pressure[2]};
+        a.liftArrowBase = Vec3{r.lift_arrow_base[0], r.lift_arrow_base[1], r.lift_arrow_base[2]};
+        a.hasNaN = r.has_nan != 0;
+        return a;
+    }
+
     bool ProbeSliceHealth(int zLayer) {
         if (!h_ || nz_ <= 0) return false;
         int diverged = 0;

@@ -151,6 +151,24 @@ int fc_lbm_forces(fc_lbm* h, double out[4]);
  * sample the force every few steps. */
 int fc_lbm_forces_device(fc_lbm* h, double* out_device);
 
+/* FieldOverlay::AerodynamicForces (FlowCraft/FieldOverlay.h:186-195): world-frame
+ * vectors, grid-y along world X and grid-x along world Y as VoxelGrid::CellCenter
+ * (FlowCraft/VoxelGrid.h:49-51). */
+typedef struct fc_lbm_aero {
+    float force_x, force_y, force_z;  /* sums over solid cells, FP64 then rounded */
+    float min_stress, max_stress;     /* colour range: 0 .. max positive stress (1e-6 if none) */
+    float center_of_pressure[3];      /* |Fz|-weighted, else |F|-weighted, else solid centroid */
+    float lift_arrow_base[3];         /* (cop.x, cop.y, top of the solid above it) */
+    int has_nan;                      /* a solid cell's field value was non-finite */
+} fc_lbm_aero;
+
+/* Replaces FieldOverlay::ComputeAerodynamicForces (FlowCraft/FieldOverlay.cpp:469-621)
+ * applied to ReadbackField(): the same quantities, computed on the device from the
+ * current populations in FP64 with a fixed reduction order, so no 4-float-per-cell
+ * field crosses to the host.  cell_size and origin[3] are the VoxelGrid's world frame.
+ * Blocking. */
+int fc_lbm_aero_forces(fc_lbm* h, float cell_size, const float origin[3], fc_lbm_aero* out);
+
 /* Replaces LbmSolver::ProbeSliceHealth (FlowCraft/LbmSolver.cpp:653-720): *diverged
  * = 1 if any cell of layer z (clamped) is non-finite, or a fluid cell has rho <= 0 or
  * |u|^2 > 4.  Blocking. */

@@ -21,6 +21,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(_HERE, "liblbm_oracle.so")
 REF_SO = os.path.join(_HERE, "_ref", "libref_lbm.so")
+REF_OVERLAY_SO = os.path.join(_HERE, "_ref", "libref_overlay.so")
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -82,6 +83,8 @@ def lib():
         L.orc_f32_to_f16_array.argtypes = [_c_float_p, _c_u16_p, ctypes.c_size_t]
         L.orc_f32_to_bf16_array.argtypes = [_c_float_p, _c_u16_p, ctypes.c_size_t]
         L.orc_f16_scale.restype = ctypes.c_float
+        L.orc_aero_forces.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_float_p, _c_u8_p,
+                                      _c_float_p, _c_float_p, ctypes.POINTER(ctypes.c_int)]
         L.orc_time_steps.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.orc_time_steps.restype = ctypes.c_double
         _lib = L
@@ -116,6 +119,49 @@ def ref_lib():
         L.ref_time_steps.restype = ctypes.c_double
         _ref = L
     return _ref
+
+
+AERO_FIELDS = ("forceX", "forceY", "forceZ", "minStress", "maxStress", "cop", "liftArrowBase", "hasNaN")
+
+
+def _aero_call(fn, solid, field, cell_size, origin) -> dict:
+    sol = np.ascontiguousarray(solid, dtype=np.uint8)
+    nz, ny, nx = sol.shape
+    f = np.ascontiguousarray(field, dtype=np.float32).reshape(-1)
+    assert f.size == sol.size * 4
+    o = np.ascontiguousarray(origin, dtype=np.float32)
+    out = np.zeros(11, np.float32)
+    nan = ctypes.c_int(0)
+    fn(nx, ny, nz, float(cell_size), _ptr(o, _c_float_p), _ptr(sol, _c_u8_p), _ptr(f, _c_float_p),
+       _ptr(out, _c_float_p), ctypes.byref(nan))
+    return dict(forceX=out[0], forceY=out[1], forceZ=out[2], minStress=out[3], maxStress=out[4],
+                cop=out[5:8].copy(), liftArrowBase=out[8:11].copy(), hasNaN=bool(nan.value))
+
+
+def aero_forces(solid: np.ndarray, field: np.ndarray, cell_size=1.0, origin=(0.0, 0.0, 0.0)) -> dict:
+    """Restated FieldOverlay::ComputeAerodynamicForces (FieldOverlay.cpp:469-621) on a
+    readback field; solid indexed [z, y, x]."""
+    return _aero_call(lib().orc_aero_forces, solid, field, cell_size, origin)
+
+
+def ref_overlay_available() -> bool:
+    return os.path.exists(REF_OVERLAY_SO)
+
+
+_ref_overlay = None
+
+
+def ref_aero_forces(solid: np.ndarray, field: np.ndarray, cell_size=1.0, origin=(0.0, 0.0, 0.0)) -> dict:
+    """FlowCraft's own FieldOverlay::ComputeAerodynamicForces, compiled into oracle/_ref."""
+    global _ref_overlay
+    if _ref_overlay is None:
+        if not ref_overlay_available():
+            raise RuntimeError("oracle/_ref/libref_overlay.so is not built (make -C oracle ref)")
+        L = ctypes.CDLL(REF_OVERLAY_SO)
+        L.ref_aero_forces.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_float_p, _c_u8_p,
+                                      _c_float_p, _c_float_p, ctypes.POINTER(ctypes.c_int)]
+        _ref_overlay = L
+    return _aero_call(_ref_overlay.ref_aero_forces, solid, field, cell_size, origin)
 
 
 class OracleLbm:

@@ -25,6 +25,8 @@
 //   Step ...................... FlowCraft/LbmReference.cpp:241-469
 //   ReadbackField ............. FlowCraft/LbmReference.cpp:475-500
 //   ComputeStats .............. FlowCraft/LbmReference.cpp:502-585
+//   ComputeAerodynamicForces .. FlowCraft/FieldOverlay.cpp:469-621 (on a readback field;
+//                               cell centres as VoxelGrid::CellCenter, VoxelGrid.h:49-51)
 // Extensions with no reference counterpart (pinned only by the FP64 build and
 // the invariants in tests/): periodic y/z wrap, FP16/BF16 offset storage.  The
 // offset formats run the engine's offset-space form of the same collision
@@ -820,7 +822,95 @@ void Dispatch(Handle* h, Fn fn) {
 
 }  // namespace
 
+// FieldOverlay::ComputeAerodynamicForces on a 4-floats/cell field (solid cells hold
+// (stress, Fx, Fy, Fz)), in one thread, cells in index order.  out[11] = Fx, Fy, Fz,
+// minStress, maxStress, centre of pressure (3), lift-arrow base (3).
+static void AeroForces(int nx, int ny, int nz, float cs, const float* origin, const uint8_t* solid, const float* field,
+                float* out, int* has_nan) {
+    for (int i = 0; i < 11; ++i) out[i] = 0.0f;
+    *has_nan = 0;
+    const size_t n = static_cast<size_t>(nx) * ny * nz;
+    if (n == 0) return;
+    double F[3] = {0, 0, 0}, lw = 0, lwp[3] = {0, 0, 0}, fw = 0, fwp[3] = {0, 0, 0}, gp[3] = {0, 0, 0};
+    size_t count = 0;
+    int top_any = -1;
+    float lo = 1e30f, hi = -1e30f;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t c = (static_cast<size_t>(z) * ny + y) * nx + x;
+                if (!solid[c]) continue;
+                // world position: grid-y along X, grid-x along Y, grid-z along Z
+                const float p[3] = {origin[0] + (y + 0.5f) * cs, origin[1] + (x + 0.5f) * cs,
+                                    origin[2] + (z + 0.5f) * cs};
+                for (int k = 0; k < 3; ++k) gp[k] += p[k];
+                ++count;
+                top_any = std::max(top_any, z);
+                const float* v = field + c * 4;
+                if (!std::isfinite(v[0]) || !std::isfinite(v[1]) || !std::isfinite(v[2]) || !std::isfinite(v[3])) {
+                    *has_nan = 1;
+                    continue;
+                }
+                for (int k = 0; k < 3; ++k) F[k] += v[1 + k];
+                const double wl = std::fabs(static_cast<double>(v[3]));
+                lw += wl;
+                for (int k = 0; k < 3; ++k) lwp[k] += wl * p[k];
+                const double wf = std::sqrt(static_cast<double>(v[1]) * v[1] + static_cast<double>(v[2]) * v[2] +
+                                            static_cast<double>(v[3]) * v[3]);
+                fw += wf;
+                for (int k = 0; k < 3; ++k) fwp[k] += wf * p[k];
+                if (v[0] > 0.0f) {
+                    lo = std::min(lo, v[0]);
+                    hi = std::max(hi, v[0]);
+                }
+            }
+    // stress colour range anchored at zero
+    if (lo > hi) {
+        lo = 0.0f;
+        hi = 1e-6f;
+    } else {
+        lo = 0.0f;
+        if (!(hi > lo)) hi = lo + 1e-6f;
+    }
+    float cp[3] = {0.0f, 0.0f, 0.0f};
+    if (lw > 1e-9) {
+        for (int k = 0; k < 3; ++k) cp[k] = static_cast<float>(lwp[k] / lw);
+    } else if (fw > 1e-9) {
+        for (int k = 0; k < 3; ++k) cp[k] = static_cast<float>(fwp[k] / fw);
+    } else if (count > 0) {
+        const double inv = 1.0 / static_cast<double>(count);
+        for (int k = 0; k < 3; ++k) cp[k] = static_cast<float>(gp[k] * inv);
+    }
+    // top solid cell of the 5x5 grid columns under the centre of pressure
+    const int cx = std::clamp(static_cast<int>(std::floor((cp[1] - origin[1]) / cs)), 0, std::max(nx - 1, 0));
+    const int cy = std::clamp(static_cast<int>(std::floor((cp[0] - origin[0]) / cs)), 0, std::max(ny - 1, 0));
+    int top = -1;
+    for (int dy = -2; dy <= 2; ++dy)
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int gx = std::clamp(cx + dx, 0, std::max(nx - 1, 0));
+            const int gy = std::clamp(cy + dy, 0, std::max(ny - 1, 0));
+            for (int z = nz - 1; z >= 0; --z)
+                if (solid[(static_cast<size_t>(z) * ny + gy) * nx + gx]) {
+                    top = std::max(top, z);
+                    break;
+                }
+        }
+    if (top < 0) top = std::max(top_any, 0);
+    for (int k = 0; k < 3; ++k) out[k] = static_cast<float>(F[k]);
+    out[3] = lo;
+    out[4] = hi;
+    for (int k = 0; k < 3; ++k) out[5 + k] = cp[k];
+    out[8] = cp[0];
+    out[9] = cp[1];
+    out[10] = origin[2] + (static_cast<float>(top) + 1.0f) * cs;
+}
+
 extern "C" {
+
+void orc_aero_forces(int nx, int ny, int nz, float cs, const float* origin, const uint8_t* solid, const float* field,
+                     float* out, int* has_nan) {
+    AeroForces(nx, ny, nz, cs, origin, solid, field, out, has_nan);
+}
 
 // real_bits: 32 or 64.  storage: 0 native, 1 FP16 offset, 2 BF16 offset (32-bit only).
 void* orc_create(int nx, int ny, int nz, int q, int real_bits, int storage, int periodic_y, int periodic_z,

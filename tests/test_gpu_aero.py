@@ -1,0 +1,93 @@
+"""On-device FieldOverlay::ComputeAerodynamicForces (fc_lbm_aero_forces) against the
+oracle's restatement (pinned to FlowCraft's own function in tests/test_aero_forces.py)
+applied to the oracle's field of the same run, and, at the 512x256x256 wing size,
+against the restatement applied to the engine's own read-back field."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from flowcraft_amd import geometry
+from flowcraft_amd.lbm import LatticeModel, Layout, Params, Storage
+from tests._cases import case_a_grid, case_a_params, make_engine, make_oracle, oracle_params, ulp_diff32, wing_grid
+
+pytestmark = pytest.mark.gpu
+
+FRAMES = [(1.0, (0.0, 0.0, 0.0)), (2.5, (-30.0, 12.25, -7.0))]
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu-case] {request.node.nodeid}", flush=True)
+    yield
+
+
+def _engine_dict(a) -> dict:
+    return dict(forceX=np.float32(a.forceX), forceY=np.float32(a.forceY), forceZ=np.float32(a.forceZ),
+                minStress=np.float32(a.minStress), maxStress=np.float32(a.maxStress),
+                cop=np.float32(a.centerOfPressure), liftArrowBase=np.float32(a.liftArrowBase), hasNaN=a.hasNaN)
+
+
+def _same(a: dict, b: dict, ulps: int = 2):
+    """FP64 sums in another order: float results within `ulps` ulp; range and flag exact."""
+    for k in ("forceX", "forceY", "forceZ"):
+        assert ulp_diff32(np.float32([a[k]]), np.float32([b[k]])) <= ulps, (k, a[k], b[k])
+    assert ulp_diff32(a["cop"], b["cop"]) <= ulps, (a["cop"], b["cop"])
+    assert ulp_diff32(a["liftArrowBase"], b["liftArrowBase"]) <= ulps, (a["liftArrowBase"], b["liftArrowBase"])
+    assert a["minStress"] == b["minStress"] and a["maxStress"] == b["maxStress"]
+    assert a["hasNaN"] == b["hasNaN"]
+
+
+@pytest.mark.parametrize("model,storage,layout", [
+    (LatticeModel.D3Q19, Storage.F32, Layout.PINGPONG),
+    (LatticeModel.D3Q19, Storage.F32, Layout.INPLACE),
+    (LatticeModel.D3Q15, Storage.F32, Layout.PINGPONG),
+    (LatticeModel.D3Q19, Storage.F16, Layout.PINGPONG),
+], ids=lambda v: v.name)
+@pytest.mark.parametrize("case", ["sphere", "wing"])
+def test_aero_forces_match_oracle(case, model, storage, layout):
+    g = case_a_grid() if case == "sphere" else wing_grid()
+    p = case_a_params()
+    eng = make_engine(g, model, storage, layout)
+    eng.Reset(p, True)
+    orc = make_oracle(g, model, storage)
+    oracle_params(orc, p)
+    orc.reset(True)
+    eng.Step(None, 30)
+    orc.step(30)
+    field = orc.field()
+    for cs, o in FRAMES:
+        a = _engine_dict(eng.ComputeAerodynamicForces(cs, o))
+        _same(a, oracle.aero_forces(g.solid, field, cs, o))
+        assert a["forceX"] > 0.0
+    eng.Shutdown()
+
+
+def test_aero_forces_quiescent_sphere_is_its_centroid():
+    """Rest state (f = w) with plain bounce-back: every momentum-exchange term is
+    (w + w) - 2w = 0 exactly, so the centre of pressure falls back to the solid
+    centroid, the sphere's centre."""
+    g = case_a_grid()
+    g.signedDistance = None
+    eng = make_engine(g)
+    eng.Reset(Params(tau=0.6, inletVelocity=0.0, smagorinskyCs=0.0), False)
+    a = eng.ComputeAerodynamicForces(1.0, (0.0, 0.0, 0.0))
+    assert (a.forceX, a.forceY, a.forceZ) == (0.0, 0.0, 0.0)
+    assert tuple(a.centerOfPressure) == (12.0, 20.0, 12.0)
+    assert a.maxStress == np.float32(1e-6) and not a.hasNaN
+    eng.Shutdown()
+
+
+def test_aero_forces_512x256x256_wing_matches_restatement_of_own_field():
+    g = geometry.naca_wing_grid(512, 256, 256, m=0.0, p=0.0, t=0.12, chord=128.0, le_x=128.0, le_y=128.0,
+                                aoa_deg=4.0)
+    eng = make_engine(g)
+    eng.Reset(Params(tau=0.5005, inletVelocity=0.08, smagorinskyCs=0.1), False)
+    eng.Step(None, 200)
+    a = _engine_dict(eng.ComputeAerodynamicForces(0.5, (-64.0, -128.0, 0.0)))
+    field = eng.ReadbackField()
+    b = oracle.aero_forces(g.solid, field, 0.5, (-64.0, -128.0, 0.0))
+    print(f"device {a}\nrestated {b}", flush=True)
+    _same(a, b)
+    eng.Shutdown()
