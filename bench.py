@@ -18,6 +18,10 @@ The CPU baseline times FlowCraft's LbmReference::Step (oracle/_ref) — or the
 oracle's restatement where that build is absent — on the host cores, over a
 bounded number of steps of the same lattice.
 
+`aero_forces` times FieldOverlay::ComputeAerodynamicForces (force sums, centre of
+pressure, lift-arrow base) on the device against FlowCraft's host path (full field
+readback plus the function, its own build, on the host cores) on the same state.
+
 `warm_start` is the metric's second half, measured live: on --warm-shapes held-out
 shapes of the warm-start distribution (256x128x128, u = 0.06, tau = 0.503, FP16
 storage), steps and wall time until the trailing-mean drag coefficient is within 1%
@@ -114,6 +118,54 @@ def _cpu_baseline_run(oracle, grid, p, cores, budget_s, max_steps):
                       f"process pinned to {cores} host cores"}
 
 
+def aero_forces_block(torch, grid, cpu=True, reps=20):
+    """FieldOverlay::ComputeAerodynamicForces of the wing after 200 steps: the engine's
+    on-device call (fc_lbm_aero_forces, blocking, wall clock per call) against FlowCraft's
+    path, ReadbackField() of the 537 MB field plus the function on the host (its own
+    build in oracle/_ref, on at most 16 host cores; the CPU-baseline leg), and whether the
+    two results agree."""
+    from flowcraft_amd.lbm import LatticeModel, LbmSolver, Params, Storage
+    frame = (0.5, (-64.0, -128.0, 0.0))
+    s = LbmSolver()
+    if not s.Init(None, grid, LatticeModel.D3Q19, storage=Storage.F32):
+        raise RuntimeError(s.LastError())
+    s.Reset(Params(tau=TAU, inletVelocity=0.08, smagorinskyCs=0.1), False)
+    s.Step(None, 200)
+    s.Sync()
+    for _ in range(2):
+        a = s.ComputeAerodynamicForces(*frame)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        a = s.ComputeAerodynamicForces(*frame)
+    dev_ms = (time.perf_counter() - t0) * 1e3 / reps
+    out = {"workload": f"{NX}x{NY}x{NZ} wing after 200 FP32 steps", "device_ms": round(dev_ms, 3),
+           "cop": [round(float(v), 4) for v in a.centerOfPressure]}
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        field = s.ReadbackField()
+        out["readback_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        fn = oracle.ref_aero_forces if oracle.ref_overlay_available() else oracle.aero_forces
+        saved = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, sorted(saved)[:16])
+        try:
+            t0 = time.perf_counter()
+            r = fn(grid.solid, field, *frame)
+            out["host_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        finally:
+            os.sched_setaffinity(0, saved)
+        out["host_kind"] = "reference" if fn is oracle.ref_aero_forces else "port"
+        out["host_cores"] = min(16, len(saved))
+        out["speedup"] = round((out["readback_ms"] + out["host_ms"]) / dev_ms, 1)
+        tol = lambda x, y: abs(float(x) - float(y)) <= 1e-6 * max(abs(float(y)), 1e-30)
+        out["match"] = bool(all(tol(getattr(a, k), r[k]) for k in ("forceX", "forceY", "forceZ")) and
+                            all(tol(x, y) for x, y in zip(a.centerOfPressure, r["cop"])) and
+                            all(tol(x, y) for x, y in zip(a.liftArrowBase, r["liftArrowBase"])) and
+                            a.maxStress == r["maxStress"])
+    s.Shutdown()
+    return out
+
+
 def warm_start_block(torch, n_shapes):
     """Live warm-vs-plain comparison on held-out shapes of the warm-start distribution
     (256x128x128 D3Q19 Smagorinsky Cs = 0.1, u = 0.06, tau = 0.503, FP16 storage, 6000-step
@@ -170,6 +222,8 @@ def main():
                          "prints the HIP-event average step time")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="storage of the --profile run")
+    ap.add_argument("--profile-aero", action="store_true",
+                    help="only the on-device aerodynamic-force summary, --steps calls (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this engine runs the lattice on a single MI355X "
@@ -185,6 +239,10 @@ def main():
     grid = wing_grid()
     print(f"# geometry {time.time() - t0:.1f}s, solid cells {int(grid.solid.sum())}", file=sys.stderr, flush=True)
 
+    if args.profile_aero:
+        print(json.dumps({"profile": "aero_forces", **aero_forces_block(torch, grid, cpu=False, reps=args.steps)}),
+              flush=True)
+        return
     if args.profile:
         st = {"f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16}[args.storage]
         r = time_variant(torch, grid, LatticeModel.D3Q19, st, layout, args.steps, args.warmup)
@@ -221,6 +279,7 @@ def main():
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "algorithmic_bytes_per_launch": bytes_per_cell * grid.CellCount(),
                 "kernel": f"fc::KStep<19, StoreF32, {layout.name.title().replace('place', 'Place')}>"}
+    aero = None if args.no_variants else aero_forces_block(torch, grid, cpu=not args.no_cpu)
     warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(grid)
     line = {
@@ -232,7 +291,7 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "warm_start": warm,
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "warm_start": warm,
         "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)

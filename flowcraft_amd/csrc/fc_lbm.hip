@@ -43,6 +43,11 @@ struct fc_lbm {
     double* dforces = nullptr;
     double* stat_part = nullptr;
     double* stat_out = nullptr;
+    double* aero_part = nullptr;  // per-block partials of the aerodynamic-force passes
+    double* aero_out = nullptr;   // [0, 13) surface sums, [16, 21) geometric sums
+    bool aero_geom_valid = false;  // geometric sums cached for aero_frame
+    float aero_frame[4] = {0, 0, 0, 0};
+    double aero_geom[kAeroGVals] = {0, 0, 0, 0, -1};
     int* dflag = nullptr;   // sticky divergence flag
     int* dprobe = nullptr;
     float* staging = nullptr;
@@ -309,7 +314,9 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     FC_HIP(h, hipMemsetAsync(h->side_mom, 0, 16ull * h->ny * h->nz * h->rbytes, h->stream));
     FC_HIP(h, hipMalloc(&h->dforces, 4 * sizeof(double)));
     FC_HIP(h, hipMalloc(&h->stat_part, kStatBlocks * 8 * sizeof(double)));
-    FC_HIP(h, hipMalloc(&h->stat_out, 20 * sizeof(double)));  // also the aerodynamic-force sums
+    FC_HIP(h, hipMalloc(&h->stat_out, 8 * sizeof(double)));
+    FC_HIP(h, hipMalloc(&h->aero_part, kAeroPartDoubles * sizeof(double)));
+    FC_HIP(h, hipMalloc(&h->aero_out, 24 * sizeof(double)));
     FC_HIP(h, hipMalloc(&h->dflag, sizeof(int)));
     FC_HIP(h, hipMalloc(&h->dprobe, sizeof(int)));
     FC_HIP(h, hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
@@ -327,6 +334,8 @@ int fc_lbm_destroy(fc_lbm* h) {
     DevFree(h->dforces);
     DevFree(h->stat_part);
     DevFree(h->stat_out);
+    DevFree(h->aero_part);
+    DevFree(h->aero_out);
     DevFree(h->dflag);
     DevFree(h->dprobe);
     DevFree(h->staging);
@@ -381,6 +390,7 @@ int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf) {
     FC_HIP(h, hipStreamSynchronize(h->stream));
     DevFree(dsdf);
     h->has_geometry = true;
+    h->aero_geom_valid = false;
     h->has_state = false;
     return FC_LBM_OK;
 }
@@ -628,22 +638,39 @@ int fc_lbm_aero_forces(fc_lbm* h, float cell_size, const float origin[3], fc_lbm
     *out = fc_lbm_aero{};
     int rc = CheckReady(h);
     if (rc) return rc;
-    static_assert(kAeroBlocks * kAeroVals <= kStatBlocks * 8, "aero partials reuse the statistics buffer");
-    LaunchSurfaceForce(h);
     const float ox = origin[0], oy = origin[1], oz = origin[2];
+    LaunchSurfaceForce(h);
     if (h->rbytes == 8)
-        hipLaunchKernelGGL(KAeroPartial<double>, dim3(kAeroBlocks), dim3(256), 0, h->stream, h->surf, h->nsurf,
-                           static_cast<const double*>(h->sforce), h->flags, h->nx, h->ny, h->nz, cell_size, ox, oy, oz,
-                           h->stat_part);
+        hipLaunchKernelGGL(KAeroSurface<double>, dim3(kAeroBlocks), dim3(256), 0, h->stream, h->surf, h->nsurf,
+                           static_cast<const double*>(h->sforce), h->nx, h->ny, cell_size, ox, oy, oz, h->aero_part);
     else
-        hipLaunchKernelGGL(KAeroPartial<float>, dim3(kAeroBlocks), dim3(256), 0, h->stream, h->surf, h->nsurf,
-                           static_cast<const float*>(h->sforce), h->flags, h->nx, h->ny, h->nz, cell_size, ox, oy, oz,
-                           h->stat_part);
-    hipLaunchKernelGGL(KAeroFinal, dim3(1), dim3(64), 0, h->stream, h->stat_part, h->stat_out);
+        hipLaunchKernelGGL(KAeroSurface<float>, dim3(kAeroBlocks), dim3(256), 0, h->stream, h->surf, h->nsurf,
+                           static_cast<const float*>(h->sforce), h->nx, h->ny, cell_size, ox, oy, oz, h->aero_part);
+    hipLaunchKernelGGL(KAeroFold<kAeroSVals>, dim3(1), dim3(256), 0, h->stream, h->aero_part, kAeroBlocks, true,
+                       false, h->aero_out);
+    // The geometric sums depend only on the geometry and the frame: computed on the first
+    // call for a frame and kept on the host.
+    const float frame[4] = {cell_size, ox, oy, oz};
+    const bool geom = !h->aero_geom_valid || std::memcmp(frame, h->aero_frame, sizeof(frame)) != 0;
+    if (geom) {
+        double* gpart = h->aero_part + kAeroBlocks * kAeroSVals;
+        hipLaunchKernelGGL(KAeroGeom, dim3(kGeomBlocks), dim3(256), 0, h->stream, h->flags, h->nx, h->ny, h->nz,
+                           cell_size, ox, oy, oz, gpart);
+        hipLaunchKernelGGL(KAeroFold<kAeroGVals>, dim3(1), dim3(256), 0, h->stream, gpart, kGeomBlocks, false, true,
+                           h->aero_out + 16);
+    }
     FC_HIP(h, hipGetLastError());
     double a[18];
-    FC_HIP(h, hipMemcpyAsync(a, h->stat_out, sizeof(a), hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipMemcpyAsync(a, h->aero_out, kAeroSVals * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (geom)
+        FC_HIP(h, hipMemcpyAsync(h->aero_geom, h->aero_out + 16, kAeroGVals * sizeof(double), hipMemcpyDeviceToHost,
+                                 h->stream));
     FC_HIP(h, hipStreamSynchronize(h->stream));
+    if (geom) {
+        std::memcpy(h->aero_frame, frame, sizeof(frame));
+        h->aero_geom_valid = true;
+    }
+    for (int i = 0; i < kAeroGVals; ++i) a[13 + i] = h->aero_geom[i];
     // Host epilogue, as FieldOverlay.cpp:553-618: stress range anchored at 0, centre of
     // pressure by |Fz|, else by |F|, else the solid centroid, and the lift-arrow base on
     // the top solid cell of the 5x5 columns around it.

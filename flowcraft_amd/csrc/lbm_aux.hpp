@@ -438,11 +438,15 @@ __global__ __launch_bounds__(1024) void KReduceForces(const R* __restrict__ sfor
 // of the current state, without the field readback.  The field value of a solid cell is
 // (stress, Fx, Fy, Fz) of KSurfaceForce rounded to float for surface cells and exactly
 // zero for the others, so the force and weighted-position sums run over the surface
-// list and only the geometric sums (count, centroid, top layer) visit every cell.
-// Cell centres follow VoxelGrid::CellCenter (VoxelGrid.h:49-51): grid-y along world X,
-// grid-x along world Y, in float as there; every sum is FP64.
-constexpr int kAeroBlocks = 256;
-constexpr int kAeroVals = 20;  // per-block partial: see KAeroPartial
+// list; the geometric sums (solid count, centroid, top layer) depend only on the
+// geometry and the world frame and are computed once per frame.  Cell centres follow
+// VoxelGrid::CellCenter (VoxelGrid.h:49-51): grid-y along world X, grid-x along world
+// Y, in float as there; every sum is FP64, reduced in a fixed order.
+constexpr int kAeroBlocks = 256;   // KAeroSurface grid
+constexpr int kAeroSVals = 13;     // its per-block partial
+constexpr int kGeomBlocks = 512;   // KAeroGeom grid
+constexpr int kAeroGVals = 5;      // its per-block partial
+constexpr int kAeroPartDoubles = kAeroBlocks * kAeroSVals + kGeomBlocks * kAeroGVals;
 
 __device__ __forceinline__ void CellCenterWorld(int x, int y, int z, float cs, float ox, float oy, float oz, double& px,
                                                 double& py, double& pz) {
@@ -451,22 +455,35 @@ __device__ __forceinline__ void CellCenterWorld(int x, int y, int z, float cs, f
     pz = static_cast<double>(oz + (static_cast<float>(z) + 0.5f) * cs);
 }
 
-// Pass 1, a fixed grid of kAeroBlocks blocks of 256 threads.  part[b*kAeroVals + i]:
-//   0-2 sum F, 3 sum |Fz|, 4-6 sum |Fz| pos, 7 sum |F|, 8-10 sum |F| pos,
-//   11 max positive stress (-1 if none), 12 non-finite seen,
-//   13 solid cells, 14-16 sum pos over solid cells, 17 top solid layer (-1 if none).
-template <class R>
-__global__ __launch_bounds__(256) void KAeroPartial(const int* __restrict__ surf, int nsurf,
-                                                    const R* __restrict__ sforce, const uint8_t* __restrict__ flags,
-                                                    int nx, int ny, int nz, float cs, float ox, float oy, float oz,
-                                                    double* __restrict__ part) {
-    double a[18];
+// Fixed-order tree reduction of V values over a 256-thread block; `is_max` selects
+// max instead of sum per value.  Returns the block result in v[i][0].
+template <int V>
+__device__ __forceinline__ void BlockReduce256(double (&v)[V][256], const bool (&is_max)[V]) {
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const int o = threadIdx.x + s;
 #pragma unroll
-    for (int i = 0; i < 18; ++i) a[i] = 0.0;
+            for (int i = 0; i < V; ++i)
+                v[i][threadIdx.x] = is_max[i] ? (v[i][threadIdx.x] > v[i][o] ? v[i][threadIdx.x] : v[i][o])
+                                              : v[i][threadIdx.x] + v[i][o];
+        }
+        __syncthreads();
+    }
+}
+
+// Surface pass: part[b*kAeroSVals + i] = 0-2 sum F, 3 sum |Fz|, 4-6 sum |Fz| pos,
+// 7 sum |F|, 8-10 sum |F| pos, 11 max positive stress (-1 if none), 12 non-finite seen.
+template <class R>
+__global__ __launch_bounds__(256) void KAeroSurface(const int* __restrict__ surf, int nsurf,
+                                                    const R* __restrict__ sforce, int nx, int ny, float cs, float ox,
+                                                    float oy, float oz, double* __restrict__ part) {
+    __shared__ double v[kAeroSVals][256];
+    double a[kAeroSVals];
+#pragma unroll
+    for (int i = 0; i < kAeroSVals; ++i) a[i] = 0.0;
     a[11] = -1.0;
-    a[17] = -1.0;
-    const int stride = kAeroBlocks * 256;
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < nsurf; k += stride) {
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < nsurf; k += kAeroBlocks * 256) {
         const int n = surf[k];
         const int x = n % nx;
         const int y = (n / nx) % (ny + 2) - 1;
@@ -497,55 +514,84 @@ __global__ __launch_bounds__(256) void KAeroPartial(const int* __restrict__ surf
         a[10] += wf * pz;
         if (st > 0.0f && static_cast<double>(st) > a[11]) a[11] = st;
     }
-    const long long N = static_cast<long long>(nx) * ny * nz;
-    for (long long c = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; c < N; c += stride) {
-        int x, y, z;
-        Coords(c, nx, ny, x, y, z);
-        if (!(flags[(static_cast<long long>(z + 1) * (ny + 2) + (y + 1)) * nx + x] & kSolid)) continue;
-        double px, py, pz;
-        CellCenterWorld(x, y, z, cs, ox, oy, oz, px, py, pz);
-        a[13] += 1.0;
-        a[14] += px;
-        a[15] += py;
-        a[16] += pz;
-        a[17] = static_cast<double>(z) > a[17] ? static_cast<double>(z) : a[17];
-    }
-    // fixed-order tree reduction: sums, except the maxima (11, 12, 17)
-    __shared__ double v[18][256];
 #pragma unroll
-    for (int i = 0; i < 18; ++i) v[i][threadIdx.x] = a[i];
+    for (int i = 0; i < kAeroSVals; ++i) v[i][threadIdx.x] = a[i];
+    constexpr bool mx[kAeroSVals] = {false, false, false, false, false, false, false,
+                                     false, false, false, false, true,  true};
+    BlockReduce256<kAeroSVals>(v, mx);
+    if (threadIdx.x < kAeroSVals) part[blockIdx.x * kAeroSVals + threadIdx.x] = v[threadIdx.x][0];
+}
+
+// Geometry pass over whole rows of the flag array (no per-cell division):
+// part[b*kAeroGVals + i] = 0 solid cells, 1-3 sum pos, 4 top solid layer (-1 if none).
+__global__ __launch_bounds__(256) void KAeroGeom(const uint8_t* __restrict__ flags, int nx, int ny, int nz, float cs,
+                                                 float ox, float oy, float oz, double* __restrict__ part) {
+    __shared__ double v[kAeroGVals][256];
+    double cnt = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, top = -1.0;
+    const int rows = ny * nz;
+    for (int r = blockIdx.x; r < rows; r += kGeomBlocks) {
+        const int z = r / ny, y = r - (r / ny) * ny;
+        const uint8_t* row = flags + (static_cast<long long>(z + 1) * (ny + 2) + (y + 1)) * nx;
+        const double py = static_cast<double>(ox + (static_cast<float>(y) + 0.5f) * cs);  // world X
+        const double pz = static_cast<double>(oz + (static_cast<float>(z) + 0.5f) * cs);
+        double c = 0.0, s = 0.0;
+        for (int x = threadIdx.x; x < nx; x += 256) {
+            if (!(row[x] & kSolid)) continue;
+            c += 1.0;
+            s += static_cast<double>(oy + (static_cast<float>(x) + 0.5f) * cs);  // world Y
+        }
+        cnt += c;
+        sx += c * py;
+        sy += s;
+        sz += c * pz;
+        if (c > 0.0 && z > top) top = z;
+    }
+    v[0][threadIdx.x] = cnt;
+    v[1][threadIdx.x] = sx;
+    v[2][threadIdx.x] = sy;
+    v[3][threadIdx.x] = sz;
+    v[4][threadIdx.x] = top;
+    constexpr bool mx[kAeroGVals] = {false, false, false, false, true};
+    BlockReduce256<kAeroGVals>(v, mx);
+    if (threadIdx.x < kAeroGVals) part[blockIdx.x * kAeroGVals + threadIdx.x] = v[threadIdx.x][0];
+}
+
+// Folds `blocks` partials of V values (fixed order: each thread its run of consecutive
+// blocks, then a tree) into out[0..V).  One block of 256 threads.
+template <int V>
+__global__ __launch_bounds__(256) void KAeroFold(const double* __restrict__ part, int blocks, bool max_last_two,
+                                                 bool max_last, double* __restrict__ out) {
+    __shared__ double v[V][256];
+    bool mx[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) mx[i] = (max_last_two && i >= V - 2) || (max_last && i == V - 1);
+    const int per = (blocks + 255) / 256;
+    double a[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) a[i] = mx[i] ? -1.0 : 0.0;
+    for (int j = 0; j < per; ++j) {
+        const int b = threadIdx.x * per + j;
+        if (b >= blocks) break;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const double p = part[b * V + i];
+            a[i] = mx[i] ? (a[i] > p ? a[i] : p) : a[i] + p;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i][threadIdx.x] = a[i];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             const int o = threadIdx.x + s;
 #pragma unroll
-            for (int i = 0; i < 18; ++i) {
-                if (i == 11 || i == 12 || i == 17)
-                    v[i][threadIdx.x] = v[i][threadIdx.x] > v[i][o] ? v[i][threadIdx.x] : v[i][o];
-                else
-                    v[i][threadIdx.x] += v[i][o];
-            }
+            for (int i = 0; i < V; ++i)
+                v[i][threadIdx.x] = mx[i] ? (v[i][threadIdx.x] > v[i][o] ? v[i][threadIdx.x] : v[i][o])
+                                          : v[i][threadIdx.x] + v[i][o];
         }
         __syncthreads();
     }
-    if (threadIdx.x < 18) part[blockIdx.x * kAeroVals + threadIdx.x] = v[threadIdx.x][0];
-}
-
-// Pass 2: one thread folds the kAeroBlocks partials in block order into out[0..17].
-__global__ void KAeroFinal(const double* __restrict__ part, double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double a[18];
-    for (int i = 0; i < 18; ++i) a[i] = part[i];
-    for (int b = 1; b < kAeroBlocks; ++b) {
-        const double* p = part + b * kAeroVals;
-        for (int i = 0; i < 18; ++i) {
-            if (i == 11 || i == 12 || i == 17)
-                a[i] = a[i] > p[i] ? a[i] : p[i];
-            else
-                a[i] += p[i];
-        }
-    }
-    for (int i = 0; i < 18; ++i) out[i] = a[i];
+    if (threadIdx.x < V) out[threadIdx.x] = v[threadIdx.x][0];
 }
 
 // Fluid statistics, pass 1: a fixed grid of kStatBlocks blocks, grid-stride over
