@@ -2,6 +2,7 @@
 
     python -m flowcraft_amd.studies precision [--steps 5000] [--every 50] [--window 1000]
     python -m flowcraft_amd.studies large [--steps 200] [--warmup 50]
+    python -m flowcraft_amd.studies drain [--steps 20000] [--every 1000]
 
 precision — the 512x256x256 NACA 0012 wing (chord 128, leading edge at x = 128,
     angle of attack 4 deg, extruded over z; D3Q19 and D3Q15 Smagorinsky Cs = 0.1,
@@ -22,6 +23,10 @@ large — the 768x512x576 bandwidth lattice (226M cells): an ellipsoid of semi-a
     Smagorinsky Cs = 0.1, u = 0.06, tau = 0.501, free-slip y/z; `warmup` untimed
     steps, then `steps` steps timed with HIP events on the solver's stream.  FP32 in
     place (one 17 GB population copy) and FP16 offsets.
+
+drain — the precision case's D3Q15 run (tau 0.6) from rest in FP32 and FP16: mean
+    density, mean speed and Cd every 1000 steps, showing how the channel's excess mass
+    drains through the outlet in each format.
 """
 from __future__ import annotations
 
@@ -160,6 +165,30 @@ def large_lattice(steps=200, warmup=50, storages=("f32", "f16"), layout=Layout.I
     return out
 
 
+def density_drain(steps=20000, every=1000, model="D3Q15", tau=TAU_D3Q15, storages=("f32", "f16"), log=print):
+    """The 512x256x256 wing from rest (u 0.08, Cs 0.1), mean density, mean speed and Cd
+    every `every` steps per storage: how the channel's excess mass drains through the
+    outlet in each population format."""
+    grid = c2_grid()
+    q = 0.5 * 0.08 * 0.08 * 128.0 * 256
+    st_map = {"f64": Storage.F64, "f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16}
+    out = {"workload": f"512x256x256 NACA 0012 chord 128 AoA 4 deg, {model} tau {tau}, Cs 0.1, u 0.08, free-slip y/z",
+           "every": every, "runs": {}}
+    for sname in storages:
+        s = _solver(grid, LatticeModel[model], st_map[sname], Layout.PINGPONG)
+        s.Reset(Params(tau=tau, inletVelocity=0.08, smagorinskyCs=0.1), False)
+        rows = []
+        for k in range(steps // every):
+            s.Step(None, every)
+            t = s.ReadbackStats()
+            rows.append({"step": (k + 1) * every, "avg_density": t.avgDensity, "avg_speed": t.avgSpeed,
+                         "total_mass": t.totalMass, "cd": float(s.Forces()[0] / q)})
+            log(f"  {sname} {rows[-1]}")
+        s.Shutdown()
+        out["runs"][sname] = rows
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m flowcraft_amd.studies")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -178,6 +207,11 @@ def main(argv=None):
     g.add_argument("--storages", nargs="+", default=["f32", "f16"])
     g.add_argument("--layout", default="inplace", choices=["inplace", "pingpong"])
     g.add_argument("--out", default=None)
+    d = sub.add_parser("drain")
+    d.add_argument("--steps", type=int, default=20000)
+    d.add_argument("--every", type=int, default=1000)
+    d.add_argument("--storages", nargs="+", default=["f32", "f16"])
+    d.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     import torch
     torch.cuda.init()
@@ -188,6 +222,8 @@ def main(argv=None):
     if a.cmd == "precision":
         r = precision_study(a.steps, a.every, a.window, tuple(a.models), tuple(a.storages), log=log, tau=a.tau,
                             tau15=a.tau15)
+    elif a.cmd == "drain":
+        r = density_drain(a.steps, a.every, storages=tuple(a.storages), log=log)
     else:
         r = large_lattice(a.steps, a.warmup, tuple(a.storages),
                           Layout.INPLACE if a.layout == "inplace" else Layout.PINGPONG, log=log)
