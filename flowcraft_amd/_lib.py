@@ -26,7 +26,7 @@ EXPORTS = (
     "fc_lbm_forces", "fc_lbm_forces_device", "fc_lbm_probe_health", "fc_lbm_has_diverged",
     "fc_lbm_get_populations", "fc_lbm_set_populations", "fc_lbm_sync", "fc_lbm_info", "fc_lbm_step_count",
     "fc_lbm_population_bytes", "fc_lbm_stream", "fc_lbm_boundary_links", "fc_lbm_surface_cells",
-    "fc_lbm_aero_forces",
+    "fc_lbm_aero_forces", "fc_vox_params_default", "fc_voxelize_frame", "fc_voxelize", "fc_voxelize_last_error",
 )
 
 
@@ -54,6 +54,15 @@ class FcLbmAero(ctypes.Structure):
                 ("min_stress", ctypes.c_float), ("max_stress", ctypes.c_float),
                 ("center_of_pressure", ctypes.c_float * 3), ("lift_arrow_base", ctypes.c_float * 3),
                 ("has_nan", ctypes.c_int)]
+
+
+class FcVoxParams(ctypes.Structure):
+    _fields_ = [("target_resolution", ctypes.c_int), ("padding", ctypes.c_float * 3)]
+
+
+class FcVoxGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("cell_size", ctypes.c_float),
+                ("origin", ctypes.c_float * 3)]
 
 
 _lib = None
@@ -106,6 +115,12 @@ def load() -> ctypes.CDLL:
     L.fc_lbm_stream.restype = ctypes.c_void_p
     L.fc_lbm_boundary_links.argtypes = [h, ctypes.c_longlong, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_longlong)]
     L.fc_lbm_aero_forces.argtypes = [h, ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(FcLbmAero)]
+    ll = ctypes.c_longlong
+    L.fc_vox_params_default.argtypes = [ctypes.POINTER(FcVoxParams)]
+    L.fc_voxelize_frame.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), ctypes.POINTER(FcVoxGrid)]
+    L.fc_voxelize.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), vp, vp]
+    L.fc_voxelize_last_error.argtypes = []
+    L.fc_voxelize_last_error.restype = ctypes.c_char_p
     L.fc_lbm_surface_cells.argtypes = [h]
     L.fc_lbm_surface_cells.restype = ctypes.c_longlong
     _lib = L

@@ -183,6 +183,45 @@ int fc_lbm_has_diverged(fc_lbm* h, int* diverged);
 int fc_lbm_get_populations(fc_lbm* h, void* out);
 int fc_lbm_set_populations(fc_lbm* h, const void* in);
 
+/* ---------------------------------------------------------------- voxelizer
+ * FlowCraft's mesh voxelizer on the device.  Meshes are world-space vertex positions
+ * (3 floats each) and triangle indices (3 per triangle, outward counter-clockwise
+ * winding); grids use the VoxelGrid conventions (grid-x along world Y, grid-y along
+ * world X, cell (x, y, z) at index (z*ny + y)*nx + x). */
+
+/* Voxelizer::Params (FlowCraft/Voxelizer.h:13-30), same defaults via
+ * fc_vox_params_default: 640 cells along the longest padded axis, padding (0.3, 1.0,
+ * 0.5) of the mesh extent per world axis. */
+typedef struct fc_vox_params {
+    int target_resolution;
+    float padding[3];
+} fc_vox_params;
+
+typedef struct fc_vox_grid {
+    int nx, ny, nz;   /* 0, 0, 0 for an empty mesh (the reference's empty VoxelGrid) */
+    float cell_size;
+    float origin[3];  /* world position of cell (0,0,0)'s min corner */
+} fc_vox_grid;
+
+int fc_vox_params_default(fc_vox_params* p);
+
+/* The grid Voxelizer::Voxelize (FlowCraft/Voxelizer.cpp:128-166) lays over a mesh:
+ * dimensions, cell size and origin from the padded bounds, so the caller can size the
+ * outputs of fc_voxelize.  Host arrays; an out-of-range index is FC_LBM_EINVAL. */
+int fc_voxelize_frame(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
+                      const fc_vox_params* params, fc_vox_grid* out);
+
+/* Replaces Voxelizer::Voxelize (FlowCraft/Voxelizer.cpp:128-432): the solid mask
+ * (nx*ny*nz bytes, 1 = solid) and the signed distance (nx*ny*nz floats, + in fluid, -
+ * in solid, +-inf far from the surface) of the frame above, into host arrays,
+ * identical to the reference's.  Computed on the current device; blocking.  Nothing
+ * is written for an empty mesh. */
+int fc_voxelize(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
+                const fc_vox_params* params, uint8_t* solid, float* sdf);
+
+/* Message of the calling thread's last voxelizer error. */
+const char* fc_voxelize_last_error(void);
+
 /* Waits for all work queued on the handle's stream. */
 int fc_lbm_sync(fc_lbm* h);
 

@@ -85,6 +85,11 @@ def lib():
         L.orc_f16_scale.restype = ctypes.c_float
         L.orc_aero_forces.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_float_p, _c_u8_p,
                                       _c_float_p, _c_float_p, ctypes.POINTER(ctypes.c_int)]
+        L.orc_vox_frame.argtypes = [_c_float_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, _c_float_p,
+                                    ctypes.POINTER(ctypes.c_int), _c_float_p]
+        L.orc_vox_frame.restype = ctypes.c_int
+        L.orc_voxelize.argtypes = [_c_float_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_uint32), ctypes.c_longlong,
+                                   ctypes.c_int, _c_float_p, _c_u8_p, _c_float_p]
         L.orc_time_steps.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.orc_time_steps.restype = ctypes.c_double
         _lib = L
@@ -162,6 +167,80 @@ def ref_aero_forces(solid: np.ndarray, field: np.ndarray, cell_size=1.0, origin=
                                       _c_float_p, _c_float_p, ctypes.POINTER(ctypes.c_int)]
         _ref_overlay = L
     return _aero_call(_ref_overlay.ref_aero_forces, solid, field, cell_size, origin)
+
+
+REF_VOXELIZER_SO = os.path.join(_HERE, "_ref", "libref_voxelizer.so")
+VOX_DEFAULT_PADDING = (0.3, 1.0, 0.5)  # Voxelizer::Params::paddingFactor (Voxelizer.h:29)
+
+
+def _mesh_args(positions, indices):
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+    return p, i
+
+
+def vox_frame(positions, indices, target=640, padding=VOX_DEFAULT_PADDING):
+    """(nx, ny, nz), cellSize, origin of Voxelizer::Voxelize's grid for a mesh."""
+    L = lib()
+    p, i = _mesh_args(positions, indices)
+    dims = np.zeros(3, np.int32)
+    fr = np.zeros(4, np.float32)
+    pad = np.ascontiguousarray(padding, dtype=np.float32)
+    L.orc_vox_frame(_ptr(p, _c_float_p), len(p), len(i), int(target), _ptr(pad, _c_float_p),
+                    dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(fr, _c_float_p))
+    return tuple(int(v) for v in dims), float(fr[0]), tuple(float(v) for v in fr[1:])
+
+
+def voxelize(positions, indices, target=640, padding=VOX_DEFAULT_PADDING):
+    """Restated Voxelizer::Voxelize: (solid uint8 [nz, ny, nx], sdf float32 [nz, ny, nx],
+    cellSize, origin)."""
+    L = lib()
+    p, i = _mesh_args(positions, indices)
+    (nx, ny, nz), cs, org = vox_frame(p, i, target, padding)
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    sdf = np.zeros((nz, ny, nx), np.float32)
+    pad = np.ascontiguousarray(padding, dtype=np.float32)
+    if solid.size:
+        L.orc_voxelize(_ptr(p, _c_float_p), len(p), _ptr(i, ctypes.POINTER(ctypes.c_uint32)), len(i), int(target),
+                       _ptr(pad, _c_float_p), _ptr(solid, _c_u8_p), _ptr(sdf, _c_float_p))
+    return solid, sdf, cs, org
+
+
+def ref_voxelizer_available() -> bool:
+    return os.path.exists(REF_VOXELIZER_SO)
+
+
+_ref_vox = None
+
+
+def ref_voxelize(positions, indices, target=640, padding=VOX_DEFAULT_PADDING):
+    """FlowCraft's own Voxelizer::Voxelize (oracle/_ref), same return as voxelize()."""
+    global _ref_vox
+    if _ref_vox is None:
+        if not ref_voxelizer_available():
+            raise RuntimeError("oracle/_ref/libref_voxelizer.so is not built (make -C oracle ref)")
+        L = ctypes.CDLL(REF_VOXELIZER_SO)
+        L.ref_voxelize.restype = ctypes.c_void_p
+        L.ref_voxelize.argtypes = [_c_float_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_uint32), ctypes.c_longlong,
+                                   ctypes.c_int, _c_float_p]
+        L.ref_vox_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _c_float_p]
+        L.ref_vox_copy.argtypes = [ctypes.c_void_p, _c_u8_p, _c_float_p]
+        L.ref_vox_free.argtypes = [ctypes.c_void_p]
+        _ref_vox = L
+    L = _ref_vox
+    p, i = _mesh_args(positions, indices)
+    pad = np.ascontiguousarray(padding, dtype=np.float32)
+    h = L.ref_voxelize(_ptr(p, _c_float_p), len(p), _ptr(i, ctypes.POINTER(ctypes.c_uint32)), len(i), int(target),
+                       _ptr(pad, _c_float_p))
+    dims = np.zeros(3, np.int32)
+    fr = np.zeros(4, np.float32)
+    L.ref_vox_info(h, dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(fr, _c_float_p))
+    nx, ny, nz = (int(v) for v in dims)
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    sdf = np.zeros((nz, ny, nx), np.float32)
+    L.ref_vox_copy(h, _ptr(solid, _c_u8_p), _ptr(sdf, _c_float_p))
+    L.ref_vox_free(h)
+    return solid, sdf, float(fr[0]), tuple(float(v) for v in fr[1:])
 
 
 class OracleLbm:
