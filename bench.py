@@ -22,6 +22,10 @@ bounded number of steps of the same lattice.
 pressure, lift-arrow base) on the device against FlowCraft's host path (full field
 readback plus the function, its own build, on the host cores) on the same state.
 
+`voxelize` times the device mesh voxelizer against FlowCraft's own Voxelizer::Voxelize
+on the host cores, on a 241,844-triangle aircraft mesh at resolution 640, and checks the
+grids are identical.
+
 `warm_start` is the metric's second half, measured live: on --warm-shapes held-out
 shapes of the warm-start distribution (256x128x128, u = 0.06, tau = 0.503, FP16
 storage), steps and wall time until the trailing-mean drag coefficient is within 1%
@@ -166,6 +170,44 @@ def aero_forces_block(torch, grid, cpu=True, reps=20):
     return out
 
 
+def voxelize_block(cpu=True, reps=3, detail=2, res=640):
+    """Voxelizer::Voxelize of a synthetic high-wing aircraft (meshes.aircraft: 241,844
+    triangles at detail 2) at FlowCraft's default resolution: the device voxelizer
+    (fc_voxelize, wall clock per blocking call: host setup, transfers and kernels, the
+    44M-cell mask and distance field landing in host memory) against FlowCraft's own
+    voxelizer (oracle/_ref, its threads on at most 16 host cores; the CPU-baseline
+    leg), and whether the grids are identical."""
+    from flowcraft_amd import meshes
+    from flowcraft_amd.voxelizer import Params, Voxelizer
+    pos, idx = meshes.aircraft(detail=detail)
+    p = Params(res)
+    g = Voxelizer.Voxelize(pos, idx, p)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = Voxelizer.Voxelize(pos, idx, p)
+    dev_ms = (time.perf_counter() - t0) * 1e3 / reps
+    out = {"workload": f"aircraft mesh, {len(idx) // 3} triangles, resolution {res}: grid {g.nx}x{g.ny}x{g.nz} "
+                       f"({g.CellCount() / 1e6:.1f}M cells)", "device_ms": round(dev_ms, 1),
+           "solid_cells": int(g.solid.sum())}
+    if cpu:
+        import oracle
+        fn = oracle.ref_voxelize if oracle.ref_voxelizer_available() else oracle.voxelize
+        saved = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, sorted(saved)[:16])
+        try:
+            t0 = time.perf_counter()
+            solid, sdf, _, _ = fn(pos, idx, res)
+            out["host_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        finally:
+            os.sched_setaffinity(0, saved)
+        out["host_kind"] = "reference" if fn is oracle.ref_voxelize else "port"
+        out["host_cores"] = min(16, len(saved))
+        out["speedup"] = round(out["host_ms"] / dev_ms, 1)
+        out["match"] = bool(np.array_equal(solid, g.solid) and
+                            np.array_equal(sdf.view(np.uint32), g.signedDistance.view(np.uint32)))
+    return out
+
+
 def warm_start_block(torch, n_shapes):
     """Live warm-vs-plain comparison on held-out shapes of the warm-start distribution
     (256x128x128 D3Q19 Smagorinsky Cs = 0.1, u = 0.06, tau = 0.503, FP16 storage, 6000-step
@@ -224,6 +266,8 @@ def main():
                     help="storage of the --profile run")
     ap.add_argument("--profile-aero", action="store_true",
                     help="only the on-device aerodynamic-force summary, --steps calls (for rocprofv3 runs)")
+    ap.add_argument("--profile-vox", action="store_true",
+                    help="only the device voxelizer on the aircraft mesh, --steps calls (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this engine runs the lattice on a single MI355X "
@@ -239,6 +283,9 @@ def main():
     grid = wing_grid()
     print(f"# geometry {time.time() - t0:.1f}s, solid cells {int(grid.solid.sum())}", file=sys.stderr, flush=True)
 
+    if args.profile_vox:
+        print(json.dumps({"profile": "voxelize", **voxelize_block(cpu=False, reps=args.steps)}), flush=True)
+        return
     if args.profile_aero:
         print(json.dumps({"profile": "aero_forces", **aero_forces_block(torch, grid, cpu=False, reps=args.steps)}),
               flush=True)
@@ -280,6 +327,7 @@ def main():
                 "algorithmic_bytes_per_launch": bytes_per_cell * grid.CellCount(),
                 "kernel": f"fc::KStep<19, StoreF32, {layout.name.title().replace('place', 'Place')}>"}
     aero = None if args.no_variants else aero_forces_block(torch, grid, cpu=not args.no_cpu)
+    vox = None if args.no_variants else voxelize_block(cpu=not args.no_cpu)
     warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(grid)
     line = {
@@ -291,7 +339,7 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "warm_start": warm,
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "warm_start": warm,
         "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)

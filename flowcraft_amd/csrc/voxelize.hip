@@ -15,10 +15,11 @@
 //                identically.  Triangles are staged through LDS in chunks.
 //   flood fill   the exterior is the 6-connected component of cell (0,0,0) among the
 //                non-shell cells (Voxelizer.cpp:295-412).  Union-find on the device:
-//                KUfHook unites every non-shell cell with its +x/+y/+z non-shell
-//                neighbours (lock-free: roots are hooked under smaller roots with a CAS,
-//                so parent[i] <= i always and every walk terminates), KUfCompress stores
-//                each cell's root.  Component membership does not depend on the order.
+//                KUfRuns links each x-run of non-shell cells to its first cell, KUfHook
+//                unites runs that touch across y and z, once per overlap (lock-free:
+//                roots are hooked under smaller roots with a CAS, so parent[i] <= i
+//                always and every walk terminates), KUfCompress stores each cell's root.
+//                Component membership does not depend on the order.
 //   KVoxFinal    cells outside the seed's component become solid; the signed distance is
 //                reconciled with the final classification (Voxelizer.cpp:414-432).
 //
@@ -206,15 +207,37 @@ __device__ __forceinline__ void UfUnion(uint32_t* parent, uint32_t a, uint32_t b
     }
 }
 
+// x-runs first, without atomics: one thread per grid row points every non-shell cell at
+// the first cell of its run of consecutive non-shell cells (a root, and <= the cell).
+__global__ void KUfRuns(const uint8_t* __restrict__ solid, uint32_t* __restrict__ parent, int nx, long long rows) {
+    const long long row = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const long long base = row * nx;
+    uint32_t start = 0;
+    bool prev = false;
+    for (int x = 0; x < nx; ++x) {
+        const bool fluid = solid[base + x] == 0;
+        if (fluid && !prev) start = static_cast<uint32_t>(base + x);
+        if (fluid) parent[base + x] = start;
+        prev = fluid;
+    }
+}
+
+// Unions across y and z.  A cell whose left neighbour and whose partner's left neighbour
+// are both non-shell skips the union: that pair was united one cell to the left, and
+// each cell is in its left neighbour's run.  So only the first cell of each overlap of
+// two runs unites them.
 __global__ void KUfHook(const uint8_t* __restrict__ solid, uint32_t* __restrict__ parent, int nx, int ny, int nz) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long layer = static_cast<long long>(nx) * ny;
     if (i >= layer * nz || solid[i]) return;
     const int x = static_cast<int>(i % nx), y = static_cast<int>((i / nx) % ny), z = static_cast<int>(i / layer);
+    const bool left = x > 0 && solid[i - 1] == 0;
     const uint32_t u = static_cast<uint32_t>(i);
-    if (x + 1 < nx && !solid[i + 1]) UfUnion(parent, u, u + 1);
-    if (y + 1 < ny && !solid[i + nx]) UfUnion(parent, u, static_cast<uint32_t>(i + nx));
-    if (z + 1 < nz && !solid[i + layer]) UfUnion(parent, u, static_cast<uint32_t>(i + layer));
+    if (y + 1 < ny && !solid[i + nx] && !(left && solid[i + nx - 1] == 0))
+        UfUnion(parent, u, static_cast<uint32_t>(i + nx));
+    if (z + 1 < nz && !solid[i + layer] && !(left && solid[i + layer - 1] == 0))
+        UfUnion(parent, u, static_cast<uint32_t>(i + layer));
 }
 
 __global__ void KUfCompress(const uint8_t* __restrict__ solid, uint32_t* __restrict__ parent, long long n) {
@@ -402,6 +425,9 @@ int fc_voxelize(const float* positions, long long nverts, const uint32_t* indice
     if (!active.empty())
         hipLaunchKernelGGL(KVoxRaster, dim3(static_cast<unsigned>(active.size())), dim3(512), 0, 0, va, dtris.p,
                            dstart.p, dlist.p, dactive.p, dsolid.p, dsdf.p);
+    const long long rows = static_cast<long long>(f.ny) * f.nz;
+    hipLaunchKernelGGL(KUfRuns, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, 0, dsolid.p, dparent.p,
+                       f.nx, rows);
     hipLaunchKernelGGL(KUfHook, dim3(cells_blocks), dim3(256), 0, 0, dsolid.p, dparent.p, f.nx, f.ny, f.nz);
     hipLaunchKernelGGL(KUfCompress, dim3(cells_blocks), dim3(256), 0, 0, dsolid.p, dparent.p, N);
     hipLaunchKernelGGL(KVoxFinal, dim3(cells_blocks), dim3(256), 0, 0, dsolid.p, dsdf.p, dparent.p, N, radius);
