@@ -2,12 +2,15 @@
 // own tests/gpu_vs_cpu_test.cpp (:27-80 and :149-172) written against the
 // replacement class, compared with the CPU oracle (oracle/liblbm_oracle.so, test
 // infrastructure) instead of LbmReference.  Exit 0 = pass.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <cstdint>
 #include <vector>
 
 #include "LbmSolver.h"
+#include "VoxelizeDevice.h"
 
 extern "C" {
 void* orc_create(int nx, int ny, int nz, int q, int real_bits, int storage, int py, int pz, int nthreads);
@@ -17,6 +20,34 @@ void orc_set_params(void*, float, float, int, int, int, float);
 void orc_reset(void*, int);
 void orc_step(void*, int);
 void orc_readback_field_f32(void*, float*);
+void orc_aero_forces(int nx, int ny, int nz, float cs, const float* origin, const uint8_t* solid, const float* field,
+                     float* out, int* has_nan);
+int orc_vox_frame(const float* pos, long long nv, long long ni, int target, const float* pad, int* dims, float* frame);
+void orc_voxelize(const float* pos, long long nv, const uint32_t* idx, long long ni, int target, const float* pad,
+                  uint8_t* solid, float* sdf);
+}
+
+// A closed latitude-longitude sphere, outward winding (FlowCraft's STL convention).
+static void SphereMesh(float r, int nlon, int nlat, std::vector<Vec3>& pos, std::vector<uint32_t>& idx) {
+    const float pi = 3.14159265f;
+    pos.push_back(Vec3{0.0f, 0.0f, r});
+    for (int i = 1; i < nlat; ++i)
+        for (int j = 0; j < nlon; ++j) {
+            const float t = pi * i / nlat, p = 2.0f * pi * j / nlon;
+            pos.push_back(Vec3{r * std::sin(t) * std::cos(p), r * std::sin(t) * std::sin(p), r * std::cos(t)});
+        }
+    pos.push_back(Vec3{0.0f, 0.0f, -r});
+    const uint32_t bot = static_cast<uint32_t>(pos.size() - 1);
+    auto v = [&](int i, int j) { return static_cast<uint32_t>(1 + i * nlon + (j % nlon)); };
+    for (int j = 0; j < nlon; ++j) {
+        idx.insert(idx.end(), {0u, v(0, j), v(0, j + 1)});
+        idx.insert(idx.end(), {bot, v(nlat - 2, j + 1), v(nlat - 2, j)});
+    }
+    for (int i = 0; i + 2 < nlat; ++i)
+        for (int j = 0; j < nlon; ++j) {
+            idx.insert(idx.end(), {v(i, j), v(i + 1, j), v(i + 1, j + 1)});
+            idx.insert(idx.end(), {v(i, j), v(i + 1, j + 1), v(i, j + 1)});
+        }
 }
 
 static VoxelGrid SphereGrid() {
@@ -117,6 +148,53 @@ int main() {
                         okWing ? "PASS" : "FAIL");
             if (!okWing) ++failures;
         }
+    }
+    {
+        // FlowCraft's STL -> voxelizer -> solver chain on the replacement: VoxelizeOnDevice
+        // equals the oracle's restated Voxelizer::Voxelize bit for bit, the grid runs, and
+        // the device aerodynamic summary equals the oracle's on the same populations' field.
+        std::vector<Vec3> pos;
+        std::vector<uint32_t> idx;
+        SphereMesh(10.0f, 48, 24, pos, idx);
+        VoxelGrid grid = VoxelizeOnDevice(pos, idx, 64);
+        std::vector<float> flat;
+        for (const Vec3& p : pos) flat.insert(flat.end(), {p.x, p.y, p.z});
+        const float pad[3] = {0.3f, 1.0f, 0.5f};
+        int dims[3];
+        float frame[4];
+        orc_vox_frame(flat.data(), (long long)pos.size(), (long long)idx.size(), 64, pad, dims, frame);
+        std::vector<uint8_t> solid(size_t(dims[0]) * dims[1] * dims[2]);
+        std::vector<float> sdf(solid.size());
+        orc_voxelize(flat.data(), (long long)pos.size(), idx.data(), (long long)idx.size(), 64, pad, solid.data(),
+                     sdf.data());
+        bool okVox = grid.nx == dims[0] && grid.ny == dims[1] && grid.nz == dims[2] && grid.cellSize == frame[0] &&
+                     grid.solid == solid &&
+                     std::equal(sdf.begin(), sdf.end(), grid.signedDistance.begin(),
+                                [](float a, float b) { return std::memcmp(&a, &b, 4) == 0; });
+        LbmSolver s;
+        LbmSolver::Params p; p.tau = 0.6f; p.inletVelocity = 0.05f; p.smagorinskyCs = 0.1f;
+        bool okRun = s.Init(nullptr, grid, LatticeModel::D3Q19);
+        if (okRun) {
+            s.Reset(p, true);
+            s.Step(nullptr, 40);
+            const LbmSolver::AerodynamicForces a = s.ComputeAerodynamicForces(grid);
+            std::vector<float> field = s.ReadbackField();
+            const float org[3] = {grid.origin.x, grid.origin.y, grid.origin.z};
+            float o[11];
+            int nan = 0;
+            orc_aero_forces(grid.nx, grid.ny, grid.nz, grid.cellSize, org, grid.solid.data(), field.data(), o, &nan);
+            auto close = [](float x, float y) { return std::fabs(x - y) <= 1e-6f * std::max(std::fabs(y), 1e-20f); };
+            okRun = !a.hasNaN && !nan && a.forceX > 0.0f && close(a.forceX, o[0]) && close(a.forceY, o[1]) &&
+                    close(a.forceZ, o[2]) && a.maxStress == o[4] && close(a.centerOfPressure.x, o[5]) &&
+                    close(a.centerOfPressure.y, o[6]) && close(a.centerOfPressure.z, o[7]) &&
+                    a.liftArrowBase.z == o[10];
+            std::printf("Mesh sphere: voxelized %dx%dx%d (%s), aero Fx=%.5f cop=(%.3f, %.3f, %.3f) -> %s\n", grid.nx,
+                        grid.ny, grid.nz, okVox ? "bit-exact" : "DIFFERS", a.forceX, a.centerOfPressure.x,
+                        a.centerOfPressure.y, a.centerOfPressure.z, okVox && okRun ? "PASS" : "FAIL");
+        } else {
+            std::printf("Mesh sphere: Init failed: %s -> FAIL\n", s.LastError().c_str());
+        }
+        if (!(okVox && okRun)) ++failures;
     }
     return failures ? 1 : 0;
 }
