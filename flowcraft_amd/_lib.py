@@ -68,6 +68,18 @@ class FcVoxGrid(ctypes.Structure):
 _lib = None
 
 
+def load_hip_runtime() -> None:
+    """Make torch's HIP runtime the process's one.  torch bundles its own
+    libamdhip64.so.7, with the same soname as ROCm's, so whichever is loaded first serves
+    every later library: loading torch before libfc_lbm.so lets torch tensors and this
+    library share the runtime torch was built against (torch initialised on a runtime
+    loaded by another library sees no GPU)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load() -> ctypes.CDLL:
     """Load libfc_lbm.so (raises if it was not built)."""
     global _lib
@@ -76,6 +88,7 @@ def load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                            "(python -c 'import __graft_entry__ as g; g.build()')")
+    load_hip_runtime()
     L = ctypes.CDLL(LIB_PATH)
     h = ctypes.c_void_p
     i = ctypes.c_int

@@ -59,6 +59,7 @@ def nn_lib() -> ctypes.CDLL:
         return _nn
     if not os.path.exists(NN_PATH):
         raise RuntimeError(f"{NN_PATH} is missing: build with __graft_entry__.build()")
+    _lib.load_hip_runtime()
     L = ctypes.CDLL(NN_PATH)
     i, ll, vp, f = ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_float
     L.fc_nn_conv.argtypes = [i, vp, vp, vp, vp, vp, vp] + [i] * 12 + [vp]

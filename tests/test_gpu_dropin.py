@@ -1,5 +1,6 @@
-"""The C++ drop-in (include/flowcraft/LbmSolver.h) driven with the reference test's
-own call sequence (tests/cpp/dropin_test.cpp), on the GPU."""
+"""The C++ drop-ins (include/flowcraft/LbmSolver.h, VoxelizeDevice.h) driven with the
+reference test's own call sequence plus the mesh -> grid -> solver -> aerodynamic-summary
+chain (tests/cpp/dropin_test.cpp), on the GPU."""
 import os
 import subprocess
 
@@ -17,4 +18,4 @@ def test_cpp_dropin_matches_oracle():
     r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
-    assert r.stdout.count("PASS") == 6
+    assert r.stdout.count("PASS") == 7 and "FAIL" not in r.stdout
