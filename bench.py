@@ -26,6 +26,10 @@ readback plus the function, its own build, on the host cores) on the same state.
 on the host cores, on a 241,844-triangle aircraft mesh at resolution 640, and checks the
 grids are identical.
 
+`smoke_trails` times SmokeTrailSystem::Step (8000 particles) on the device through the
+solver's state against FlowCraft's path (field readback plus its own Step on the host)
+and checks that the trails agree.
+
 `warm_start` is the metric's second half, measured live: on --warm-shapes held-out
 shapes of the warm-start distribution (256x128x128, u = 0.06, tau = 0.503, FP16
 storage), steps and wall time until the trailing-mean drag coefficient is within 1%
@@ -170,6 +174,58 @@ def aero_forces_block(torch, grid, cpu=True, reps=20):
     return out
 
 
+def smoke_block(torch, grid, cpu=True, particles=8000, trail=10, steps=20):
+    """SmokeTrailSystem::Step on the 512x256x256 wing (FlowCraft's slider maximum of 8000
+    particles, trails of 10): the device system stepping through the solver's state
+    (field computed on the device, particles advanced, respawns drawn on the host) against
+    FlowCraft's path, ReadbackField() plus its own Step on the host (oracle/_ref; the
+    CPU-baseline leg), and whether the trails agree bit for bit."""
+    from flowcraft_amd.lbm import LatticeModel, LbmSolver, Params, Storage
+    from flowcraft_amd.smoke import SmokeTrailSystem
+    s = LbmSolver()
+    if not s.Init(None, grid, LatticeModel.D3Q19, storage=Storage.F32):
+        raise RuntimeError(s.LastError())
+    s.Reset(Params(tau=TAU, inletVelocity=0.08, smagorinskyCs=0.1), False)
+    s.Step(None, 200)
+    s.Sync()
+    dev = SmokeTrailSystem(grid)
+    dev.Reset(grid, False, particles, trail)
+    dev.Step(grid, s, False, 300.0, 0.03)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.Step(grid, s, False, 300.0, 0.03)
+    dev_ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = {"workload": f"{NX}x{NY}x{NZ} wing after 200 steps, {particles} particles, trails of {trail}",
+           "device_ms": round(dev_ms, 3)}
+    if cpu:
+        import oracle
+        ref = oracle.RefSmoke if oracle.ref_smoke_available() else oracle.OracleSmoke
+        h = ref(grid.solid, grid.cellSize, grid.origin)
+        h.reset(False, particles, trail)
+        chk = SmokeTrailSystem(grid)
+        chk.Reset(grid, False, particles, trail)
+        t_read = t_step = 0.0
+        for _ in range(3):
+            t0 = time.perf_counter()
+            field = s.ReadbackField()
+            t1 = time.perf_counter()
+            h.step(field, False, 300.0, 0.03)
+            t2 = time.perf_counter()
+            chk.Step(grid, s, False, 300.0, 0.03)
+            t_read += t1 - t0
+            t_step += t2 - t1
+        out["readback_ms"] = round(t_read * 1e3 / 3, 1)
+        out["host_step_ms"] = round(t_step * 1e3 / 3, 3)
+        out["host_kind"] = "reference" if ref is oracle.RefSmoke else "port"
+        out["speedup"] = round((out["readback_ms"] + out["host_step_ms"]) / dev_ms, 1)
+        ta, na = chk.Trails()
+        tb, nb = h.trails()
+        out["match"] = bool(np.array_equal(na, nb) and all(
+            np.array_equal(ta[p, :na[p]].view(np.uint32), tb[p, :nb[p]].view(np.uint32)) for p in range(len(na))))
+    s.Shutdown()
+    return out
+
+
 def voxelize_block(cpu=True, reps=3, detail=2, res=640):
     """Voxelizer::Voxelize of a synthetic high-wing aircraft (meshes.aircraft: 241,844
     triangles at detail 2) at FlowCraft's default resolution: the device voxelizer
@@ -266,6 +322,8 @@ def main():
                     help="storage of the --profile run")
     ap.add_argument("--profile-aero", action="store_true",
                     help="only the on-device aerodynamic-force summary, --steps calls (for rocprofv3 runs)")
+    ap.add_argument("--profile-smoke", action="store_true",
+                    help="only the device smoke-trail step on the wing, --steps steps (for rocprofv3 runs)")
     ap.add_argument("--profile-vox", action="store_true",
                     help="only the device voxelizer on the aircraft mesh, --steps calls (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -283,6 +341,10 @@ def main():
     grid = wing_grid()
     print(f"# geometry {time.time() - t0:.1f}s, solid cells {int(grid.solid.sum())}", file=sys.stderr, flush=True)
 
+    if args.profile_smoke:
+        print(json.dumps({"profile": "smoke_trails", **smoke_block(torch, grid, cpu=False, steps=args.steps)}),
+              flush=True)
+        return
     if args.profile_vox:
         print(json.dumps({"profile": "voxelize", **voxelize_block(cpu=False, reps=args.steps)}), flush=True)
         return
@@ -328,6 +390,7 @@ def main():
                 "kernel": f"fc::KStep<19, StoreF32, {layout.name.title().replace('place', 'Place')}>"}
     aero = None if args.no_variants else aero_forces_block(torch, grid, cpu=not args.no_cpu)
     vox = None if args.no_variants else voxelize_block(cpu=not args.no_cpu)
+    smoke = None if args.no_variants else smoke_block(torch, grid, cpu=not args.no_cpu)
     warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(grid)
     line = {
@@ -339,7 +402,7 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "warm_start": warm,
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "warm_start": warm,
         "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)

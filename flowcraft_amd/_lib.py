@@ -27,6 +27,8 @@ EXPORTS = (
     "fc_lbm_get_populations", "fc_lbm_set_populations", "fc_lbm_sync", "fc_lbm_info", "fc_lbm_step_count",
     "fc_lbm_population_bytes", "fc_lbm_stream", "fc_lbm_boundary_links", "fc_lbm_surface_cells",
     "fc_lbm_aero_forces", "fc_vox_params_default", "fc_voxelize_frame", "fc_voxelize", "fc_voxelize_last_error",
+    "fc_smoke_create", "fc_smoke_destroy", "fc_smoke_last_error", "fc_smoke_reset", "fc_smoke_scatter",
+    "fc_smoke_shift_plane", "fc_smoke_step", "fc_smoke_step_field", "fc_smoke_trails", "fc_smoke_info",
 )
 
 
@@ -134,6 +136,18 @@ def load() -> ctypes.CDLL:
     L.fc_voxelize.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), vp, vp]
     L.fc_voxelize_last_error.argtypes = []
     L.fc_voxelize_last_error.restype = ctypes.c_char_p
+    fl = ctypes.c_float
+    L.fc_smoke_create.argtypes = [i, i, i, fl, ctypes.POINTER(fl), vp, ctypes.POINTER(h)]
+    L.fc_smoke_destroy.argtypes = [h]
+    L.fc_smoke_last_error.argtypes = [h]
+    L.fc_smoke_last_error.restype = ctypes.c_char_p
+    L.fc_smoke_reset.argtypes = [h, i, i, i, i, i]
+    L.fc_smoke_scatter.argtypes = [h, i, i]
+    L.fc_smoke_shift_plane.argtypes = [h, i, i]
+    L.fc_smoke_step.argtypes = [h, h, i, fl, fl, i, i, i]
+    L.fc_smoke_step_field.argtypes = [h, vp, i, fl, fl, i, i, i]
+    L.fc_smoke_trails.argtypes = [h, vp, vp]
+    L.fc_smoke_info.argtypes = [h, ctypes.POINTER(i), ctypes.POINTER(i)]
     L.fc_lbm_surface_cells.argtypes = [h]
     L.fc_lbm_surface_cells.restype = ctypes.c_longlong
     _lib = L

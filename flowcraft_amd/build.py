@@ -32,7 +32,7 @@ COMMON = COMPILE + ["-shared"]
 _STEP_UNITS = [("step_inst.hip", [f"-DFC_Q={q}", f"-DFC_STORE={st}"])
                for q in (19, 15) for st in ("StoreF32", "StoreF16", "StoreBF16", "StoreF64")]
 TARGETS = {
-    "libfc_lbm.so": [("fc_lbm.hip", []), ("voxelize.hip", [])] + _STEP_UNITS,
+    "libfc_lbm.so": [("fc_lbm.hip", []), ("voxelize.hip", []), ("smoke.hip", [])] + _STEP_UNITS,
     "libfc_nn.so": [("nn_kernels.hip", [])],
 }
 EXECUTABLES = {

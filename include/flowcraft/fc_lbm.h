@@ -222,6 +222,47 @@ int fc_voxelize(const float* positions, long long nverts, const uint32_t* indice
 /* Message of the calling thread's last voxelizer error. */
 const char* fc_voxelize_last_error(void);
 
+/* ---------------------------------------------------------------- smoke trails
+ * FlowCraft's SmokeTrailSystem (FlowCraft/SmokeTrail.h:10-184) with its particles and
+ * trails on the device, advected through the solver's field without reading it back.
+ * Grid conventions as VoxelGrid (world X = grid y, world Y = grid x).  confine_axis:
+ * -1 none, 0/1/2 = the grid-x/y/z plane at confine_index (SmokeTrailSystem's
+ * PlaneConfinement).  Random picks use the reference's generator and distributions
+ * (std::mt19937 seeded 12345), in the reference's order. */
+typedef struct fc_smoke fc_smoke;
+
+/* A particle system over a grid: dimensions, cell size, origin and solid mask (host,
+ * nx*ny*nz bytes, copied). */
+int fc_smoke_create(int nx, int ny, int nz, float cell_size, const float origin[3], const uint8_t* solid,
+                    fc_smoke** out);
+int fc_smoke_destroy(fc_smoke* s);
+const char* fc_smoke_last_error(const fc_smoke* s);
+
+/* Replaces SmokeTrailSystem::Reset (FlowCraft/SmokeTrail.cpp:257-275): particle_count
+ * particles scattered over the grid (or the plane), trails of at most
+ * max(trail_length, 2) points. */
+int fc_smoke_reset(fc_smoke* s, int reverse_flow, int particle_count, int trail_length, int confine_axis,
+                   int confine_index);
+
+/* Replaces SmokeTrailSystem::ScatterRandom (FlowCraft/SmokeTrail.cpp:202-216). */
+int fc_smoke_scatter(fc_smoke* s, int confine_axis, int confine_index);
+
+/* Replaces SmokeTrailSystem::ShiftPlane (FlowCraft/SmokeTrail.cpp:218-255). */
+int fc_smoke_shift_plane(fc_smoke* s, int axis, int index);
+
+/* Replaces SmokeTrailSystem::Step (FlowCraft/SmokeTrail.cpp:277-345) with the field of
+ * the solver's current state (same lattice), computed on the device.  Blocking. */
+int fc_smoke_step(fc_smoke* s, fc_lbm* h, int reverse_flow, float mm_per_sec_per_lattice_unit, float dt_seconds,
+                  int suspend_respawn, int confine_axis, int confine_index);
+
+/* Same with a caller's field: device pointer, 4 floats per cell (fc_lbm_field_device). */
+int fc_smoke_step_field(fc_smoke* s, const float* field_device, int reverse_flow, float mm_per_sec_per_lattice_unit,
+                        float dt_seconds, int suspend_respawn, int confine_axis, int confine_index);
+
+/* The trails, host: out[(p*trail_length + k)*3 + c] for k < lengths[p], oldest first. */
+int fc_smoke_trails(fc_smoke* s, float* out, int* lengths);
+int fc_smoke_info(const fc_smoke* s, int* particles, int* trail_length);
+
 /* Waits for all work queued on the handle's stream. */
 int fc_lbm_sync(fc_lbm* h);
 

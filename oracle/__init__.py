@@ -243,6 +243,90 @@ def ref_voxelize(positions, indices, target=640, padding=VOX_DEFAULT_PADDING):
     return solid, sdf, float(fr[0]), tuple(float(v) for v in fr[1:])
 
 
+REF_SMOKE_SO = os.path.join(_HERE, "_ref", "libref_smoke.so")
+
+
+def ref_smoke_available() -> bool:
+    return os.path.exists(REF_SMOKE_SO)
+
+
+class _SmokeBase:
+    """Common front end of the restated (orc_smoke_*) and FlowCraft's own (ref_smoke_*)
+    SmokeTrailSystem.  confine: None or (axis 0/1/2 = grid x/y/z, index)."""
+    prefix = "orc_smoke_"
+
+    def __init__(self, L, solid, cell_size=1.0, origin=(0.0, 0.0, 0.0)):
+        self.L = L
+        for name, argt in (("create", [ctypes.c_int] * 3 + [ctypes.c_float, _c_float_p, _c_u8_p]),
+                           ("destroy", [ctypes.c_void_p]),
+                           ("reset", [ctypes.c_void_p] + [ctypes.c_int] * 5),
+                           ("scatter", [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+                           ("shift_plane", [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+                           ("step", [ctypes.c_void_p, _c_float_p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+                           ("trails", [ctypes.c_void_p, _c_float_p, ctypes.POINTER(ctypes.c_int)])):
+            getattr(L, self.prefix + name).argtypes = argt
+        L_ = getattr(L, self.prefix + "create")
+        L_.restype = ctypes.c_void_p
+        self._solid = np.ascontiguousarray(solid, dtype=np.uint8)
+        nz, ny, nx = self._solid.shape
+        o = np.ascontiguousarray(origin, dtype=np.float32)
+        self.h = L_(nx, ny, nz, float(cell_size), _ptr(o, _c_float_p), _ptr(self._solid, _c_u8_p))
+        self.P, self.T = 0, 2
+
+    def _f(self, name):
+        return getattr(self.L, self.prefix + name)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._f("destroy")(self.h)
+            self.h = None
+
+    @staticmethod
+    def _c(confine):
+        return (-1, 0) if confine is None else (int(confine[0]), int(confine[1]))
+
+    def reset(self, reverse, count, trail, confine=None):
+        self.P, self.T = max(count, 0), max(trail, 2)
+        self._f("reset")(self.h, int(reverse), int(count), int(trail), *self._c(confine))
+
+    def scatter(self, confine=None):
+        self._f("scatter")(self.h, *self._c(confine))
+
+    def shift_plane(self, confine):
+        self._f("shift_plane")(self.h, *self._c(confine))
+
+    def step(self, field, reverse, mmps, dt, suspend=False, confine=None):
+        f = np.ascontiguousarray(field, dtype=np.float32)
+        self._f("step")(self.h, _ptr(f, _c_float_p), int(reverse), float(mmps), float(dt), int(suspend),
+                        *self._c(confine))
+
+    def trails(self):
+        """(points float32 [P, T, 3] with unused slots zero, lengths int32 [P])."""
+        out = np.zeros((self.P, self.T, 3), np.float32)
+        n = np.zeros(self.P, np.int32)
+        if self.P:
+            self._f("trails")(self.h, _ptr(out, _c_float_p), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return out, n
+
+
+class OracleSmoke(_SmokeBase):
+    """The restated SmokeTrailSystem (oracle/smoke_oracle.cpp)."""
+
+    def __init__(self, solid, cell_size=1.0, origin=(0.0, 0.0, 0.0)):
+        super().__init__(lib(), solid, cell_size, origin)
+
+
+class RefSmoke(_SmokeBase):
+    """FlowCraft's own SmokeTrailSystem, compiled into oracle/_ref."""
+    prefix = "ref_smoke_"
+
+    def __init__(self, solid, cell_size=1.0, origin=(0.0, 0.0, 0.0)):
+        if not ref_smoke_available():
+            raise RuntimeError("oracle/_ref/libref_smoke.so is not built (make -C oracle ref)")
+        super().__init__(ctypes.CDLL(REF_SMOKE_SO), solid, cell_size, origin)
+
+
 class OracleLbm:
     """The restated LbmReference.  bits=32 (parity twin) or 64 (FP64 oracle)."""
 

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     text = open(os.path.join(ROOT, "include", "flowcraft", "fc_lbm.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(fc_(?:lbm|vox|voxelize)_?[a-z_0-9]*)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(fc_(?:lbm|vox|voxelize|smoke)_?[a-z_0-9]*)\s*\(", text)))
 
 
 def test_header_and_binding_agree():
