@@ -324,7 +324,20 @@ class RefSmoke(_SmokeBase):
     def __init__(self, solid, cell_size=1.0, origin=(0.0, 0.0, 0.0)):
         if not ref_smoke_available():
             raise RuntimeError("oracle/_ref/libref_smoke.so is not built (make -C oracle ref)")
-        super().__init__(ctypes.CDLL(REF_SMOKE_SO), solid, cell_size, origin)
+        L = ctypes.CDLL(REF_SMOKE_SO)
+        L.ref_smoke_load_field.argtypes = [ctypes.c_void_p, _c_float_p]
+        L.ref_smoke_step_loaded.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        super().__init__(L, solid, cell_size, origin)
+
+    def load_field(self, field):
+        """The field into the std::vector the reference steps from (FlowCraft's
+        ReadbackField result)."""
+        f = np.ascontiguousarray(field, dtype=np.float32)
+        self.L.ref_smoke_load_field(self.h, _ptr(f, _c_float_p))
+
+    def step_loaded(self, rev, mmps, dt, suspend=False, confine=None):
+        self.L.ref_smoke_step_loaded(self.h, int(rev), float(mmps), float(dt), int(suspend), *self._c(confine))
 
 
 class OracleLbm:

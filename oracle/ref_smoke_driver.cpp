@@ -27,6 +27,7 @@ namespace {
 struct RefSmoke {
     VoxelGrid grid;
     SmokeTrailSystem sys;
+    std::vector<float> field;  // the ReadbackField() vector the reference steps from
     int L = 10;
 };
 SmokeTrailSystem::PlaneConfinement Plane(int axis, int index) {
@@ -63,11 +64,20 @@ void ref_smoke_shift_plane(void* h, int axis, int index) {
     RefSmoke* r = static_cast<RefSmoke*>(h);
     r->sys.ShiftPlane(r->grid, Plane(axis, index));
 }
-void ref_smoke_step(void* h, const float* field, int reverse, float mmps, float dt, int suspend, int axis, int index) {
+// The field as FlowCraft holds it (a std::vector filled by ReadbackField), then the step
+// on it: split so that a timing of the step excludes this copy.
+void ref_smoke_load_field(void* h, const float* field) {
     RefSmoke* r = static_cast<RefSmoke*>(h);
-    std::vector<float> f(field, field + r->grid.CellCount() * 4);
+    r->field.assign(field, field + r->grid.CellCount() * 4);
+}
+void ref_smoke_step_loaded(void* h, int reverse, float mmps, float dt, int suspend, int axis, int index) {
+    RefSmoke* r = static_cast<RefSmoke*>(h);
     const SmokeTrailSystem::PlaneConfinement c = Plane(axis, index);
-    r->sys.Step(r->grid, f, reverse != 0, mmps, dt, suspend != 0, axis >= 0 ? &c : nullptr);
+    r->sys.Step(r->grid, r->field, reverse != 0, mmps, dt, suspend != 0, axis >= 0 ? &c : nullptr);
+}
+void ref_smoke_step(void* h, const float* field, int reverse, float mmps, float dt, int suspend, int axis, int index) {
+    ref_smoke_load_field(h, field);
+    ref_smoke_step_loaded(h, reverse, mmps, dt, suspend, axis, index);
 }
 void ref_smoke_trails(void* h, float* out, int* lengths) {
     RefSmoke* r = static_cast<RefSmoke*>(h);
