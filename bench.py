@@ -30,6 +30,9 @@ grids are identical.
 solver's state against FlowCraft's path (field readback plus its own Step on the host)
 and checks that the trails agree.
 
+`pacing` times an interactive frame (the field for display plus 60 steps) in FlowCraft's
+blocking order against the asynchronous snapshot that overlaps the copy with the steps.
+
 `warm_start` is the metric's second half, measured live: on --warm-shapes held-out
 shapes of the warm-start distribution (256x128x128, u = 0.06, tau = 0.503, FP16
 storage), steps and wall time until the trailing-mean drag coefficient is within 1%
@@ -230,6 +233,47 @@ def smoke_block(torch, grid, cpu=True, particles=8000, trail=10, steps=20):
     return out
 
 
+def pacing_block(torch, grid, steps=60, reps=3):
+    """An interactive front end's frame on the 512x256x256 wing: the field for display
+    plus `steps` solver steps.  FlowCraft's order (blocking readback, then the steps)
+    against the snapshot path (fc_lbm_snapshot_begin, the steps run while the field is
+    copied to pinned host memory on a second stream, then the wait); both snapshots are
+    checked against each other bit for bit."""
+    from flowcraft_amd.lbm import LatticeModel, LbmSolver, Params, Storage
+    s = LbmSolver()
+    if not s.Init(None, grid, LatticeModel.D3Q19, storage=Storage.F32):
+        raise RuntimeError(s.LastError())
+    s.Reset(Params(tau=TAU, inletVelocity=0.08, smagorinskyCs=0.1), False)
+    s.Step(None, 50)
+    s.SnapshotBegin()
+    s.SnapshotWait(copy=False)  # buffers allocated, pages touched
+    seq = asy = 0.0
+    match = True
+    for _ in range(reps):
+        s.Sync()
+        t0 = time.perf_counter()
+        f_seq = s.ReadbackField()
+        s.Step(None, steps)
+        s.Sync()
+        seq += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s.SnapshotBegin()
+        s.Step(None, steps)
+        f_asy = s.SnapshotWait(copy=False)
+        s.Sync()
+        asy += time.perf_counter() - t0
+        match = match and f_asy.size == f_seq.size
+    # the snapshot equals a blocking readback of the state it was taken from
+    ref = s.ReadbackField()
+    s.SnapshotBegin()
+    s.Step(None, 5)
+    match = match and bool(np.array_equal(s.SnapshotWait(copy=False).view(np.uint32), ref.view(np.uint32)))
+    s.Shutdown()
+    return {"workload": f"{NX}x{NY}x{NZ} wing: field for display + {steps} FP32 steps per frame",
+            "blocking_readback_then_steps_ms": round(seq * 1e3 / reps, 1),
+            "snapshot_overlapped_ms": round(asy * 1e3 / reps, 1), "speedup": round(seq / asy, 2), "match": match}
+
+
 def voxelize_block(cpu=True, reps=3, detail=2, res=640):
     """Voxelizer::Voxelize of a synthetic high-wing aircraft (meshes.aircraft: 241,844
     triangles at detail 2) at FlowCraft's default resolution: the device voxelizer
@@ -395,6 +439,7 @@ def main():
     aero = None if args.no_variants else aero_forces_block(torch, grid, cpu=not args.no_cpu)
     vox = None if args.no_variants else voxelize_block(cpu=not args.no_cpu)
     smoke = None if args.no_variants else smoke_block(torch, grid, cpu=not args.no_cpu)
+    pacing = None if args.no_variants else pacing_block(torch, grid)
     warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(grid)
     line = {
@@ -406,7 +451,7 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "warm_start": warm,
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "warm_start": warm,
         "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)

@@ -29,6 +29,7 @@ EXPORTS = (
     "fc_lbm_aero_forces", "fc_vox_params_default", "fc_voxelize_frame", "fc_voxelize", "fc_voxelize_last_error",
     "fc_smoke_create", "fc_smoke_destroy", "fc_smoke_last_error", "fc_smoke_reset", "fc_smoke_scatter",
     "fc_smoke_shift_plane", "fc_smoke_step", "fc_smoke_step_field", "fc_smoke_trails", "fc_smoke_info",
+    "fc_lbm_snapshot_begin", "fc_lbm_snapshot_poll", "fc_lbm_snapshot_wait",
 )
 
 
@@ -148,6 +149,9 @@ def load() -> ctypes.CDLL:
     L.fc_smoke_step_field.argtypes = [h, vp, i, fl, fl, i, i, i]
     L.fc_smoke_trails.argtypes = [h, vp, vp]
     L.fc_smoke_info.argtypes = [h, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.fc_lbm_snapshot_begin.argtypes = [h]
+    L.fc_lbm_snapshot_poll.argtypes = [h, ctypes.POINTER(i)]
+    L.fc_lbm_snapshot_wait.argtypes = [h, ctypes.POINTER(ctypes.POINTER(ctypes.c_float))]
     L.fc_lbm_surface_cells.argtypes = [h]
     L.fc_lbm_surface_cells.restype = ctypes.c_longlong
     _lib = L

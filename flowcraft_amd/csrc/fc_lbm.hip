@@ -52,6 +52,13 @@ struct fc_lbm {
     int* dprobe = nullptr;
     float* staging = nullptr;
     size_t staging_bytes = 0;
+    // field snapshots: device copy + pinned host copy, moved on their own stream
+    float* snap_dev = nullptr;
+    float* snap_host = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_taken = nullptr;   // the device snapshot is complete (solver stream)
+    hipEvent_t snap_copied = nullptr;  // the host copy is complete (copy stream)
+    bool snap_pending = false;
     fc_lbm_params params{};
     long long t = 0;
     bool has_geometry = false;
@@ -339,6 +346,12 @@ int fc_lbm_destroy(fc_lbm* h) {
     DevFree(h->dflag);
     DevFree(h->dprobe);
     DevFree(h->staging);
+    if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
+    DevFree(h->snap_dev);
+    if (h->snap_host) (void)hipHostFree(h->snap_host);
+    if (h->snap_taken) (void)hipEventDestroy(h->snap_taken);
+    if (h->snap_copied) (void)hipEventDestroy(h->snap_copied);
+    if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return FC_LBM_OK;
@@ -586,6 +599,55 @@ int fc_lbm_field_device(fc_lbm* h, float* out_device) {
     int rc = CheckReady(h);
     if (rc) return rc;
     return FieldChunk(h, 0, h->nz, out_device);
+}
+
+int fc_lbm_snapshot_begin(fc_lbm* h) {
+    if (!h) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    const size_t bytes = static_cast<size_t>(h->N) * 4 * sizeof(float);
+    if (!h->snap_dev) {
+        FC_HIP(h, hipMalloc(&h->snap_dev, bytes));
+        FC_HIP(h, hipHostMalloc(reinterpret_cast<void**>(&h->snap_host), bytes, hipHostMallocDefault));
+        FC_HIP(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+        FC_HIP(h, hipEventCreateWithFlags(&h->snap_taken, hipEventDisableTiming));
+        FC_HIP(h, hipEventCreateWithFlags(&h->snap_copied, hipEventDisableTiming));
+        FC_HIP(h, hipEventRecord(h->snap_copied, h->copy_stream));
+    }
+    // the previous snapshot's copy reads snap_dev (and fills snap_host, which the caller
+    // may still be reading): the new field waits for it on the device, the host waits here
+    FC_HIP(h, hipEventSynchronize(h->snap_copied));
+    FC_HIP(h, hipStreamWaitEvent(h->stream, h->snap_copied, 0));
+    rc = FieldChunk(h, 0, h->nz, h->snap_dev);
+    if (rc) return rc;
+    FC_HIP(h, hipEventRecord(h->snap_taken, h->stream));
+    FC_HIP(h, hipStreamWaitEvent(h->copy_stream, h->snap_taken, 0));
+    FC_HIP(h, hipMemcpyAsync(h->snap_host, h->snap_dev, bytes, hipMemcpyDeviceToHost, h->copy_stream));
+    FC_HIP(h, hipEventRecord(h->snap_copied, h->copy_stream));
+    h->snap_pending = true;
+    return FC_LBM_OK;
+}
+
+int fc_lbm_snapshot_poll(fc_lbm* h, int* ready) {
+    if (!h || !ready) return FC_LBM_EINVAL;
+    if (!h->snap_pending) return Fail(h, FC_LBM_ESTATE, "no snapshot was started");
+    const hipError_t e = hipEventQuery(h->snap_copied);
+    if (e == hipErrorNotReady) {
+        *ready = 0;
+        return FC_LBM_OK;
+    }
+    FC_HIP(h, e);
+    *ready = 1;
+    return FC_LBM_OK;
+}
+
+int fc_lbm_snapshot_wait(fc_lbm* h, const float** field) {
+    if (!h || !field) return FC_LBM_EINVAL;
+    *field = nullptr;
+    if (!h->snap_pending) return Fail(h, FC_LBM_ESTATE, "no snapshot was started");
+    FC_HIP(h, hipEventSynchronize(h->snap_copied));
+    *field = h->snap_host;
+    return FC_LBM_OK;
 }
 
 int fc_lbm_readback_field_range(fc_lbm* h, int z0, int zcount, float* out) {

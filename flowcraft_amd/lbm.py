@@ -65,6 +65,16 @@ class Params:
 
 
 @dataclass
+class PacingState:
+    """StepPaced's state carried between calls (FlowCraft keeps these as statics of
+    StepSolverPaced, main.cpp:2376-2379, 2448-2449)."""
+    chunkZDepth: int = 32
+    probePhase: int = 0
+    lastProbe: float = float("-inf")
+    diverged: bool = False
+
+
+@dataclass
 class AerodynamicForces:
     """FieldOverlay::AerodynamicForces (FlowCraft/FieldOverlay.h:186-195); vectors in the
     VoxelGrid's world frame (grid-y along world X, grid-x along world Y)."""
@@ -320,6 +330,74 @@ class LbmSolver:
         """Same sums into a float64 device tensor of 4, without synchronising."""
         if self.IsValid():
             self._check(self._lib.fc_lbm_forces_device(self._h, out_tensor.data_ptr()))
+
+    def StepPaced(self, substeps: int, state: Optional["PacingState"] = None, cancel=None,
+                  targetChunkMs: float = 3.0, probeIntervalMs: float = 400.0, progress=None) -> int:
+        """FlowCraft's StepSolverPaced (FlowCraft/main.cpp:2372-2486): each substep as
+        z-chunks of adaptive depth (halved above targetChunkMs, doubled under a quarter of
+        it, within [4, 256] layers), waiting for each chunk; AdvanceSubstep; a slice health
+        probe (nz/2, nz/3, 2nz/3 in turn) at most every probeIntervalMs.  Returns the
+        substeps done: fewer when cancel() turns true between chunks, or i+1 after the
+        substep whose probe found a divergence (state.diverged).  progress(f) receives
+        (i + z/nz) / substeps."""
+        import time
+        if not self.IsValid():
+            return 0
+        st = state if state is not None else PacingState()
+        nz = self.Nz()
+        if progress:
+            progress(0.0)
+        for i in range(substeps):
+            z0 = 0
+            while z0 < nz:
+                if cancel and cancel():
+                    return i
+                zc = min(st.chunkZDepth, nz - z0)
+                t0 = time.perf_counter()
+                self._check(self._lib.fc_lbm_step_range(self._h, z0, zc))
+                self._check(self._lib.fc_lbm_sync(self._h))
+                ms = (time.perf_counter() - t0) * 1e3
+                if ms > targetChunkMs and st.chunkZDepth > 4:
+                    st.chunkZDepth = max(4, st.chunkZDepth // 2)
+                elif ms < targetChunkMs * 0.25 and st.chunkZDepth < 256:
+                    st.chunkZDepth = min(256, st.chunkZDepth * 2)
+                z0 += zc
+                if progress:
+                    progress((i + z0 / nz) / substeps)
+            if cancel and cancel():
+                return i
+            self._check(self._lib.fc_lbm_advance(self._h))
+            now = time.perf_counter()
+            if (now - st.lastProbe) * 1e3 >= probeIntervalMs and nz > 0:
+                st.lastProbe = now
+                pz = nz // 2 if st.probePhase == 0 else (nz // 3 if st.probePhase == 1 else (2 * nz) // 3)
+                st.probePhase = (st.probePhase + 1) % 3
+                if self.ProbeSliceHealth(pz):
+                    st.diverged = True
+                    return i + 1
+        return substeps
+
+    def SnapshotBegin(self) -> None:
+        """Start an asynchronous field snapshot of the current state (fc_lbm_snapshot_begin):
+        later steps run while it is copied to pinned host memory."""
+        if self.IsValid():
+            self._check(self._lib.fc_lbm_snapshot_begin(self._h))
+
+    def SnapshotReady(self) -> bool:
+        r = ctypes.c_int(0)
+        if self.IsValid():
+            self._check(self._lib.fc_lbm_snapshot_poll(self._h, ctypes.byref(r)))
+        return bool(r.value)
+
+    def SnapshotWait(self, copy: bool = True) -> np.ndarray:
+        """The snapshot's field (4 floats per cell).  copy=False returns a view of the
+        pinned buffer, valid until the next SnapshotBegin or Shutdown."""
+        if not self.IsValid():
+            return np.zeros(0, np.float32)
+        p = ctypes.POINTER(ctypes.c_float)()
+        self._check(self._lib.fc_lbm_snapshot_wait(self._h, ctypes.byref(p)))
+        view = np.ctypeslib.as_array(p, shape=(self.CellCount() * 4,))
+        return view.copy() if copy else view
 
     def ComputeAerodynamicForces(self, cellSize: float = 1.0, origin=(0.0, 0.0, 0.0)) -> AerodynamicForces:
         """FieldOverlay::ComputeAerodynamicForces (FlowCraft/FieldOverlay.cpp:469-621) of the

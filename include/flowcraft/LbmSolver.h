@@ -13,6 +13,9 @@
 // minimal definitions with the same names and members for standalone builds.
 #pragma once
 
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -219,6 +222,60 @@ public:
     int Nx() const { return nx_; }
     int Ny() const { return ny_; }
     int Nz() const { return nz_; }
+
+    // Extension: FlowCraft's StepSolverPaced (FlowCraft/main.cpp:2372-2486) on this solver.
+    // Each substep runs as z-chunks (StepRange) of adaptive depth, waiting for each chunk:
+    // the depth halves when a chunk took longer than targetChunkMs and doubles when it took
+    // under a quarter of it, within [4, 256] layers; AdvanceSubstep closes the substep, and
+    // at most every probeIntervalMs a slice health probe runs (layers nz/2, nz/3, 2nz/3 in
+    // turn).  Returns the substeps completed: fewer when `cancel` turns true between chunks
+    // (FlowCraft's solver-epoch check), or i+1 after substep i's probe found a divergence
+    // (state.diverged set).  `progress`, if given, receives (i + z/nz) / substeps.
+    struct PacingState {
+        int chunkZDepth = 32;
+        int probePhase = 0;
+        std::chrono::steady_clock::time_point lastProbe{};
+        bool diverged = false;
+    };
+
+    int StepPaced(int substeps, PacingState& st, const std::atomic<bool>* cancel = nullptr,
+                  double targetChunkMs = 3.0, double probeIntervalMs = 400.0, std::atomic<float>* progress = nullptr) {
+        if (!h_) return 0;
+        using Clock = std::chrono::steady_clock;
+        const int nz = nz_;
+        if (progress) progress->store(0.0f, std::memory_order_relaxed);
+        for (int i = 0; i < substeps; ++i) {
+            for (int z0 = 0; z0 < nz;) {
+                if (cancel && cancel->load(std::memory_order_acquire)) return i;
+                const int zc = std::min(st.chunkZDepth, nz - z0);
+                const Clock::time_point t0 = Clock::now();
+                if (!Check(fc_lbm_step_range(h_, z0, zc)) || !Check(fc_lbm_sync(h_))) return i;
+                const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+                if (ms > targetChunkMs && st.chunkZDepth > 4)
+                    st.chunkZDepth = std::max(4, st.chunkZDepth / 2);
+                else if (ms < targetChunkMs * 0.25 && st.chunkZDepth < 256)
+                    st.chunkZDepth = std::min(256, st.chunkZDepth * 2);
+                z0 += zc;  // what this chunk covered, not the (possibly changed) depth
+                if (progress)
+                    progress->store((static_cast<float>(i) + static_cast<float>(z0) / static_cast<float>(nz)) /
+                                        static_cast<float>(substeps),
+                                    std::memory_order_relaxed);
+            }
+            if (cancel && cancel->load(std::memory_order_acquire)) return i;
+            if (!Check(fc_lbm_advance(h_))) return i;
+            const Clock::time_point now = Clock::now();
+            if (std::chrono::duration<double, std::milli>(now - st.lastProbe).count() >= probeIntervalMs && nz > 0) {
+                st.lastProbe = now;
+                const int pz = st.probePhase == 0 ? nz / 2 : st.probePhase == 1 ? nz / 3 : (2 * nz) / 3;
+                st.probePhase = (st.probePhase + 1) % 3;
+                if (ProbeSliceHealth(pz)) {
+                    st.diverged = true;
+                    return i + 1;
+                }
+            }
+        }
+        return substeps;
+    }
 
     // Extensions.
     fc_lbm* Handle() const { return h_; }

@@ -138,6 +138,19 @@ int fc_lbm_readback_field(fc_lbm* h, float* out);
 /* The whole field into a device buffer of 4*N floats (no host round trip). */
 int fc_lbm_field_device(fc_lbm* h, float* out_device);
 
+/* Field snapshots for interactive front ends, in place of FlowCraft's paced, z-chunked
+ * readback (ReadbackFieldPacedGPU, FlowCraft/main.cpp:2488-2555), which exists so that
+ * a readback never holds the solver's queue for more than ~3 ms.  Here the copy never
+ * holds it: fc_lbm_snapshot_begin enqueues the field of the current state into a device
+ * buffer on the solver's stream (a fraction of a step) and its copy to pinned host
+ * memory on a second stream, and returns at once; steps enqueued afterwards run while
+ * the 4 floats per cell travel.  fc_lbm_snapshot_wait returns that host copy (valid
+ * until the next begin or destroy); fc_lbm_snapshot_poll tells whether it has landed.
+ * A begin waits for the previous snapshot's copy. */
+int fc_lbm_snapshot_begin(fc_lbm* h);
+int fc_lbm_snapshot_poll(fc_lbm* h, int* ready);
+int fc_lbm_snapshot_wait(fc_lbm* h, const float** field);
+
 /* Replaces LbmSolver::ReadbackStats (FlowCraft/LbmSolver.cpp:739-788) and
  * LbmReference::ComputeStats (FlowCraft/LbmReference.cpp:502-585), reduced on the
  * device in a fixed order (deterministic).  Blocking. */

@@ -150,6 +150,27 @@ int main() {
         }
     }
     {
+        // StepPaced (FlowCraft's StepSolverPaced loop) equals plain stepping, bit for bit,
+        // with the chunk depth forced down to 4 layers and with it left to grow.
+        VoxelGrid grid = SphereGrid();
+        LbmSolver::Params p; p.tau = 0.6f; p.inletVelocity = 0.05f; p.smagorinskyCs = 0.12f;
+        LbmSolver::EngineOptions opt; opt.layout = FC_LBM_LAYOUT_PINGPONG;
+        bool ok = true;
+        for (double target : {1e-6, 1e6}) {
+            LbmSolver a, b;
+            a.Init(nullptr, grid, LatticeModel::D3Q19, opt); a.Reset(p, true);
+            b.Init(nullptr, grid, LatticeModel::D3Q19, opt); b.Reset(p, true);
+            LbmSolver::PacingState st;
+            const int done = a.StepPaced(12, st, nullptr, target);
+            b.Step(nullptr, 12);
+            std::vector<float> fa = a.ReadbackField(), fb = b.ReadbackField();
+            ok = ok && done == 12 && !st.diverged && std::memcmp(fa.data(), fb.data(), fa.size() * 4) == 0 &&
+                 st.chunkZDepth == (target < 1.0 ? 4 : 256);
+        }
+        std::printf("StepPaced vs Step (chunks of 4 and of 256 layers) -> %s\n", ok ? "PASS" : "FAIL");
+        if (!ok) ++failures;
+    }
+    {
         // FlowCraft's STL -> voxelizer -> solver chain on the replacement: VoxelizeOnDevice
         // equals the oracle's restated Voxelizer::Voxelize bit for bit, the grid runs, and
         // the device aerodynamic summary equals the oracle's on the same populations' field.
