@@ -606,11 +606,12 @@ int fc_lbm_snapshot_begin(fc_lbm* h) {
     int rc = CheckReady(h);
     if (rc) return rc;
     const size_t bytes = static_cast<size_t>(h->N) * 4 * sizeof(float);
-    if (!h->snap_dev) {
-        FC_HIP(h, hipMalloc(&h->snap_dev, bytes));
-        FC_HIP(h, hipHostMalloc(reinterpret_cast<void**>(&h->snap_host), bytes, hipHostMallocDefault));
-        FC_HIP(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
-        FC_HIP(h, hipEventCreateWithFlags(&h->snap_taken, hipEventDisableTiming));
+    // first use: each resource on its own, so a call after a failed set-up completes it
+    if (!h->snap_dev) FC_HIP(h, hipMalloc(&h->snap_dev, bytes));
+    if (!h->snap_host) FC_HIP(h, hipHostMalloc(reinterpret_cast<void**>(&h->snap_host), bytes, hipHostMallocDefault));
+    if (!h->copy_stream) FC_HIP(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+    if (!h->snap_taken) FC_HIP(h, hipEventCreateWithFlags(&h->snap_taken, hipEventDisableTiming));
+    if (!h->snap_copied) {
         FC_HIP(h, hipEventCreateWithFlags(&h->snap_copied, hipEventDisableTiming));
         FC_HIP(h, hipEventRecord(h->snap_copied, h->copy_stream));
     }

@@ -259,7 +259,7 @@ int fc_smoke_create(int nx, int ny, int nz, float cell_size, const float origin[
     *out = nullptr;
     if (nx <= 0 || ny <= 0 || nz <= 0 || !origin || !solid || !(cell_size > 0.0f)) return FC_LBM_EINVAL;
     fc_smoke* s = new fc_smoke();
-    *out = s;
+    *out = s;  // returned even on failure so fc_smoke_last_error can explain; destroy it either way
     s->nx = nx;
     s->ny = ny;
     s->nz = nz;
@@ -344,7 +344,11 @@ int fc_smoke_step_field(fc_smoke* s, const float* field_device, int reverse_flow
                        s->dtrail, s->dhead, s->dcount, s->dflag, s->P, s->L, scale, suspend_respawn ? 1 : 0,
                        confine_axis);
     SMOKE_HIP(s, hipGetLastError());
-    if (suspend_respawn) return FC_LBM_OK;
+    if (suspend_respawn) {
+        // nothing to respawn; still blocking, so the caller may reuse the field at once
+        SMOKE_HIP(s, hipStreamSynchronize(s->stream));
+        return FC_LBM_OK;
+    }
     std::vector<uint8_t> flag(s->P);
     SMOKE_HIP(s, hipMemcpyAsync(flag.data(), s->dflag, s->P, hipMemcpyDeviceToHost, s->stream));
     SMOKE_HIP(s, hipStreamSynchronize(s->stream));

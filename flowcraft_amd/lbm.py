@@ -339,18 +339,21 @@ class LbmSolver:
         probe (nz/2, nz/3, 2nz/3 in turn) at most every probeIntervalMs.  Returns the
         substeps done: fewer when cancel() turns true between chunks, or i+1 after the
         substep whose probe found a divergence (state.diverged).  progress(f) receives
-        (i + z/nz) / substeps."""
+        (i + z/nz) / substeps.  In the in-place layout the chunks of a substep overwrite
+        the state, so a started substep is finished and cancel() is only asked at
+        substep boundaries."""
         import time
         if not self.IsValid():
             return 0
         st = state if state is not None else PacingState()
         nz = self.Nz()
+        undoable = self._layout == Layout.PINGPONG
         if progress:
             progress(0.0)
         for i in range(substeps):
             z0 = 0
             while z0 < nz:
-                if cancel and cancel():
+                if (z0 == 0 or undoable) and cancel and cancel():
                     return i
                 zc = min(st.chunkZDepth, nz - z0)
                 t0 = time.perf_counter()
@@ -364,7 +367,7 @@ class LbmSolver:
                 z0 += zc
                 if progress:
                     progress((i + z0 / nz) / substeps)
-            if cancel and cancel():
+            if undoable and cancel and cancel():
                 return i
             self._check(self._lib.fc_lbm_advance(self._h))
             now = time.perf_counter()

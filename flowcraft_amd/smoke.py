@@ -70,6 +70,13 @@ class SmokeTrailSystem:
         if isinstance(source, LbmSolver):
             self._check(self._lib.fc_smoke_step(self._h, source._h, *args))
         else:
+            import torch
+            n = self._dims[0] * self._dims[1] * self._dims[2] * 4
+            if not (isinstance(source, torch.Tensor) and source.is_cuda and source.dtype == torch.float32
+                    and source.is_contiguous() and source.numel() == n):
+                raise ValueError(f"source must be an LbmSolver or a contiguous float32 device tensor of {n} values")
+            # the library reads the field on its own stream: the work that wrote it must be done
+            torch.cuda.current_stream(source.device).synchronize()
             self._check(self._lib.fc_smoke_step_field(self._h, ctypes.c_void_p(source.data_ptr()), *args))
 
     def Trails(self):

@@ -243,10 +243,17 @@ public:
         if (!h_) return 0;
         using Clock = std::chrono::steady_clock;
         const int nz = nz_;
+        // A ping-pong substep writes the other copy, so chunks already run are dropped
+        // by not advancing; in place they overwrite the state, so a started substep is
+        // finished and a cancel takes effect at the next substep boundary.
+        int nx_i, ny_i, nz_i, q_i, st_i, layout = FC_LBM_LAYOUT_INPLACE;
+        fc_lbm_info(h_, &nx_i, &ny_i, &nz_i, &q_i, &st_i, &layout);
+        const bool undoable = layout == FC_LBM_LAYOUT_PINGPONG;
+        auto cancelled = [&]() { return cancel && cancel->load(std::memory_order_acquire); };
         if (progress) progress->store(0.0f, std::memory_order_relaxed);
         for (int i = 0; i < substeps; ++i) {
             for (int z0 = 0; z0 < nz;) {
-                if (cancel && cancel->load(std::memory_order_acquire)) return i;
+                if ((z0 == 0 || undoable) && cancelled()) return i;
                 const int zc = std::min(st.chunkZDepth, nz - z0);
                 const Clock::time_point t0 = Clock::now();
                 if (!Check(fc_lbm_step_range(h_, z0, zc)) || !Check(fc_lbm_sync(h_))) return i;
@@ -261,7 +268,7 @@ public:
                                         static_cast<float>(substeps),
                                     std::memory_order_relaxed);
             }
-            if (cancel && cancel->load(std::memory_order_acquire)) return i;
+            if (undoable && cancelled()) return i;
             if (!Check(fc_lbm_advance(h_))) return i;
             const Clock::time_point now = Clock::now();
             if (std::chrono::duration<double, std::milli>(now - st.lastProbe).count() >= probeIntervalMs && nz > 0) {

@@ -245,7 +245,8 @@ const char* fc_voxelize_last_error(void);
 typedef struct fc_smoke fc_smoke;
 
 /* A particle system over a grid: dimensions, cell size, origin and solid mask (host,
- * nx*ny*nz bytes, copied). */
+ * nx*ny*nz bytes, copied).  On a device failure *out still receives the handle, for
+ * fc_smoke_last_error; destroy it either way. */
 int fc_smoke_create(int nx, int ny, int nz, float cell_size, const float origin[3], const uint8_t* solid,
                     fc_smoke** out);
 int fc_smoke_destroy(fc_smoke* s);
@@ -268,7 +269,9 @@ int fc_smoke_shift_plane(fc_smoke* s, int axis, int index);
 int fc_smoke_step(fc_smoke* s, fc_lbm* h, int reverse_flow, float mm_per_sec_per_lattice_unit, float dt_seconds,
                   int suspend_respawn, int confine_axis, int confine_index);
 
-/* Same with a caller's field: device pointer, 4 floats per cell (fc_lbm_field_device). */
+/* Same with a caller's field: device pointer, 4 floats per cell (fc_lbm_field_device),
+ * complete when the call is made (the work that wrote it already synchronised).
+ * Blocking: the field may be reused when the call returns. */
 int fc_smoke_step_field(fc_smoke* s, const float* field_device, int reverse_flow, float mm_per_sec_per_lattice_unit,
                         float dt_seconds, int suspend_respawn, int confine_axis, int confine_index);
 

@@ -57,6 +57,25 @@ def test_cancel_between_chunks_keeps_the_last_full_substep():
     assert np.array_equal(a.ReadbackField().view(np.uint32), b.ReadbackField().view(np.uint32))
 
 
+@pytest.mark.parametrize("layout", [Layout.PINGPONG, Layout.INPLACE], ids=lambda l: l.name)
+def test_cancel_raised_inside_a_substep(layout):
+    """Cancel raised after the second chunk of substep 3: ping-pong drops the chunks
+    already run (3 substeps); in place the substep is finished first (4), since its
+    chunks have overwritten the state."""
+    a, b = _pair(layout)
+    st = PacingState(chunkZDepth=4)
+    flag = {"on": False}
+
+    def progress(f):
+        if f > 3.3 / 10:
+            flag["on"] = True
+
+    done = a.StepPaced(10, st, cancel=lambda: flag["on"], targetChunkMs=1e-6, progress=progress)
+    assert done == (3 if layout == Layout.PINGPONG else 4)
+    b.Step(None, done)
+    assert np.array_equal(a.ReadbackField().view(np.uint32), b.ReadbackField().view(np.uint32))
+
+
 def test_probe_stops_a_diverged_run():
     """A non-finite rest population (it stays in its cell) planted on the host in the
     layer the first probe reads (nz/2): the paced loop returns after that substep with
