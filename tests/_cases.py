@@ -83,3 +83,20 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     b = np.asarray(b, np.float64)
     scale = max(float(np.abs(b).max()), 1e-30)
     return float(np.abs(a - b).max()) / scale
+
+
+# Lattices at the edges of the layout: x not a multiple of the 64-cell block (one partial
+# block, a full block plus one cell), the minimum nx = 4, one-cell and two-cell y/z
+# extents where a cell touches both walls, obstacles next to the walls.  Each entry:
+# (grid builder, free-slip y, free-slip z).
+RAGGED_CASES = {
+    "100x3x5_sphere": (lambda: geometry.sphere_grid(100, 3, 5, (30, 1, 2), 1.5), True, True),
+    "70x1x1_channel": (lambda: geometry.sphere_grid(70, 1, 1, (-50, 0, 0), 1.0), True, True),
+    "4x2x3_noslip": (lambda: geometry.sphere_grid(4, 2, 3, (-50, 0, 0), 1.0), False, False),
+    "129x5x2_mixed": (lambda: geometry.sphere_grid(129, 5, 2, (64, 2, 1), 1.8), True, False),
+    "65x7x1_sphere_on_wall": (lambda: geometry.sphere_grid(65, 7, 1, (20, 0, 0), 2.5), False, True),
+}
+
+
+def ragged_params(fsy: bool, fsz: bool) -> Params:
+    return Params(tau=0.6, inletVelocity=0.05, smagorinskyCs=0.12, freeSlipY=fsy, freeSlipZ=fsz)

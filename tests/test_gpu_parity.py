@@ -14,8 +14,8 @@ import numpy as np
 import pytest
 
 from flowcraft_amd.lbm import LatticeModel, Layout, Params, Storage
-from tests._cases import (case_a_grid, case_a_params, make_engine, make_oracle, oracle_params, rel_err,
-                          small_sphere_grid, ulp_diff32, wing_grid)
+from tests._cases import (RAGGED_CASES, case_a_grid, case_a_params, make_engine, make_oracle, oracle_params,
+                          ragged_params, rel_err, small_sphere_grid, ulp_diff32, wing_grid)
 
 pytestmark = pytest.mark.gpu
 
@@ -105,6 +105,32 @@ def test_variants_fp32(variant, layout, steps):
     for model in MODELS:
         eng, orc = _run_pair(grid, v["params"], model, Storage.F32, layout, steps, v.get("seed_mid", True))
         _fields_close(eng.ReadbackField(), orc.field(), 1e-5, grid.solid)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: l.name)
+@pytest.mark.parametrize("name", sorted(RAGGED_CASES))
+def test_ragged_lattices_bitexact(name, layout):
+    """Lattices at the layout's edges (partial 64-cell blocks, nx = 4, one- and two-cell
+    y/z extents, an obstacle on a wall; the oracle equals FlowCraft's LbmReference on
+    them bit for bit): FP32 fields identical to the oracle's, FP16 identical to its
+    emulation of the format, after 1 and 15 steps."""
+    build, fsy, fsz = RAGGED_CASES[name]
+    grid = build()
+    p = ragged_params(fsy, fsz)
+    for model in MODELS:
+        for storage in (Storage.F32, Storage.F16):
+            eng = make_engine(grid, model, storage, layout)
+            eng.Reset(p, True)
+            orc = make_oracle(grid, model, storage)
+            oracle_params(orc, p)
+            orc.reset(True)
+            for n in (1, 14):
+                eng.Step(None, n)
+                orc.step(n)
+                g, o = eng.ReadbackField(), orc.field()
+                assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), \
+                    f"{model.name} {storage.name}: max ulp {ulp_diff32(g, o)}"
+            eng.Shutdown()
 
 
 @pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: l.name)
