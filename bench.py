@@ -10,8 +10,9 @@ deterministic geometry.  A "step" is one fused stream-collide pass over all
 33.5M cells; MLUPS counts every cell (solid included), the convention of
 FlowCraft/main.cpp:3781-3783.
 
-`value` is the FP32-storage throughput; FP16 and BF16 offset storage, and D3Q15,
-are reported under `variants`.  Timing: W untimed steps, then K steps bracketed
+`value` is the FP32-storage throughput; FP16 and BF16 offset storage, D3Q15, the
+other population layout and the 128x64x64 sphere (0.5M cells, launch-bound) are
+reported under `variants`.  Timing: W untimed steps, then K steps bracketed
 by HIP events recorded on the solver's own stream, after a device synchronise on
 both sides.  The roofline uses the algorithmic 2*Q*s bytes per cell update.
 The CPU baseline times FlowCraft's LbmReference::Step (oracle/_ref) — or the
@@ -69,12 +70,12 @@ def wing_grid():
                                    aoa_deg=4.0)
 
 
-def time_variant(torch, grid, model, storage, layout, steps, warmup, tau=None):
+def time_variant(torch, grid, model, storage, layout, steps, warmup, tau=None, params=None):
     from flowcraft_amd.lbm import LbmSolver, Params
     s = LbmSolver()
     if not s.Init(None, grid, model, storage=storage, layout=layout):
         raise RuntimeError(s.LastError())
-    s.Reset(Params(tau=TAU if tau is None else tau, inletVelocity=0.08, smagorinskyCs=0.1), False)
+    s.Reset(params or Params(tau=TAU if tau is None else tau, inletVelocity=0.08, smagorinskyCs=0.1), False)
     stream = torch.cuda.ExternalStream(s.Stream())
     s.Step(None, warmup)
     s.Sync()
@@ -381,7 +382,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from flowcraft_amd.lbm import LatticeModel, Layout, Storage
+    from flowcraft_amd.lbm import LatticeModel, Layout, Params, Storage
 
     torch.cuda.init()
     layout = Layout.INPLACE if args.layout == "inplace" else Layout.PINGPONG
@@ -427,6 +428,27 @@ def main():
                 "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "finite": r["finite"]}
             print(f"# {model.name} {storage.name}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         variants["fp16_over_fp32"] = round(variants["D3Q19_F16"]["mlups"] / main_r["mlups"], 3)
+        # the other population layout, same lattice
+        other = Layout.INPLACE if layout == Layout.PINGPONG else Layout.PINGPONG
+        for storage in (Storage.F32, Storage.F16):
+            r = time_variant(torch, grid, LatticeModel.D3Q19, storage, other, args.steps, args.warmup)
+            b = 4 if storage == Storage.F32 else 2
+            gbs = r["mlups"] * 1e6 * 2 * 19 * b / 1e9
+            variants[f"D3Q19_{storage.name}_{other.name.lower()}"] = {
+                "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+                "achieved_GBs": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                "population_bytes": r["population_bytes"], "finite": r["finite"]}
+        # the 128x64x64 sphere (0.5M cells, one wave of workgroups): launch- and
+        # latency-bound, reported beside the bandwidth-bound lattices
+        from flowcraft_amd import geometry
+        small = geometry.sphere_grid(128, 64, 64, (32, 32, 32), 8)
+        r = time_variant(torch, small, LatticeModel.D3Q19, Storage.F32, layout, 2000, 100,
+                         params=Params(tau=0.56, inletVelocity=0.05, smagorinskyCs=0.0))
+        gbs = r["mlups"] * 1e6 * 152 / 1e9
+        variants["D3Q19_F32_128x64x64"] = {
+            "workload": "128x64x64 sphere r 8 at (32,32,32), Cs 0, tau 0.56, u 0.05", "mlups": round(r["mlups"], 1),
+            "us_per_step": round(r["ms_per_step"] * 1e3, 2), "achieved_GBs": round(gbs, 1),
+            "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "finite": r["finite"]}
 
     bytes_per_cell = 2 * 19 * 4
     achieved = main_r["mlups"] * 1e6 * bytes_per_cell / 1e9
