@@ -34,6 +34,9 @@ and checks that the trails agree.
 `pacing` times an interactive frame (the field for display plus 60 steps) in FlowCraft's
 blocking order against the asynchronous snapshot that overlaps the copy with the steps.
 
+`error_vs_fp64` states each storage format's error against the FP64 engine after the
+same warm-up + timed steps from the same start (field and force sums).
+
 `warm_start` is the metric's second half, measured live: on --warm-shapes held-out
 shapes of the warm-start distribution (256x128x128, u = 0.06, tau = 0.503, FP16
 storage), steps and wall time until the trailing-mean drag coefficient is within 1%
@@ -231,6 +234,40 @@ def smoke_block(torch, grid, cpu=True, particles=8000, trail=10, steps=20):
         out["match"] = bool(np.array_equal(na, nb) and all(
             np.array_equal(ta[p, :na[p]].view(np.uint32), tb[p, :nb[p]].view(np.uint32)) for p in range(len(na))))
     s.Shutdown()
+    return out
+
+
+def fp64_error_block(torch, grid, steps):
+    """Error of each storage format against the FP64 engine (the GPU twin of the FP64
+    oracle, within 1e-12 of it on the parity lattices) after the bench's own number of
+    steps from the same start: per-channel max |difference| / max |FP64| over fluid cells
+    (density, velocity) and the relative error of the drag and lift sums."""
+    from flowcraft_amd.lbm import LatticeModel, LbmSolver, Params, Storage
+    fluid = torch.from_numpy(np.ascontiguousarray(grid.solid.reshape(-1) == 0)).cuda()
+    out = {"steps": steps, "reference": "FP64 engine, same lattice and start"}
+    ref = None
+    for st in (Storage.F64, Storage.F32, Storage.F16, Storage.BF16):
+        s = LbmSolver()
+        if not s.Init(None, grid, LatticeModel.D3Q19, storage=st):
+            raise RuntimeError(s.LastError())
+        s.Reset(Params(tau=TAU, inletVelocity=0.08, smagorinskyCs=0.1), False)
+        s.Step(None, steps)
+        f = torch.empty(grid.CellCount() * 4, dtype=torch.float32, device="cuda")
+        s.FieldDevice(f)
+        s.Sync()
+        forces = s.Forces()
+        s.Shutdown()
+        f = f.view(-1, 4)[fluid].double()
+        if ref is None:
+            ref = (f, forces)
+            continue
+        rf, rforces = ref
+        e = ((f - rf).abs().amax(0) / rf.abs().amax(0).clamp_min(1e-30)).tolist()
+        out[st.name.lower()] = {"rho": float(f"{e[0]:.3e}"), "u": float(f"{max(e[1:]):.3e}"),
+                                "drag": float(f"{abs(forces[0] - rforces[0]) / abs(rforces[0]):.3e}"),
+                                "lift": float(f"{abs(forces[1] - rforces[1]) / abs(rforces[1]):.3e}")}
+        del f
+    torch.cuda.empty_cache()
     return out
 
 
@@ -462,6 +499,7 @@ def main():
     vox = None if args.no_variants else voxelize_block(cpu=not args.no_cpu)
     smoke = None if args.no_variants else smoke_block(torch, grid, cpu=not args.no_cpu)
     pacing = None if args.no_variants else pacing_block(torch, grid)
+    fp64_err = None if args.no_variants else fp64_error_block(torch, grid, args.warmup + args.steps)
     warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(grid)
     line = {
@@ -473,7 +511,7 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "warm_start": warm,
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "error_vs_fp64": fp64_err, "warm_start": warm,
         "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)
