@@ -1037,8 +1037,20 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
 #endif
 // One workgroup = a kTileX by kTileY*CPT tile of one z layer; each thread updates CPT
 // cells kTileY rows apart.  The bulk pulls of all CPT cells are issued first.
+// The per-direction scalar offsets hold ~100 SGPRs, which caps the step at 7 waves per
+// SIMD.  Forcing 8 (FC_LBM_WAVES_PER_EU=8: 78 SGPRs, the rest spilled to VGPR lanes)
+// measured slower on the 512x256x256 wing (FP32 ping-pong 37.06 -> 36.9 GLUPS, FP16
+// 70.1 -> 68.2), so no limit is set by default.
+#ifndef FC_LBM_WAVES_PER_EU
+#define FC_LBM_WAVES_PER_EU 0
+#endif
+#if FC_LBM_WAVES_PER_EU > 0
+#define FC_LBM_STEP_ATTR __attribute__((amdgpu_waves_per_eu(FC_LBM_WAVES_PER_EU)))
+#else
+#define FC_LBM_STEP_ATTR
+#endif
 template <int Q, class S, class Lay, int CPT, bool PER>
-__global__ __launch_bounds__(256) void KStep(const StepArgs a) {
+__global__ __launch_bounds__(256) FC_LBM_STEP_ATTR void KStep(const StepArgs a) {
     using R = typename S::R;
     using T = typename S::T;
     S::StepMode();
