@@ -40,9 +40,10 @@ same warm-up + timed steps from the same start (field and force sums).
 `warm_start` is the metric's second half, measured live: on --warm-shapes held-out
 shapes of the warm-start distribution (256x128x128, u = 0.06, tau = 0.503, FP16
 storage), steps and wall time until the trailing-mean drag coefficient is within 1%
-of the plain start's value at 6000 steps, for the plain start, the learned warm
-start (committed surrogate weights, inference included) and FlowCraft's coarse-grid
-warm start.
+of the plain start's value at 6000 steps, for the plain start, the learned warm start
+(committed surrogate weights, inference included) and FlowCraft's coarse-grid warm start.
+`warm_start_settled` is the same study against the plain start's value at 24000 steps,
+where its trailing mean has settled, with the weights trained on 15000-step fields.
 """
 from __future__ import annotations
 
@@ -350,22 +351,27 @@ def voxelize_block(cpu=True, reps=3, detail=2, res=640):
     return out
 
 
-def warm_start_block(torch, n_shapes):
+def warm_start_block(torch, n_shapes, settled=False):
     """Live warm-vs-plain comparison on held-out shapes of the warm-start distribution
-    (256x128x128 D3Q19 Smagorinsky Cs = 0.1, u = 0.06, tau = 0.503, FP16 storage, 6000-step
-    horizon), with the committed surrogate weights: steps and wall time until the
-    trailing-1000-step mean Cd is within 1% of the plain start's value at the horizon,
-    under the 'reach', 'hold' (1000 steps) and 'horizon' (stays to the end) rules."""
+    (256x128x128 D3Q19 Smagorinsky Cs = 0.1, u = 0.06, tau = 0.503, FP16 storage) with
+    committed surrogate weights: steps and wall time until the trailing-1000-step mean Cd
+    is within 1% of the plain start's value at the horizon, under the 'reach', 'hold'
+    (1000 steps) and 'horizon' (stays to the end) rules.  Default: the 6000-step horizon
+    and the weights trained on 3000-step fields.  settled=True: the 24000-step horizon,
+    where the plain run's Cd has settled, and the weights trained on 15000-step fields."""
     from flowcraft_amd import surrogate as sg
     from flowcraft_amd import warmstart as ws
     net = sg.Surrogate(seed=0)
-    ws.load_weights(net, ws.DEFAULT_WEIGHTS)
+    ws.load_weights(net, ws.SETTLED_WEIGHTS if settled else ws.DEFAULT_WEIGHTS)
+    horizon = ws.SETTLED_HORIZON if settled else ws.HORIZON
     t0 = time.time()
-    recs = ws.evaluate(net, (256, 128, 128), n_shapes, seed=1001, horizon=6000, every=10, window=1000, tol=0.01,
+    recs = ws.evaluate(net, (256, 128, 128), n_shapes, seed=1001, horizon=horizon, every=10, window=1000, tol=0.01,
                        u_in=0.06, tau=0.503, cs=0.1, log=lambda m: print("#" + m, file=sys.stderr, flush=True))
-    summ = ws.summarize(recs, 6000)
+    summ = ws.summarize(recs, horizon)
+    trained = ("320 engine-generated fields (15000 steps, mean of the last 1000)" if settled else
+               "600 engine-generated fields (3000 steps, mean of the last 1000)")
     out = {"workload": "256x128x128 D3Q19 Cs 0.1, u 0.06, tau 0.503, FP16 storage; held-out shapes seed 1001; "
-                       "surrogate trained on 600 engine-generated fields (3000 steps, mean of the last 1000)",
+                       f"surrogate trained on {trained}; horizon {horizon} steps",
            "shapes": len(recs), "wall_s": round(time.time() - t0, 1),
            "infer_ms_mean": round(float(np.mean([r["infer_ms"] for r in recs])), 2)}
     for rule in ws.RULES:
@@ -375,6 +381,7 @@ def warm_start_block(torch, n_shapes):
             continue
         out[rule] = {"plain_steps_p50": o["plain_steps"]["p50"], "warm_steps_p50": o["warm_steps"]["p50"],
                      "coarse_steps_p50": o["coarse_steps"]["p50"], "speedup_wall_p50": round(o["speedup_wall"]["p50"], 3),
+                     "speedup_wall_mean": round(o["speedup_wall"]["mean"], 3),
                      "speedup_wall_of_means": round(o["speedup_wall_of_means"], 3),
                      "coarse_speedup_wall_of_means": round(o["coarse_speedup_wall_of_means"], 3),
                      "warm_fallbacks": o["warm_fallbacks"], "coarse_fallbacks": o["coarse_fallbacks"]}
@@ -501,6 +508,7 @@ def main():
     pacing = None if args.no_variants else pacing_block(torch, grid)
     fp64_err = None if args.no_variants else fp64_error_block(torch, grid, args.warmup + args.steps)
     warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
+    warm_settled = warm_start_block(torch, args.warm_shapes, settled=True) if args.warm_shapes > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(grid)
     line = {
         "metric": METRIC, "value": round(main_r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
@@ -511,7 +519,7 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "error_vs_fp64": fp64_err, "warm_start": warm,
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "error_vs_fp64": fp64_err, "warm_start": warm, "warm_start_settled": warm_settled,
         "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)
