@@ -86,6 +86,19 @@ void DevFree(void* p) {
     if (p) (void)hipFree(p);
 }
 
+// A temporary device buffer freed on every return path, after the stream that may still
+// read it has drained.
+struct DevTmp {
+    hipStream_t st;
+    void* p = nullptr;
+    ~DevTmp() {
+        if (p) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(p);
+        }
+    }
+};
+
 #define FC_HIP(h, call)                                                                         \
     do {                                                                                        \
         hipError_t e_ = (call);                                                                 \
@@ -148,8 +161,9 @@ void FreeGeometry(fc_lbm* h) {
 // global cell order).  Allocates *out (at least one element).
 int Compact(fc_lbm* h, uint8_t mask, int** out, int* count) {
     const unsigned chunks = Blocks(h->N, kChunk);
-    int* dcounts = nullptr;
-    FC_HIP(h, hipMalloc(&dcounts, sizeof(int) * chunks));
+    DevTmp counts_buf{h->stream};
+    FC_HIP(h, hipMalloc(&counts_buf.p, sizeof(int) * chunks));
+    int* dcounts = static_cast<int*>(counts_buf.p);
     hipLaunchKernelGGL(KCountFlags, dim3(chunks), dim3(256), 0, h->stream, h->flags, mask, h->nx, h->ny, h->nz,
                        dcounts);
     std::vector<int> counts(chunks), offs(chunks);
@@ -165,7 +179,6 @@ int Compact(fc_lbm* h, uint8_t mask, int** out, int* count) {
     hipLaunchKernelGGL(KScatterFlags, dim3(chunks), dim3(256), 0, h->stream, h->flags, mask, h->nx, h->ny, h->nz,
                        dcounts, *out);
     FC_HIP(h, hipStreamSynchronize(h->stream));
-    DevFree(dcounts);
     *count = static_cast<int>(total);
     return FC_LBM_OK;
 }
@@ -368,9 +381,12 @@ int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf) {
     h->solid_host.assign(solid, solid + h->N);
     FC_HIP(h, hipMalloc(&h->dsolid, h->N));
     FC_HIP(h, hipMemcpyAsync(h->dsolid, solid, h->N, hipMemcpyHostToDevice, h->stream));
+    // the device copy of the SDF lives only while the link table is built
+    DevTmp sdfbuf{h->stream};
     float* dsdf = nullptr;
     if (sdf) {
-        FC_HIP(h, hipMalloc(&dsdf, h->N * sizeof(float)));
+        FC_HIP(h, hipMalloc(&sdfbuf.p, h->N * sizeof(float)));
+        dsdf = static_cast<float*>(sdfbuf.p);
         FC_HIP(h, hipMemcpyAsync(dsdf, sdf, h->N * sizeof(float), hipMemcpyHostToDevice, h->stream));
     }
     FC_HIP(h, hipMalloc(&h->flags, h->P));
@@ -401,7 +417,6 @@ int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf) {
     FC_HIP(h, hipMalloc(&h->sforce, 4 * h->rbytes * std::max(h->nsurf, 1)));
     FC_HIP(h, hipGetLastError());
     FC_HIP(h, hipStreamSynchronize(h->stream));
-    DevFree(dsdf);
     h->has_geometry = true;
     h->aero_geom_valid = false;
     h->has_state = false;
@@ -871,6 +886,11 @@ int fc_lbm_set_populations(fc_lbm* h, const void* in) {
     void* tmp = nullptr;
     FC_HIP(h, hipMalloc(&tmp, bytes));
     hipError_t e = hipMemcpyAsync(tmp, in, bytes, hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(h->stream);
+        DevFree(tmp);
+        return Fail(h, FC_LBM_EDEVICE, std::string("set_populations: ") + hipGetErrorString(e));
+    }
     Dispatch(h, [&]<int Q, class S, class Lay>() {
         using R = typename S::R;
         hipLaunchKernelGGL((KSetPops<Q, S, Lay>), dim3(Blocks(h->N, 256)), dim3(256), 0, h->stream, h->pops[0],
@@ -878,7 +898,7 @@ int fc_lbm_set_populations(fc_lbm* h, const void* in) {
     });
     h->has_state = true;
     RefreshSide(h);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    e = hipStreamSynchronize(h->stream);
     DevFree(tmp);
     if (e != hipSuccess) return Fail(h, FC_LBM_EDEVICE, std::string("set_populations: ") + hipGetErrorString(e));
     return FC_LBM_OK;
