@@ -4,6 +4,7 @@
 // infrastructure) instead of LbmReference.  Exit 0 = pass.
 #include <algorithm>
 #include <cmath>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <cstdint>
@@ -167,7 +168,20 @@ int main() {
             ok = ok && done == 12 && !st.diverged && std::memcmp(fa.data(), fb.data(), fa.size() * 4) == 0 &&
                  st.chunkZDepth == (target < 1.0 ? 4 : 256);
         }
-        std::printf("StepPaced vs Step (chunks of 4 and of 256 layers) -> %s\n", ok ? "PASS" : "FAIL");
+        // a cancel already raised stops before the first chunk in either layout: no substep
+        // done, state untouched
+        for (int layout : {FC_LBM_LAYOUT_PINGPONG, FC_LBM_LAYOUT_INPLACE}) {
+            LbmSolver::EngineOptions o2; o2.layout = layout;
+            LbmSolver a, b;
+            a.Init(nullptr, grid, LatticeModel::D3Q19, o2); a.Reset(p, true); a.Step(nullptr, 3);
+            b.Init(nullptr, grid, LatticeModel::D3Q19, o2); b.Reset(p, true); b.Step(nullptr, 3);
+            std::atomic<bool> cancel{true};
+            LbmSolver::PacingState st;
+            const int done = a.StepPaced(5, st, &cancel);
+            std::vector<float> fa = a.ReadbackField(), fb = b.ReadbackField();
+            ok = ok && done == 0 && std::memcmp(fa.data(), fb.data(), fa.size() * 4) == 0;
+        }
+        std::printf("StepPaced vs Step (chunks of 4 and of 256 layers; raised cancel) -> %s\n", ok ? "PASS" : "FAIL");
         if (!ok) ++failures;
     }
     {

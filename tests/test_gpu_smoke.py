@@ -87,3 +87,20 @@ def test_device_steps_through_the_solver_state():
     for p in range(len(na)):
         assert np.array_equal(ta[p, :na[p]].view(np.uint32), tb[p, :nb[p]].view(np.uint32))
     eng.Shutdown()
+
+
+def test_field_source_is_checked():
+    """A field that is not a contiguous float32 device tensor of 4 values per cell is
+    refused before anything reaches the device."""
+    import torch
+    g = case_a_grid()
+    dev = SmokeTrailSystem(g)
+    dev.Reset(g, False, 50, 4)
+    n = g.nx * g.ny * g.nz * 4
+    for bad in (torch.zeros(n), torch.zeros(n - 4, device="cuda"), torch.zeros(n, dtype=torch.float64, device="cuda"),
+                torch.zeros(2 * n, device="cuda")[::2]):
+        with pytest.raises(ValueError):
+            dev.Step(g, bad, False, 300.0, 0.05)
+    dev.Step(g, torch.zeros(n, device="cuda"), False, 300.0, 0.05)  # a still field: accepted
+    _, lengths = dev.Trails()
+    assert (lengths == 2).all()
