@@ -616,6 +616,20 @@ int fc_lbm_field_device(fc_lbm* h, float* out_device) {
     return FieldChunk(h, 0, h->nz, out_device);
 }
 
+int fc_lbm_field_at(fc_lbm* h, const int* cells_device, long long n, float* out_device) {
+    if (!h || n < 0 || (n > 0 && (!cells_device || !out_device))) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    if (n == 0) return FC_LBM_OK;
+    Dispatch(h, [&]<int Q, class S, class Lay>() {
+        hipLaunchKernelGGL((KFieldAt<Q, S, Lay>), dim3(Blocks(n, 256)), dim3(256), 0, h->stream, h->pops[0],
+                           h->pops[1], h->t, h->nx, h->ny, h->nz, h->pery, h->perz, h->flags, cells_device, n,
+                           out_device);
+    });
+    FC_HIP(h, hipGetLastError());
+    return FC_LBM_OK;
+}
+
 int fc_lbm_snapshot_begin(fc_lbm* h) {
     if (!h) return FC_LBM_EINVAL;
     int rc = CheckReady(h);

@@ -347,10 +347,30 @@ __global__ void KSurfaceForce(const void* pops0, const void* pops1, long long P,
 // indexing).  Fluid: moments of the current populations (LbmReference.cpp:43-61);
 // solid: zero here, surface cells are filled by KScatterSurface.
 template <int Q, class S, class Lay>
+__device__ __forceinline__ float4 FluidField(const void* pops0, const void* pops1, long long t, const Geo& g, int x,
+                                             int y, int z) {
+    using R = typename S::R;
+    R rho = 0, ux = 0, uy = 0, uz = 0;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        const R fi = CurrentPop<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
+        rho += fi;
+        ux += fi * static_cast<R>(Lat<Q>::ex[d]);
+        uy += fi * static_cast<R>(Lat<Q>::ey[d]);
+        uz += fi * static_cast<R>(Lat<Q>::ez[d]);
+    }
+    if (rho > K<R>(1e-6f, 1e-6)) {
+        ux /= rho;
+        uy /= rho;
+        uz /= rho;
+    }
+    return make_float4(static_cast<float>(rho), static_cast<float>(ux), static_cast<float>(uy), static_cast<float>(uz));
+}
+
+template <int Q, class S, class Lay>
 __global__ void KField(const void* pops0, const void* pops1, long long P, long long t, int nx, int ny, int nz,
                        int pery, int perz, const uint8_t* __restrict__ flags, int z0, int zc,
                        float* __restrict__ out) {
-    using R = typename S::R;
     const long long cnt = static_cast<long long>(nx) * ny * zc;
     const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= cnt) return;
@@ -359,24 +379,24 @@ __global__ void KField(const void* pops0, const void* pops1, long long P, long l
     const int z = z0 + zr;
     const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!(flags[g.Idx(x, y, z)] & kSolid)) {
-        R rho = 0, ux = 0, uy = 0, uz = 0;
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            const R fi = CurrentPop<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
-            rho += fi;
-            ux += fi * static_cast<R>(Lat<Q>::ex[d]);
-            uy += fi * static_cast<R>(Lat<Q>::ey[d]);
-            uz += fi * static_cast<R>(Lat<Q>::ez[d]);
-        }
-        if (rho > K<R>(1e-6f, 1e-6)) {
-            ux /= rho;
-            uy /= rho;
-            uz /= rho;
-        }
-        v = make_float4(static_cast<float>(rho), static_cast<float>(ux), static_cast<float>(uy), static_cast<float>(uz));
-    }
+    if (!(flags[g.Idx(x, y, z)] & kSolid)) v = FluidField<Q, S, Lay>(pops0, pops1, t, g, x, y, z);
     reinterpret_cast<float4*>(out)[j] = v;
+}
+
+// The fluid part of the field at a list of cells (global index (z*ny + y)*nx + x, each in
+// [0, nx*ny*nz)): the same values as KField there, zero for solid cells.
+template <int Q, class S, class Lay>
+__global__ void KFieldAt(const void* pops0, const void* pops1, long long t, int nx, int ny, int nz, int pery,
+                         int perz, const uint8_t* __restrict__ flags, const int* __restrict__ cells, long long n,
+                         float* __restrict__ out) {
+    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int x, y, z;
+    Coords(cells[k], nx, ny, x, y, z);
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!(flags[g.Idx(x, y, z)] & kSolid)) v = FluidField<Q, S, Lay>(pops0, pops1, t, g, x, y, z);
+    reinterpret_cast<float4*>(out)[k] = v;
 }
 
 template <class R>

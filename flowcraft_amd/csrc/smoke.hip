@@ -38,7 +38,9 @@ struct fc_smoke {
     int* dhead = nullptr;     // oldest point of each ring
     int* dcount = nullptr;    // points held (0..L)
     uint8_t* dflag = nullptr; // after an advance: 0 moved or idle, 1 respawn, 2 cleared
-    float* dfield = nullptr;  // 4 floats per cell when stepping from a solver
+    int* dcells = nullptr;    // stepping from a solver: 8 corner cells per particle
+    float* dgath = nullptr;   // and the solver's field at them (4 floats each)
+    int gath_cap = 0;         // particles the two buffers hold
     float* dplace = nullptr;  // respawn positions (4 floats: index bits, x, y, z)
     int place_cap = 0;
     hipStream_t stream = nullptr;
@@ -68,29 +70,45 @@ __device__ __forceinline__ bool Solid(const SmokeGrid& g, const uint8_t* solid, 
     return solid[(static_cast<size_t>(z) * g.ny + y) * g.nx + x] != 0;
 }
 
-// Velocity at a world position: trilinear over the 8 surrounding cell centres, clamped at
-// the grid edge, solid cells zero, grid (ux, uy, uz) returned as world (uy, ux, uz).
-__device__ __forceinline__ void Sample(const SmokeGrid& g, const uint8_t* solid, const float* field, float px,
-                                       float py, float pz, float& vx, float& vy, float& vz) {
+// The 8 cell centres around a world position (clamped at the grid edge) and the
+// trilinear weights, grid x from world Y and grid y from world X.
+__device__ __forceinline__ void CornerCells(const SmokeGrid& g, float px, float py, float pz, int (&cell)[8], float& tx,
+                                            float& ty, float& tz) {
     const float lx = px - g.ox, ly = py - g.oy, lz = pz - g.oz;
     const float gxf = ly / g.cs - 0.5f, gyf = lx / g.cs - 0.5f, gzf = lz / g.cs - 0.5f;
     const int gx0 = static_cast<int>(floorf(gxf)), gy0 = static_cast<int>(floorf(gyf)),
               gz0 = static_cast<int>(floorf(gzf));
-    const float tx = gxf - static_cast<float>(gx0), ty = gyf - static_cast<float>(gy0),
-                tz = gzf - static_cast<float>(gz0);
-    float c[8][3];
+    tx = gxf - static_cast<float>(gx0);
+    ty = gyf - static_cast<float>(gy0);
+    tz = gzf - static_cast<float>(gz0);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int x = min(max(gx0 + (k & 1), 0), g.nx - 1);
         const int y = min(max(gy0 + ((k >> 1) & 1), 0), g.ny - 1);
         const int z = min(max(gz0 + (k >> 2), 0), g.nz - 1);
-        const size_t cell = (static_cast<size_t>(z) * g.ny + y) * g.nx + x;
-        if (solid[cell]) {
+        cell[k] = (z * g.ny + y) * g.nx + x;
+    }
+}
+
+// Velocity at a world position: trilinear over the 8 surrounding cell centres, solid
+// cells zero, grid (ux, uy, uz) returned as world (uy, ux, uz).  fetch(k, cell) gives the
+// 4-float field entry of corner k (the whole field, or values gathered per particle).
+template <class Fetch>
+__device__ __forceinline__ void Sample(const SmokeGrid& g, const uint8_t* solid, Fetch fetch, float px, float py,
+                                       float pz, float& vx, float& vy, float& vz) {
+    int cell[8];
+    float tx, ty, tz;
+    CornerCells(g, px, py, pz, cell, tx, ty, tz);
+    float c[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (solid[cell[k]]) {
             c[k][0] = c[k][1] = c[k][2] = 0.0f;
         } else {
-            c[k][0] = field[cell * 4 + 2];
-            c[k][1] = field[cell * 4 + 1];
-            c[k][2] = field[cell * 4 + 3];
+            const float* f = fetch(k, cell[k]);
+            c[k][0] = f[2];
+            c[k][1] = f[1];
+            c[k][2] = f[3];
         }
     }
     float r[3];
@@ -109,6 +127,32 @@ __device__ __forceinline__ void Sample(const SmokeGrid& g, const uint8_t* solid,
     vz = r[2];
 }
 
+// The newest trail point of particle p (its head).
+__device__ __forceinline__ const float* Head(const float* trail, const int* head, int p, int n, int L) {
+    const int last = (head[p] + n - 1) % L;
+    return trail + (static_cast<size_t>(p) * L + last) * 3;
+}
+
+// Corner cells of every particle's head for a gathered field (8 per particle; a particle
+// with no points, or a head outside the grid, lists valid cells whose values go unused).
+__global__ void KSmokeCorners(const SmokeGrid g, const float* __restrict__ trail, const int* __restrict__ head,
+                              const int* __restrict__ count, int P, int L, int* __restrict__ cells) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int n = count[p];
+    if (n > 0) {
+        const float* t = Head(trail, head, p, n, L);
+        float tx, ty, tz;
+        CornerCells(g, t[0], t[1], t[2], c, tx, ty, tz);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cells[static_cast<size_t>(p) * 8 + k] = c[k];
+}
+
+// GATHER: `field` holds 8 gathered entries per particle (KSmokeCorners' cells) instead
+// of the whole field.
+template <bool GATHER>
 __global__ void KSmokeAdvance(const SmokeGrid g, const uint8_t* __restrict__ solid, const float* __restrict__ field,
                               float* __restrict__ trail, int* __restrict__ head, int* __restrict__ count,
                               uint8_t* __restrict__ flag, int P, int L, float scale, int suspend, int confine_axis) {
@@ -138,7 +182,11 @@ __global__ void KSmokeAdvance(const SmokeGrid g, const uint8_t* __restrict__ sol
         return;
     }
     float vx, vy, vz;
-    Sample(g, solid, field, cx, cy, cz, vx, vy, vz);
+    if constexpr (GATHER)
+        Sample(g, solid, [&](int k, int) { return field + (static_cast<size_t>(p) * 8 + k) * 4; }, cx, cy, cz, vx, vy,
+               vz);
+    else
+        Sample(g, solid, [&](int, int c) { return field + static_cast<size_t>(c) * 4; }, cx, cy, cz, vx, vy, vz);
     if (confine_axis == 0) vy = 0.0f;  // grid-x plane: world Y fixed
     if (confine_axis == 1) vx = 0.0f;  // grid-y plane: world X fixed
     if (confine_axis == 2) vz = 0.0f;
@@ -249,6 +297,23 @@ int CheckAxis(fc_smoke* s, int axis) {
     return FC_LBM_OK;
 }
 
+// After an advance (complete on the device): respawn the marked particles, in particle
+// order, with the reference's generator.
+int FinishStep(fc_smoke* s, int reverse_flow, int suspend_respawn, int confine_axis, int confine_index) {
+    if (suspend_respawn) return FC_LBM_OK;
+    std::vector<uint8_t> flag(s->P);
+    SMOKE_HIP(s, hipMemcpyAsync(flag.data(), s->dflag, s->P, hipMemcpyDeviceToHost, s->stream));
+    SMOKE_HIP(s, hipStreamSynchronize(s->stream));
+    std::vector<float> rec;
+    for (int p = 0; p < s->P; ++p) {
+        if (flag[p] != 1) continue;
+        float pos[3];
+        Respawn(s, reverse_flow, confine_axis, confine_index, pos);
+        PushRec(rec, p, pos);
+    }
+    return Place(s, rec);
+}
+
 }  // namespace
 
 extern "C" {
@@ -258,6 +323,7 @@ int fc_smoke_create(int nx, int ny, int nz, float cell_size, const float origin[
     if (!out) return FC_LBM_EINVAL;
     *out = nullptr;
     if (nx <= 0 || ny <= 0 || nz <= 0 || !origin || !solid || !(cell_size > 0.0f)) return FC_LBM_EINVAL;
+    if (static_cast<long long>(nx) * ny * nz >= (1LL << 31)) return FC_LBM_EINVAL;  // cell indices are int
     fc_smoke* s = new fc_smoke();
     *out = s;  // returned even on failure so fc_smoke_last_error can explain; destroy it either way
     s->nx = nx;
@@ -277,8 +343,8 @@ int fc_smoke_destroy(fc_smoke* s) {
     if (!s) return FC_LBM_EINVAL;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (void* p : {static_cast<void*>(s->dsolid), static_cast<void*>(s->dtrail), static_cast<void*>(s->dhead),
-                    static_cast<void*>(s->dcount), static_cast<void*>(s->dflag), static_cast<void*>(s->dfield),
-                    static_cast<void*>(s->dplace)})
+                    static_cast<void*>(s->dcount), static_cast<void*>(s->dflag), static_cast<void*>(s->dcells),
+                    static_cast<void*>(s->dgath), static_cast<void*>(s->dplace)})
         if (p) (void)hipFree(p);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -340,42 +406,52 @@ int fc_smoke_step_field(fc_smoke* s, const float* field_device, int reverse_flow
     const float dt = std::min(dt_seconds, 0.1f);
     const float scale = mm_per_sec_per_lattice_unit * dt;
     const SmokeGrid g{s->nx, s->ny, s->nz, s->cs, s->o[0], s->o[1], s->o[2]};
-    hipLaunchKernelGGL(KSmokeAdvance, dim3((s->P + 255) / 256), dim3(256), 0, s->stream, g, s->dsolid, field_device,
-                       s->dtrail, s->dhead, s->dcount, s->dflag, s->P, s->L, scale, suspend_respawn ? 1 : 0,
-                       confine_axis);
+    hipLaunchKernelGGL(KSmokeAdvance<false>, dim3((s->P + 255) / 256), dim3(256), 0, s->stream, g, s->dsolid,
+                       field_device, s->dtrail, s->dhead, s->dcount, s->dflag, s->P, s->L, scale,
+                       suspend_respawn ? 1 : 0, confine_axis);
     SMOKE_HIP(s, hipGetLastError());
-    if (suspend_respawn) {
-        // nothing to respawn; still blocking, so the caller may reuse the field at once
-        SMOKE_HIP(s, hipStreamSynchronize(s->stream));
-        return FC_LBM_OK;
-    }
-    std::vector<uint8_t> flag(s->P);
-    SMOKE_HIP(s, hipMemcpyAsync(flag.data(), s->dflag, s->P, hipMemcpyDeviceToHost, s->stream));
+    // blocking, so the caller may reuse the field at once
     SMOKE_HIP(s, hipStreamSynchronize(s->stream));
-    std::vector<float> rec;
-    for (int p = 0; p < s->P; ++p) {
-        if (flag[p] != 1) continue;
-        float pos[3];
-        Respawn(s, reverse_flow, confine_axis, confine_index, pos);
-        PushRec(rec, p, pos);
-    }
-    return Place(s, rec);
+    return FinishStep(s, reverse_flow, suspend_respawn, confine_axis, confine_index);
 }
 
 int fc_smoke_step(fc_smoke* s, fc_lbm* h, int reverse_flow, float mm_per_sec_per_lattice_unit, float dt_seconds,
                   int suspend_respawn, int confine_axis, int confine_index) {
     if (!s || !h) return FC_LBM_EINVAL;
+    int rc = CheckAxis(s, confine_axis);
+    if (rc) return rc;
     int nx, ny, nz, q, st, lay;
     if (fc_lbm_info(h, &nx, &ny, &nz, &q, &st, &lay) != FC_LBM_OK || nx != s->nx || ny != s->ny || nz != s->nz)
         return SmokeFail(s, FC_LBM_EINVAL, "solver lattice does not match the smoke grid");
     if (dt_seconds <= 0.0f || s->P == 0) return FC_LBM_OK;
-    if (!s->dfield) SMOKE_HIP(s, hipMalloc(&s->dfield, static_cast<size_t>(nx) * ny * nz * 4 * sizeof(float)));
-    int rc = fc_lbm_field_device(h, s->dfield);
+    // Only the 8 cells around each head are read: their field values are gathered from the
+    // solver's populations (fc_lbm_field_at, the same values as the whole field there),
+    // all on the solver's stream, behind the steps already queued on it.
+    if (s->gath_cap < s->P) {
+        if (s->dcells) SMOKE_HIP(s, hipFree(s->dcells));
+        if (s->dgath) SMOKE_HIP(s, hipFree(s->dgath));
+        s->dcells = nullptr;
+        s->dgath = nullptr;
+        s->gath_cap = 0;
+        SMOKE_HIP(s, hipMalloc(&s->dcells, static_cast<size_t>(s->P) * 8 * sizeof(int)));
+        SMOKE_HIP(s, hipMalloc(&s->dgath, static_cast<size_t>(s->P) * 8 * 4 * sizeof(float)));
+        s->gath_cap = s->P;
+    }
+    const hipStream_t hs = static_cast<hipStream_t>(fc_lbm_stream(h));
+    const SmokeGrid g{s->nx, s->ny, s->nz, s->cs, s->o[0], s->o[1], s->o[2]};
+    const dim3 grid((s->P + 255) / 256), block(256);
+    // the trails were last written on the smoke stream
+    SMOKE_HIP(s, hipStreamSynchronize(s->stream));
+    hipLaunchKernelGGL(KSmokeCorners, grid, block, 0, hs, g, s->dtrail, s->dhead, s->dcount, s->P, s->L, s->dcells);
+    SMOKE_HIP(s, hipGetLastError());
+    rc = fc_lbm_field_at(h, s->dcells, static_cast<long long>(s->P) * 8, s->dgath);
     if (rc) return SmokeFail(s, rc, fc_lbm_last_error(h));
-    rc = fc_lbm_sync(h);
-    if (rc) return SmokeFail(s, rc, fc_lbm_last_error(h));
-    return fc_smoke_step_field(s, s->dfield, reverse_flow, mm_per_sec_per_lattice_unit, dt_seconds, suspend_respawn,
-                               confine_axis, confine_index);
+    const float dt = std::min(dt_seconds, 0.1f);
+    hipLaunchKernelGGL(KSmokeAdvance<true>, grid, block, 0, hs, g, s->dsolid, s->dgath, s->dtrail, s->dhead, s->dcount,
+                       s->dflag, s->P, s->L, mm_per_sec_per_lattice_unit * dt, suspend_respawn ? 1 : 0, confine_axis);
+    SMOKE_HIP(s, hipGetLastError());
+    SMOKE_HIP(s, hipStreamSynchronize(hs));
+    return FinishStep(s, reverse_flow, suspend_respawn, confine_axis, confine_index);
 }
 
 int fc_smoke_shift_plane(fc_smoke* s, int axis, int index) {

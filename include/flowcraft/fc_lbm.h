@@ -138,6 +138,12 @@ int fc_lbm_readback_field(fc_lbm* h, float* out);
 /* The whole field into a device buffer of 4*N floats (no host round trip). */
 int fc_lbm_field_device(fc_lbm* h, float* out_device);
 
+/* Extension: the field at n listed cells (device array of global cell indices
+ * (z*ny + y)*nx + x, each in [0, N)) into a device buffer of 4*n floats: fluid cells as
+ * fc_lbm_field_device gives them, solid cells zero.  Enqueued on the handle's stream
+ * (asynchronous); for consumers that sample a few cells (smoke trails). */
+int fc_lbm_field_at(fc_lbm* h, const int* cells_device, long long n, float* out_device);
+
 /* Field snapshots for interactive front ends, in place of FlowCraft's paced, z-chunked
  * readback (ReadbackFieldPacedGPU, FlowCraft/main.cpp:2488-2555), which exists so that
  * a readback never holds the solver's queue for more than ~3 ms.  Here the copy never
