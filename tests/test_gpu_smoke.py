@@ -104,3 +104,31 @@ def test_field_source_is_checked():
     dev.Step(g, torch.zeros(n, device="cuda"), False, 300.0, 0.05)  # a still field: accepted
     _, lengths = dev.Trails()
     assert (lengths == 2).all()
+
+
+@pytest.mark.parametrize("layout,storage", [("PINGPONG", "F32"), ("INPLACE", "F32"), ("PINGPONG", "F16")])
+def test_field_at_equals_whole_field(layout, storage):
+    """fc_lbm_field_at (what the smoke step gathers) gives, bit for bit, the whole field's
+    values at the listed cells for fluid cells and zeros for solid ones."""
+    import torch
+    from flowcraft_amd.lbm import Layout, Storage
+    g = case_a_grid()
+    eng = make_engine(g, storage=Storage[storage], layout=Layout[layout])
+    eng.Reset(case_a_params(), True)
+    eng.Step(None, 7)
+    N = g.nx * g.ny * g.nz
+    whole = torch.empty(N * 4, device="cuda")
+    eng.FieldDevice(whole)
+    rng = np.random.default_rng(5)
+    cells = np.concatenate([rng.integers(0, N, 3000), np.flatnonzero(g.solid.reshape(-1))[:50], [0, N - 1]])
+    dc = torch.from_numpy(cells.astype(np.int32)).cuda()
+    out = torch.empty(len(cells) * 4, device="cuda")
+    assert eng._lib.fc_lbm_field_at(eng._h, dc.data_ptr(), len(cells), out.data_ptr()) == 0
+    eng.Sync()
+    got = out.view(-1, 4).cpu().numpy()
+    ref = whole.view(-1, 4).cpu().numpy()[cells]
+    solid = g.solid.reshape(-1)[cells] == 1
+    assert solid.any() and (~solid).any()
+    assert np.array_equal(got[~solid].view(np.uint32), ref[~solid].view(np.uint32))
+    assert (got[solid] == 0).all()
+    eng.Shutdown()
