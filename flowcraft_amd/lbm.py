@@ -310,6 +310,14 @@ class LbmSolver:
         if self.IsValid():
             self._check(self._lib.fc_lbm_field_device(self._h, out_tensor.data_ptr()))
 
+    def FieldAt(self, cells_tensor, out_tensor) -> None:
+        """The field at listed cells (int32 device tensor of global indices) into a float32
+        device tensor of 4 values per cell: fluid cells as FieldDevice, solid cells zero.
+        Asynchronous on the solver's stream (Sync before reading on another)."""
+        if self.IsValid():
+            self._check(self._lib.fc_lbm_field_at(self._h, cells_tensor.data_ptr(), int(cells_tensor.numel()),
+                                                  out_tensor.data_ptr()))
+
     def ReadbackStats(self) -> Stats:
         """LbmSolver::ReadbackStats (FlowCraft/LbmSolver.cpp:739-788), reduced on device."""
         if not self.IsValid():
