@@ -383,18 +383,24 @@ __global__ void KField(const void* pops0, const void* pops1, long long P, long l
     reinterpret_cast<float4*>(out)[j] = v;
 }
 
-// The fluid part of the field at a list of cells (global index (z*ny + y)*nx + x, each in
-// [0, nx*ny*nz)): the same values as KField there, zero for solid cells.
+// The fluid part of the field at a list of cells (global index (z*ny + y)*nx + x): the
+// same values as KField there, zero for solid cells and for indices outside
+// [0, nx*ny*nz), which read nothing.
 template <int Q, class S, class Lay>
 __global__ void KFieldAt(const void* pops0, const void* pops1, long long t, int nx, int ny, int nz, int pery,
                          int perz, const uint8_t* __restrict__ flags, const int* __restrict__ cells, long long n,
                          float* __restrict__ out) {
     const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    int x, y, z;
-    Coords(cells[k], nx, ny, x, y, z);
-    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
+    const int c = cells[k];
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (c < 0 || static_cast<long long>(c) >= static_cast<long long>(nx) * ny * nz) {
+        reinterpret_cast<float4*>(out)[k] = v;
+        return;
+    }
+    int x, y, z;
+    Coords(c, nx, ny, x, y, z);
+    const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     if (!(flags[g.Idx(x, y, z)] & kSolid)) v = FluidField<Q, S, Lay>(pops0, pops1, t, g, x, y, z);
     reinterpret_cast<float4*>(out)[k] = v;
 }

@@ -139,9 +139,10 @@ int fc_lbm_readback_field(fc_lbm* h, float* out);
 int fc_lbm_field_device(fc_lbm* h, float* out_device);
 
 /* Extension: the field at n listed cells (device array of global cell indices
- * (z*ny + y)*nx + x, each in [0, N)) into a device buffer of 4*n floats: fluid cells as
- * fc_lbm_field_device gives them, solid cells zero.  Enqueued on the handle's stream
- * (asynchronous); for consumers that sample a few cells (smoke trails). */
+ * (z*ny + y)*nx + x) into a device buffer of 4*n floats: fluid cells as
+ * fc_lbm_field_device gives them, solid cells and indices outside [0, N) zero.  Enqueued
+ * on the handle's stream (asynchronous); for consumers that sample a few cells (smoke
+ * trails). */
 int fc_lbm_field_at(fc_lbm* h, const int* cells_device, long long n, float* out_device);
 
 /* Field snapshots for interactive front ends, in place of FlowCraft's paced, z-chunked

@@ -120,12 +120,15 @@ def test_field_at_equals_whole_field(layout, storage):
     whole = torch.empty(N * 4, device="cuda")
     eng.FieldDevice(whole)
     rng = np.random.default_rng(5)
-    cells = np.concatenate([rng.integers(0, N, 3000), np.flatnonzero(g.solid.reshape(-1))[:50], [0, N - 1]])
+    cells = np.concatenate([rng.integers(0, N, 3000), np.flatnonzero(g.solid.reshape(-1))[:50], [0, N - 1],
+                            [-1, N, N + 12345]])  # the last three are outside the lattice: zeros
     dc = torch.from_numpy(cells.astype(np.int32)).cuda()
     out = torch.empty(len(cells) * 4, device="cuda")
     eng.FieldAt(dc, out)
     eng.Sync()
     got = out.view(-1, 4).cpu().numpy()
+    assert (got[-3:] == 0).all()
+    got, cells = got[:-3], cells[:-3]
     ref = whole.view(-1, 4).cpu().numpy()[cells]
     solid = g.solid.reshape(-1)[cells] == 1
     assert solid.any() and (~solid).any()
