@@ -319,7 +319,8 @@ def voxelize_block(cpu=True, reps=3, detail=2, res=640):
     (fc_voxelize, wall clock per blocking call: host setup, transfers and kernels, the
     44M-cell mask and distance field landing in host memory) against FlowCraft's own
     voxelizer (oracle/_ref, its threads on at most 16 host cores; the CPU-baseline
-    leg), and whether the grids are identical."""
+    leg), and whether the grids are identical.  `to_solver`: mesh to a solver ready to
+    step, through host arrays or with the grid left in HBM."""
     from flowcraft_amd import meshes
     from flowcraft_amd.voxelizer import Params, Voxelizer
     pos, idx = meshes.aircraft(detail=detail)
@@ -332,6 +333,29 @@ def voxelize_block(cpu=True, reps=3, detail=2, res=640):
     out = {"workload": f"aircraft mesh, {len(idx) // 3} triangles, resolution {res}: grid {g.nx}x{g.ny}x{g.nz} "
                        f"({g.CellCount() / 1e6:.1f}M cells)", "device_ms": round(dev_ms, 1),
            "solid_cells": int(g.solid.sum())}
+    # mesh to a solver ready to step (FP32 ping-pong engine of the frame): the grid through
+    # host arrays (Voxelize + set_geometry), or left in HBM (VoxelizeDevice + SetGeometryDevice)
+    import torch
+    from flowcraft_amd.lbm import LatticeModel, LbmSolver
+    eng = LbmSolver()
+    if not eng.Init(None, g, LatticeModel.D3Q19):
+        raise RuntimeError(eng.LastError())
+    solid_flat, sdf_flat = g.solid.reshape(-1), g.signedDistance.reshape(-1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gh = Voxelizer.Voxelize(pos, idx, p)
+        eng._check(eng._lib.fc_lbm_set_geometry(eng._h, gh.solid.ctypes.data, gh.signedDistance.ctypes.data))
+    host_chain = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, _, _, dsolid, dsdf = Voxelizer.VoxelizeDevice(pos, idx, p)
+        eng.SetGeometryDevice(dsolid, dsdf)
+    dev_chain = (time.perf_counter() - t0) * 1e3 / reps
+    same = bool(np.array_equal(eng.SolidMask(), solid_flat))
+    eng.Shutdown()
+    del dsolid, dsdf, solid_flat, sdf_flat
+    torch.cuda.empty_cache()
+    out["to_solver"] = {"host_arrays_ms": round(host_chain, 1), "in_hbm_ms": round(dev_chain, 1), "same_mask": same}
     if cpu:
         import oracle
         fn = oracle.ref_voxelize if oracle.ref_voxelizer_available() else oracle.voxelize

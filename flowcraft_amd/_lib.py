@@ -19,14 +19,14 @@ LAYOUT_PINGPONG, LAYOUT_INPLACE = 0, 1
 
 # Every symbol include/flowcraft/fc_lbm.h declares (tests check the export table).
 EXPORTS = (
-    "fc_lbm_create", "fc_lbm_destroy", "fc_lbm_last_error", "fc_lbm_params_default", "fc_lbm_set_geometry",
+    "fc_lbm_create", "fc_lbm_destroy", "fc_lbm_last_error", "fc_lbm_params_default", "fc_lbm_set_geometry", "fc_lbm_set_geometry_device",
     "fc_lbm_set_params", "fc_lbm_get_params", "fc_lbm_reset_uniform", "fc_lbm_reset_from_field",
     "fc_lbm_reset_from_field_device", "fc_lbm_step", "fc_lbm_step_range", "fc_lbm_advance",
     "fc_lbm_readback_field_range", "fc_lbm_readback_field", "fc_lbm_field_device", "fc_lbm_field_at", "fc_lbm_readback_stats",
     "fc_lbm_forces", "fc_lbm_forces_device", "fc_lbm_probe_health", "fc_lbm_has_diverged",
     "fc_lbm_get_populations", "fc_lbm_set_populations", "fc_lbm_sync", "fc_lbm_info", "fc_lbm_step_count",
     "fc_lbm_population_bytes", "fc_lbm_stream", "fc_lbm_boundary_links", "fc_lbm_surface_cells",
-    "fc_lbm_aero_forces", "fc_vox_params_default", "fc_voxelize_frame", "fc_voxelize", "fc_voxelize_last_error",
+    "fc_lbm_aero_forces", "fc_vox_params_default", "fc_voxelize_frame", "fc_voxelize", "fc_voxelize_device", "fc_voxelize_last_error",
     "fc_smoke_create", "fc_smoke_destroy", "fc_smoke_last_error", "fc_smoke_reset", "fc_smoke_scatter",
     "fc_smoke_shift_plane", "fc_smoke_step", "fc_smoke_step_field", "fc_smoke_trails", "fc_smoke_info",
     "fc_lbm_snapshot_begin", "fc_lbm_snapshot_poll", "fc_lbm_snapshot_wait",
@@ -103,6 +103,7 @@ def load() -> ctypes.CDLL:
     L.fc_lbm_params_default.argtypes = [ctypes.POINTER(FcLbmParams)]
     L.fc_lbm_params_default.restype = None
     L.fc_lbm_set_geometry.argtypes = [h, vp, vp]
+    L.fc_lbm_set_geometry_device.argtypes = [h, vp, vp]
     L.fc_lbm_set_params.argtypes = [h, ctypes.POINTER(FcLbmParams)]
     L.fc_lbm_get_params.argtypes = [h, ctypes.POINTER(FcLbmParams)]
     L.fc_lbm_reset_uniform.argtypes = [h, i]
@@ -136,6 +137,7 @@ def load() -> ctypes.CDLL:
     L.fc_vox_params_default.argtypes = [ctypes.POINTER(FcVoxParams)]
     L.fc_voxelize_frame.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), ctypes.POINTER(FcVoxGrid)]
     L.fc_voxelize.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), vp, vp]
+    L.fc_voxelize_device.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), vp, vp]
     L.fc_voxelize_last_error.argtypes = []
     L.fc_voxelize_last_error.restype = ctypes.c_char_p
     fl = ctypes.c_float

@@ -372,22 +372,28 @@ int fc_lbm_destroy(fc_lbm* h) {
 
 const char* fc_lbm_last_error(const fc_lbm* h) { return h ? h->err.c_str() : kNoHandle; }
 
-int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf) {
+// Mask and SDF from host arrays or (device) from device buffers.
+static int SetGeometry(fc_lbm* h, const uint8_t* solid, const float* sdf, bool device) {
     if (!h) return FC_LBM_EINVAL;
     if (!h->pops[0]) return Fail(h, FC_LBM_ESTATE, "handle was not created successfully");
     if (!solid) return Fail(h, FC_LBM_EINVAL, "solid mask is required");
     FC_HIP(h, hipStreamSynchronize(h->stream));
     FreeGeometry(h);
-    h->solid_host.assign(solid, solid + h->N);
+    const hipMemcpyKind in_kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     FC_HIP(h, hipMalloc(&h->dsolid, h->N));
-    FC_HIP(h, hipMemcpyAsync(h->dsolid, solid, h->N, hipMemcpyHostToDevice, h->stream));
+    FC_HIP(h, hipMemcpyAsync(h->dsolid, solid, h->N, in_kind, h->stream));
+    h->solid_host.resize(h->N);  // the mask stays on the host too (SolidMask, surface lists)
+    if (device)
+        FC_HIP(h, hipMemcpyAsync(h->solid_host.data(), h->dsolid, h->N, hipMemcpyDeviceToHost, h->stream));
+    else
+        std::copy(solid, solid + h->N, h->solid_host.begin());
     // the device copy of the SDF lives only while the link table is built
     DevTmp sdfbuf{h->stream};
     float* dsdf = nullptr;
     if (sdf) {
         FC_HIP(h, hipMalloc(&sdfbuf.p, h->N * sizeof(float)));
         dsdf = static_cast<float*>(sdfbuf.p);
-        FC_HIP(h, hipMemcpyAsync(dsdf, sdf, h->N * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        FC_HIP(h, hipMemcpyAsync(dsdf, sdf, h->N * sizeof(float), in_kind, h->stream));
     }
     FC_HIP(h, hipMalloc(&h->flags, h->P));
     FC_HIP(h, hipMemsetAsync(h->flags, 0, h->P, h->stream));
@@ -421,6 +427,14 @@ int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf) {
     h->aero_geom_valid = false;
     h->has_state = false;
     return FC_LBM_OK;
+}
+
+int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf) {
+    return SetGeometry(h, solid, sdf, false);
+}
+
+int fc_lbm_set_geometry_device(fc_lbm* h, const uint8_t* solid_device, const float* sdf_device) {
+    return SetGeometry(h, solid_device, sdf_device, true);
 }
 
 int fc_lbm_set_params(fc_lbm* h, const fc_lbm_params* p) {

@@ -338,8 +338,10 @@ int fc_voxelize_frame(const float* positions, long long nverts, const uint32_t* 
     return FC_LBM_OK;
 }
 
-int fc_voxelize(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
-                const fc_vox_params* params, uint8_t* solid, float* sdf) {
+// The whole voxelizer; results into host arrays (copied back) or, device_out, straight
+// into the caller's device buffers.
+static int Voxelize(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
+                    const fc_vox_params* params, uint8_t* solid, float* sdf, bool device_out) {
     int rc = CheckMesh(positions, nverts, indices, nindices, params);
     if (rc) return rc;
     if (nverts == 0 || nindices < 3) return FC_LBM_OK;
@@ -402,12 +404,16 @@ int fc_voxelize(const float* positions, long long nverts, const uint32_t* indice
     for (size_t b = 0; b < static_cast<size_t>(nbricks); ++b)
         if (start[b + 1] > start[b]) active.push_back(static_cast<uint32_t>(b));
 
-    DevBuf<uint8_t> dsolid;
-    DevBuf<float> dsdf;
+    DevBuf<uint8_t> own_solid;
+    DevBuf<float> own_sdf;
     DevBuf<uint32_t> dparent, dstart, dlist, dactive;
     DevBuf<TriRec> dtris;
-    VOX_HIP(hipMalloc(&dsolid.p, N));
-    VOX_HIP(hipMalloc(&dsdf.p, N * sizeof(float)));
+    if (!device_out) {
+        VOX_HIP(hipMalloc(&own_solid.p, N));
+        VOX_HIP(hipMalloc(&own_sdf.p, N * sizeof(float)));
+    }
+    uint8_t* const dsolid = device_out ? solid : own_solid.p;
+    float* const dsdf = device_out ? sdf : own_sdf.p;
     VOX_HIP(hipMalloc(&dparent.p, N * sizeof(uint32_t)));
     VOX_HIP(hipMalloc(&dtris.p, std::max<size_t>(recs.size(), 1) * sizeof(TriRec)));
     VOX_HIP(hipMalloc(&dstart.p, start.size() * sizeof(uint32_t)));
@@ -420,21 +426,35 @@ int fc_voxelize(const float* positions, long long nverts, const uint32_t* indice
         VOX_HIP(hipMemcpy(dactive.p, active.data(), active.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 
     const unsigned cells_blocks = static_cast<unsigned>((N + 255) / 256);
-    hipLaunchKernelGGL(KVoxInit, dim3(cells_blocks), dim3(256), 0, 0, dsolid.p, dsdf.p, dparent.p, N);
+    hipLaunchKernelGGL(KVoxInit, dim3(cells_blocks), dim3(256), 0, 0, dsolid, dsdf, dparent.p, N);
     const VoxArgs va{f.nx, f.ny, f.nz, f.cs, f.o.x, f.o.y, f.o.z, radius, tol, bx, by};
     if (!active.empty())
         hipLaunchKernelGGL(KVoxRaster, dim3(static_cast<unsigned>(active.size())), dim3(512), 0, 0, va, dtris.p,
-                           dstart.p, dlist.p, dactive.p, dsolid.p, dsdf.p);
+                           dstart.p, dlist.p, dactive.p, dsolid, dsdf);
     const long long rows = static_cast<long long>(f.ny) * f.nz;
-    hipLaunchKernelGGL(KUfRuns, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, 0, dsolid.p, dparent.p,
+    hipLaunchKernelGGL(KUfRuns, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, 0, dsolid, dparent.p,
                        f.nx, rows);
-    hipLaunchKernelGGL(KUfHook, dim3(cells_blocks), dim3(256), 0, 0, dsolid.p, dparent.p, f.nx, f.ny, f.nz);
-    hipLaunchKernelGGL(KUfCompress, dim3(cells_blocks), dim3(256), 0, 0, dsolid.p, dparent.p, N);
-    hipLaunchKernelGGL(KVoxFinal, dim3(cells_blocks), dim3(256), 0, 0, dsolid.p, dsdf.p, dparent.p, N, radius);
+    hipLaunchKernelGGL(KUfHook, dim3(cells_blocks), dim3(256), 0, 0, dsolid, dparent.p, f.nx, f.ny, f.nz);
+    hipLaunchKernelGGL(KUfCompress, dim3(cells_blocks), dim3(256), 0, 0, dsolid, dparent.p, N);
+    hipLaunchKernelGGL(KVoxFinal, dim3(cells_blocks), dim3(256), 0, 0, dsolid, dsdf, dparent.p, N, radius);
     VOX_HIP(hipGetLastError());
-    VOX_HIP(hipMemcpy(solid, dsolid.p, N, hipMemcpyDeviceToHost));
-    VOX_HIP(hipMemcpy(sdf, dsdf.p, N * sizeof(float), hipMemcpyDeviceToHost));
+    if (device_out) {
+        VOX_HIP(hipStreamSynchronize(0));
+    } else {
+        VOX_HIP(hipMemcpy(solid, dsolid, N, hipMemcpyDeviceToHost));
+        VOX_HIP(hipMemcpy(sdf, dsdf, N * sizeof(float), hipMemcpyDeviceToHost));
+    }
     return FC_LBM_OK;
+}
+
+int fc_voxelize(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
+                const fc_vox_params* params, uint8_t* solid, float* sdf) {
+    return Voxelize(positions, nverts, indices, nindices, params, solid, sdf, false);
+}
+
+int fc_voxelize_device(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
+                       const fc_vox_params* params, uint8_t* solid_device, float* sdf_device) {
+    return Voxelize(positions, nverts, indices, nindices, params, solid_device, sdf_device, true);
 }
 
 }  // extern "C"

@@ -310,6 +310,20 @@ class LbmSolver:
         if self.IsValid():
             self._check(self._lib.fc_lbm_field_device(self._h, out_tensor.data_ptr()))
 
+    def SetGeometryDevice(self, solid_tensor, sdf_tensor=None) -> None:
+        """Extension: replace the geometry from device tensors (uint8 mask and float32 SDF
+        of CellCount() values, e.g. Voxelizer.VoxelizeDevice's); a reset must follow."""
+        if not self.IsValid():
+            return
+        n = self.CellCount()
+        if solid_tensor.numel() != n or (sdf_tensor is not None and sdf_tensor.numel() != n):
+            raise ValueError(f"geometry tensors must hold {n} values")
+        import torch
+        torch.cuda.current_stream(solid_tensor.device).synchronize()
+        self._check(self._lib.fc_lbm_set_geometry_device(self._h, solid_tensor.data_ptr(),
+                                                         None if sdf_tensor is None else sdf_tensor.data_ptr()))
+        self._solid = solid_tensor.reshape(-1).cpu().numpy().astype(np.uint8)
+
     def FieldAt(self, cells_tensor, out_tensor) -> None:
         """The field at listed cells (int32 device tensor of global indices) into a float32
         device tensor of 4 values per cell: fluid cells as FieldDevice, solid cells zero.

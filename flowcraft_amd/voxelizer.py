@@ -62,6 +62,26 @@ class Voxelizer:
         return VoxelGrid(nx, ny, nz, solid, sdf, cs, origin)
 
 
+    @staticmethod
+    def VoxelizeDevice(positions, indices, params: Params = Params()):
+        """Extension: the same grid left in HBM, for LbmSolver.SetGeometryDevice:
+        ((nx, ny, nz), cellSize, origin, solid uint8 tensor [nz, ny, nx], signed distance
+        float32 tensor [nz, ny, nx]), both on the current device."""
+        import torch
+        lib = L.load()
+        pos, idx = _arrays(positions, indices)
+        (nx, ny, nz), cs, origin = Voxelizer.Frame(pos, idx, params)
+        solid = torch.empty((nz, ny, nx), dtype=torch.uint8, device="cuda")
+        sdf = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+        if nx * ny * nz:
+            torch.cuda.current_stream().synchronize()  # the buffers exist before the library writes them
+            rc = lib.fc_voxelize_device(pos.ctypes.data, len(pos), idx.ctypes.data, len(idx),
+                                        ctypes.byref(params._c()), solid.data_ptr(), sdf.data_ptr())
+            if rc != L.FC_LBM_OK:
+                raise L.FcLbmError(rc, lib.fc_voxelize_last_error().decode())
+        return (nx, ny, nz), cs, origin, solid, sdf
+
+
 def _arrays(positions, indices):
     pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
     idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)

@@ -101,6 +101,10 @@ void fc_lbm_params_default(fc_lbm_params* p);
  * (FlowCraft/LbmReference.cpp:104-112) and the solid-surface list. Host pointers. */
 int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf);
 
+/* Extension: the same from device buffers (N mask bytes, N SDF floats or NULL), e.g. the
+ * outputs of fc_voxelize_device; the mask is also copied back for the host-side uses. */
+int fc_lbm_set_geometry_device(fc_lbm* h, const uint8_t* solid_device, const float* sdf_device);
+
 /* Replaces LbmSolver::SetParams (FlowCraft/LbmSolver.h:135): state untouched. */
 int fc_lbm_set_params(fc_lbm* h, const fc_lbm_params* p);
 int fc_lbm_get_params(const fc_lbm* h, fc_lbm_params* p);
@@ -238,6 +242,11 @@ int fc_voxelize_frame(const float* positions, long long nverts, const uint32_t* 
  * is written for an empty mesh. */
 int fc_voxelize(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
                 const fc_vox_params* params, uint8_t* solid, float* sdf);
+
+/* Extension: the same into device buffers (nx*ny*nz bytes and floats of the frame), for
+ * fc_lbm_set_geometry_device: mesh to running solver without the grid crossing PCIe. */
+int fc_voxelize_device(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
+                       const fc_vox_params* params, uint8_t* solid_device, float* sdf_device);
 
 /* Message of the calling thread's last voxelizer error. */
 const char* fc_voxelize_last_error(void);

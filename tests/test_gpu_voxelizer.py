@@ -63,3 +63,29 @@ def test_voxelized_wing_runs_in_the_solver():
     st = s.ReadbackStats()
     assert not st.hasNaN and st.forceX > 0.0
     s.Shutdown()
+
+
+def test_mesh_to_solver_on_the_device():
+    """fc_voxelize_device + fc_lbm_set_geometry_device: the grid never leaves HBM, and a
+    solver set up that way steps bit for bit like one set up from the host grid."""
+    import torch
+    from flowcraft_amd.lbm import LatticeModel, LbmSolver, Params
+    from flowcraft_amd.voxelizer import Params as VParams, Voxelizer
+    pos, idx = meshes.wing(0.04, 0.4, 0.15, 100.0, 300.0, (0.0, 0.0, 0.0), "x", 8.0)
+    vp = VParams(96)
+    host = Voxelizer.Voxelize(pos, idx, vp)
+    dims, cs, org, dsolid, dsdf = Voxelizer.VoxelizeDevice(pos, idx, vp)
+    assert dims == (host.nx, host.ny, host.nz)
+    assert np.array_equal(dsolid.cpu().numpy(), host.solid)
+    assert np.array_equal(dsdf.cpu().numpy().view(np.uint32), host.signedDistance.view(np.uint32))
+    p = Params(tau=0.6, inletVelocity=0.05, smagorinskyCs=0.12)
+    a, b = LbmSolver(), LbmSolver()
+    assert a.Init(None, host, LatticeModel.D3Q19) and b.Init(None, host, LatticeModel.D3Q19)
+    b.SetGeometryDevice(dsolid, dsdf)
+    assert np.array_equal(b.SolidMask(), host.solid.reshape(-1))
+    for e in (a, b):
+        e.Reset(p, True)
+        e.Step(None, 20)
+    assert np.array_equal(a.ReadbackField().view(np.uint32), b.ReadbackField().view(np.uint32))
+    a.Shutdown()
+    b.Shutdown()
