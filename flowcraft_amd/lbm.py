@@ -135,6 +135,7 @@ class LbmSolver:
         self._error = ""
         self._params = Params()
         self._solid = np.zeros(0, np.uint8)
+        self._solid_dev = None
         self._dims = (0, 0, 0)
         self._model = LatticeModel.D3Q19
         self._storage = Storage.F32
@@ -177,6 +178,7 @@ class LbmSolver:
         self._h = h
         self._error = ""
         self._solid = solid
+        self._solid_dev = None
         self._dims = (grid.nx, grid.ny, grid.nz)
         self._model = LatticeModel(model)
         self._storage = Storage(storage)
@@ -216,6 +218,9 @@ class LbmSolver:
         return self._dims[0] * self._dims[1] * self._dims[2]
 
     def SolidMask(self) -> np.ndarray:
+        if self._solid is None and getattr(self, "_solid_dev", None) is not None:
+            self._solid = self._solid_dev.reshape(-1).cpu().numpy().astype(np.uint8)
+            self._solid_dev = None
         return self._solid
 
     # ---------------------------------------------------------------- state
@@ -322,7 +327,9 @@ class LbmSolver:
         torch.cuda.current_stream(solid_tensor.device).synchronize()
         self._check(self._lib.fc_lbm_set_geometry_device(self._h, solid_tensor.data_ptr(),
                                                          None if sdf_tensor is None else sdf_tensor.data_ptr()))
-        self._solid = solid_tensor.reshape(-1).cpu().numpy().astype(np.uint8)
+        # the host copy of the mask is made when SolidMask() is first asked for
+        self._solid = None
+        self._solid_dev = solid_tensor
 
     def FieldAt(self, cells_tensor, out_tensor) -> None:
         """The field at listed cells (int32 device tensor of global indices) into a float32
