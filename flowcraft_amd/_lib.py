@@ -22,7 +22,7 @@ EXPORTS = (
     "fc_lbm_create", "fc_lbm_destroy", "fc_lbm_last_error", "fc_lbm_params_default", "fc_lbm_set_geometry", "fc_lbm_set_geometry_device",
     "fc_lbm_set_params", "fc_lbm_get_params", "fc_lbm_reset_uniform", "fc_lbm_reset_from_field",
     "fc_lbm_reset_from_field_device", "fc_lbm_step", "fc_lbm_step_range", "fc_lbm_advance",
-    "fc_lbm_readback_field_range", "fc_lbm_readback_field", "fc_lbm_field_device", "fc_lbm_field_at", "fc_lbm_readback_stats",
+    "fc_lbm_readback_field_range", "fc_lbm_readback_field", "fc_lbm_readback_field_range_noalloc", "fc_lbm_field_device", "fc_lbm_field_at", "fc_lbm_readback_stats",
     "fc_lbm_forces", "fc_lbm_forces_device", "fc_lbm_probe_health", "fc_lbm_has_diverged",
     "fc_lbm_get_populations", "fc_lbm_set_populations", "fc_lbm_sync", "fc_lbm_info", "fc_lbm_step_count",
     "fc_lbm_population_bytes", "fc_lbm_stream", "fc_lbm_boundary_links", "fc_lbm_surface_cells",
@@ -114,6 +114,7 @@ def load() -> ctypes.CDLL:
     L.fc_lbm_advance.argtypes = [h]
     L.fc_lbm_readback_field_range.argtypes = [h, i, i, vp]
     L.fc_lbm_readback_field.argtypes = [h, vp]
+    L.fc_lbm_readback_field_range_noalloc.argtypes = [h, i, i, vp]
     L.fc_lbm_field_device.argtypes = [h, vp]
     L.fc_lbm_field_at.argtypes = [h, vp, ctypes.c_longlong, vp]
     L.fc_lbm_readback_stats.argtypes = [h, ctypes.POINTER(FcLbmStats)]

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,36 @@
 #include "step_launch.hpp"
 
 using namespace fc;
+
+// Staging of the allocation-free readback (fc_lbm_readback_field_range_noalloc): the
+// field of a z range passes through this array in chunks of whole layers.  Each GPU holds
+// its own copy for the life of the process, shared by every handle on that GPU.  Host code
+// never names it directly: its device address is resolved per device at handle creation
+// (ScratchAddress) and kept in the handle.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) float scratch[kScratchBytes / sizeof(float)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (which must be `device`), resolved
+// once per device.
+hipError_t ScratchAddress(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+}  // namespace
 
 struct fc_lbm {
     int nx = 0, ny = 0, nz = 0, Q = 19, storage = FC_LBM_STORE_F32, layout = FC_LBM_LAYOUT_PINGPONG;
@@ -52,6 +83,7 @@ struct fc_lbm {
     int* dprobe = nullptr;
     float* staging = nullptr;
     size_t staging_bytes = 0;
+    float* scratch = nullptr;  // this device's copy of the file-scope `scratch` array
     // field snapshots: device copy + pinned host copy, moved on their own stream
     float* snap_dev = nullptr;
     float* snap_host = nullptr;
@@ -231,9 +263,11 @@ void LaunchReduceForces(fc_lbm* h, double* out) {
                            static_cast<const float*>(h->sforce), h->nsurf, out);
 }
 
-int FieldChunk(fc_lbm* h, int z0, int zc, float* dout) {
+// The field of layers [z0, z0+zc) into dout (4 floats per cell, every cell written).
+// surface_ready: the surface forces of the current state are already in sforce.
+int FieldChunk(fc_lbm* h, int z0, int zc, float* dout, bool surface_ready = false) {
     const long long cnt = static_cast<long long>(h->nx) * h->ny * zc;
-    LaunchSurfaceForce(h);
+    if (!surface_ready) LaunchSurfaceForce(h);
     Dispatch(h, [&]<int Q, class S, class Lay>() {
         hipLaunchKernelGGL((KField<Q, S, Lay>), dim3(Blocks(cnt, 256)), dim3(256), 0, h->stream, h->pops[0],
                            h->pops[1], h->P, h->t, h->nx, h->ny, h->nz, h->pery, h->perz, h->flags, z0, zc, dout);
@@ -314,6 +348,11 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     h->cells_per_thread = 1;
     if (const char* e = std::getenv("FC_LBM_CELLS_PER_THREAD")) h->cells_per_thread = std::atoi(e) == 2 ? 2 : 1;
     FC_HIP(h, hipGetDevice(&h->device));
+    {
+        void* p = nullptr;
+        FC_HIP(h, ScratchAddress(h->device, &p));
+        h->scratch = static_cast<float*>(p);
+    }
     if (desc->stream) {
         h->stream = static_cast<hipStream_t>(desc->stream);
     } else {
@@ -718,6 +757,35 @@ int fc_lbm_readback_field_range(fc_lbm* h, int z0, int zcount, float* out) {
 int fc_lbm_readback_field(fc_lbm* h, float* out) {
     if (!h) return FC_LBM_EINVAL;
     return fc_lbm_readback_field_range(h, 0, h->nz, out);
+}
+
+int fc_lbm_readback_field_range_noalloc(fc_lbm* h, int z0, int zcount, float* out) {
+    if (!h || !out) return FC_LBM_EINVAL;
+    int rc = CheckReady(h);
+    if (rc) return rc;
+    if (!h->scratch) return Fail(h, FC_LBM_ESTATE, "handle was not created successfully");
+    const size_t layer = static_cast<size_t>(h->nx) * h->ny * 4;
+    if (layer * sizeof(float) > kScratchBytes)
+        return Fail(h, FC_LBM_EINVAL,
+                    "readback_field_range_noalloc: one z layer (nx*ny*16 = " + std::to_string(layer * sizeof(float)) +
+                        " bytes) exceeds the " + std::to_string(kScratchBytes >> 20) + " MiB scratch");
+    if (z0 < 0) {
+        zcount += z0;
+        z0 = 0;
+    }
+    if (z0 + zcount > h->nz) zcount = h->nz - z0;
+    if (zcount <= 0) return FC_LBM_OK;
+    const int per = static_cast<int>(kScratchBytes / (layer * sizeof(float)));
+    LaunchSurfaceForce(h);
+    for (int z = z0; z < z0 + zcount; z += per) {
+        const int zc = std::min(per, z0 + zcount - z);
+        rc = FieldChunk(h, z, zc, h->scratch, true);
+        if (rc) return rc;
+        FC_HIP(h, hipMemcpyAsync(out + layer * z, h->scratch, layer * zc * sizeof(float), hipMemcpyDeviceToHost,
+                                 h->stream));
+    }
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
 }
 
 int fc_lbm_forces_device(fc_lbm* h, double* out_device) {

@@ -139,6 +139,16 @@ int fc_lbm_readback_field_range(fc_lbm* h, int z0, int zcount, float* out);
 /* Replaces LbmSolver::ReadbackField (FlowCraft/LbmSolver.cpp:722-737): whole field, host. */
 int fc_lbm_readback_field(fc_lbm* h, float* out);
 
+/* No counterpart in the reference: fc_lbm_readback_field_range for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, clamping and result, bit for bit; it calls no hipMalloc/hipFree.  The field
+ * is staged through a fixed 32 MiB device array of the library (one per GPU, held until
+ * the process ends) in chunks of whole z layers, so any range works as long as ONE layer
+ * fits: nx*ny*16 bytes <= 32 MiB, else FC_LBM_EINVAL.  Every handle on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (handles on different threads or streams)
+ * race and must be serialised by the caller.  Blocking. */
+int fc_lbm_readback_field_range_noalloc(fc_lbm* h, int z0, int zcount, float* out);
+
 /* The whole field into a device buffer of 4*N floats (no host round trip). */
 int fc_lbm_field_device(fc_lbm* h, float* out_device);
 
