@@ -18,4 +18,4 @@ def test_cpp_dropin_matches_oracle():
     r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
-    assert r.stdout.count("PASS") == 8 and "FAIL" not in r.stdout
+    assert r.stdout.count("PASS") == 9 and "FAIL" not in r.stdout
