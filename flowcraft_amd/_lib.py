@@ -14,12 +14,12 @@ LIB_PATH = os.path.join(LIB_DIR, "libfc_lbm.so")
 
 FC_LBM_OK, FC_LBM_EINVAL, FC_LBM_ENOMEM, FC_LBM_EDEVICE, FC_LBM_ESTATE = 0, 1, 2, 3, 4
 D3Q15, D3Q19 = 0, 1
-STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
+STORE_F32, STORE_F16, STORE_BF16, STORE_F64, STORE_F16_SR, STORE_BF16_SR = 0, 1, 2, 3, 4, 5
 LAYOUT_PINGPONG, LAYOUT_INPLACE = 0, 1
 
 # Every symbol include/flowcraft/fc_lbm.h declares (tests check the export table).
 EXPORTS = (
-    "fc_lbm_create", "fc_lbm_destroy", "fc_lbm_last_error", "fc_lbm_params_default", "fc_lbm_set_geometry", "fc_lbm_set_geometry_device",
+    "fc_lbm_create", "fc_lbm_destroy", "fc_lbm_last_error", "fc_lbm_params_default", "fc_lbm_set_geometry", "fc_lbm_set_geometry_device", "fc_lbm_solid_mask",
     "fc_lbm_set_params", "fc_lbm_get_params", "fc_lbm_reset_uniform", "fc_lbm_reset_from_field",
     "fc_lbm_reset_from_field_device", "fc_lbm_step", "fc_lbm_step_range", "fc_lbm_advance",
     "fc_lbm_readback_field_range", "fc_lbm_readback_field", "fc_lbm_readback_field_range_noalloc", "fc_lbm_field_device", "fc_lbm_field_at", "fc_lbm_readback_stats",
@@ -104,6 +104,7 @@ def load() -> ctypes.CDLL:
     L.fc_lbm_params_default.restype = None
     L.fc_lbm_set_geometry.argtypes = [h, vp, vp]
     L.fc_lbm_set_geometry_device.argtypes = [h, vp, vp]
+    L.fc_lbm_solid_mask.argtypes = [h, vp]
     L.fc_lbm_set_params.argtypes = [h, ctypes.POINTER(FcLbmParams)]
     L.fc_lbm_get_params.argtypes = [h, ctypes.POINTER(FcLbmParams)]
     L.fc_lbm_reset_uniform.argtypes = [h, i]
@@ -138,7 +139,7 @@ def load() -> ctypes.CDLL:
     L.fc_vox_params_default.argtypes = [ctypes.POINTER(FcVoxParams)]
     L.fc_voxelize_frame.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), ctypes.POINTER(FcVoxGrid)]
     L.fc_voxelize.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), vp, vp]
-    L.fc_voxelize_device.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), vp, vp]
+    L.fc_voxelize_device.argtypes = [vp, ll, vp, ll, ctypes.POINTER(FcVoxParams), vp, vp, ll]
     L.fc_voxelize_last_error.argtypes = []
     L.fc_voxelize_last_error.restype = ctypes.c_char_p
     fl = ctypes.c_float

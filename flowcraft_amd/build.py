@@ -30,7 +30,8 @@ COMMON = COMPILE + ["-shared"]
 # in parallel and linked together.  The step kernel's instantiations are split per
 # lattice and storage format (step_inst.hip), the rest of the engine is fc_lbm.hip.
 _STEP_UNITS = [("step_inst.hip", [f"-DFC_Q={q}", f"-DFC_STORE={st}"])
-               for q in (19, 15) for st in ("StoreF32", "StoreF16", "StoreBF16", "StoreF64")]
+               for q in (19, 15) for st in ("StoreF32", "StoreF16", "StoreBF16", "StoreF64", "StoreF16SR",
+                                            "StoreBF16SR")]
 TARGETS = {
     "libfc_lbm.so": [("fc_lbm.hip", []), ("voxelize.hip", []), ("smoke.hip", [])] + _STEP_UNITS,
     "libfc_nn.so": [("nn_kernels.hip", [])],

@@ -153,11 +153,15 @@ void DispatchL(const fc_lbm* h, F& f) {
         f.template operator()<Q, S, PingPong>();
 }
 
+// Storage dispatch of every kernel but the step: the stochastically rounded formats hold
+// the same elements as their round-to-nearest twins and seed/restore/read back alike.
 template <int Q, class F>
 void DispatchS(const fc_lbm* h, F& f) {
     switch (h->storage) {
-        case FC_LBM_STORE_F16: DispatchL<Q, StoreF16<Q>>(h, f); break;
-        case FC_LBM_STORE_BF16: DispatchL<Q, StoreBF16<Q>>(h, f); break;
+        case FC_LBM_STORE_F16:
+        case FC_LBM_STORE_F16_SR: DispatchL<Q, StoreF16<Q>>(h, f); break;
+        case FC_LBM_STORE_BF16:
+        case FC_LBM_STORE_BF16_SR: DispatchL<Q, StoreBF16<Q>>(h, f); break;
         case FC_LBM_STORE_F64: DispatchL<Q, StoreF64<Q>>(h, f); break;
         default: DispatchL<Q, StoreF32<Q>>(h, f); break;
     }
@@ -169,6 +173,24 @@ void Dispatch(const fc_lbm* h, F&& f) {
         DispatchS<15>(h, f);
     else
         DispatchS<19>(h, f);
+}
+
+// The step kernel's dispatch, where the rounding of the stores differs.
+template <int Q, class F>
+void DispatchStepS(const fc_lbm* h, F& f) {
+    switch (h->storage) {
+        case FC_LBM_STORE_F16_SR: DispatchL<Q, StoreF16SR<Q>>(h, f); break;
+        case FC_LBM_STORE_BF16_SR: DispatchL<Q, StoreBF16SR<Q>>(h, f); break;
+        default: DispatchS<Q>(h, f); break;
+    }
+}
+
+template <class F>
+void DispatchStep(const fc_lbm* h, F&& f) {
+    if (h->Q == 15)
+        DispatchStepS<15>(h, f);
+    else
+        DispatchStepS<19>(h, f);
 }
 
 unsigned Blocks(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
@@ -222,6 +244,16 @@ int EnsureStaging(fc_lbm* h, size_t bytes) {
     h->staging_bytes = 0;
     FC_HIP(h, hipMalloc(&h->staging, bytes));
     h->staging_bytes = bytes;
+    return FC_LBM_OK;
+}
+
+// Host copy of the solid mask, made from the device copy when the geometry came from
+// device buffers.
+int EnsureSolidHost(fc_lbm* h) {
+    if (static_cast<long long>(h->solid_host.size()) == h->N) return FC_LBM_OK;
+    h->solid_host.resize(h->N);
+    FC_HIP(h, hipMemcpyAsync(h->solid_host.data(), h->dsolid, h->N, hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
     return FC_LBM_OK;
 }
 
@@ -319,7 +351,7 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     if (desc->nx < 4 || desc->ny < 1 || desc->nz < 1)
         return Fail(h, FC_LBM_EINVAL, "lattice must be at least 4 x 1 x 1");
     if (desc->model != FC_LBM_D3Q15 && desc->model != FC_LBM_D3Q19) return Fail(h, FC_LBM_EINVAL, "unknown model");
-    if (desc->storage < FC_LBM_STORE_F32 || desc->storage > FC_LBM_STORE_F64)
+    if (desc->storage < FC_LBM_STORE_F32 || desc->storage > FC_LBM_STORE_BF16_SR)
         return Fail(h, FC_LBM_EINVAL, "unknown storage");
     if (desc->layout != FC_LBM_LAYOUT_PINGPONG && desc->layout != FC_LBM_LAYOUT_INPLACE)
         return Fail(h, FC_LBM_EINVAL, "unknown layout");
@@ -421,11 +453,10 @@ static int SetGeometry(fc_lbm* h, const uint8_t* solid, const float* sdf, bool d
     const hipMemcpyKind in_kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     FC_HIP(h, hipMalloc(&h->dsolid, h->N));
     FC_HIP(h, hipMemcpyAsync(h->dsolid, solid, h->N, in_kind, h->stream));
-    h->solid_host.resize(h->N);  // the mask stays on the host too (SolidMask, surface lists)
-    if (device)
-        FC_HIP(h, hipMemcpyAsync(h->solid_host.data(), h->dsolid, h->N, hipMemcpyDeviceToHost, h->stream));
-    else
-        std::copy(solid, solid + h->N, h->solid_host.begin());
+    // A host copy of the mask serves the aerodynamic summary's lift-arrow column search; from
+    // host input it is kept now, from device input it is made on first use (EnsureSolidHost).
+    h->solid_host.clear();
+    if (!device) h->solid_host.assign(solid, solid + h->N);
     // the device copy of the SDF lives only while the link table is built
     DevTmp sdfbuf{h->stream};
     float* dsdf = nullptr;
@@ -474,6 +505,14 @@ int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf) {
 
 int fc_lbm_set_geometry_device(fc_lbm* h, const uint8_t* solid_device, const float* sdf_device) {
     return SetGeometry(h, solid_device, sdf_device, true);
+}
+
+int fc_lbm_solid_mask(fc_lbm* h, uint8_t* out) {
+    if (!h || !out) return FC_LBM_EINVAL;
+    if (!h->has_geometry) return Fail(h, FC_LBM_ESTATE, "no geometry: call fc_lbm_set_geometry first");
+    FC_HIP(h, hipMemcpyAsync(out, h->dsolid, h->N, hipMemcpyDeviceToHost, h->stream));
+    FC_HIP(h, hipStreamSynchronize(h->stream));
+    return FC_LBM_OK;
 }
 
 int fc_lbm_set_params(fc_lbm* h, const fc_lbm_params* p) {
@@ -623,6 +662,7 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     a.z0 = z0;
     a.zc = zcount;
     a.parity = par;
+    a.step = static_cast<uint32_t>(h->t);
     a.reverse = h->params.reverse_flow ? 1 : 0;
     a.fsy = h->params.free_slip_y ? 1 : 0;
     a.fsz = h->params.free_slip_z ? 1 : 0;
@@ -638,7 +678,7 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     const int tx = TileX(static_cast<int>(h->tbytes)), ty = TileY(static_cast<int>(h->tbytes));
     const long long tiles = static_cast<long long>((h->nx + tx - 1) / tx) * ((h->ny + ty * cpt - 1) / (ty * cpt)) * zcount;
     hipError_t e = hipSuccess;
-    Dispatch(h, [&]<int Q, class S, class Lay>() {
+    DispatchStep(h, [&]<int Q, class S, class Lay>() {
         InletPops<Q, S>(h, a);
         e = LaunchStep<Q, S, Lay>(a, static_cast<unsigned>(tiles), cpt, (h->pery | h->perz) != 0, h->stream);
     });
@@ -811,6 +851,8 @@ int fc_lbm_aero_forces(fc_lbm* h, float cell_size, const float origin[3], fc_lbm
     if (!h || !out || !origin) return FC_LBM_EINVAL;
     *out = fc_lbm_aero{};
     int rc = CheckReady(h);
+    if (rc) return rc;
+    rc = EnsureSolidHost(h);
     if (rc) return rc;
     const float ox = origin[0], oy = origin[1], oz = origin[2];
     LaunchSurfaceForce(h);

@@ -149,6 +149,7 @@ struct StoreF32 {
     using T = float;
     using R = float;
     static constexpr bool kOffset = false;
+    static constexpr bool kStochastic = false;
     static __device__ __forceinline__ R Load(T v, int) { return v; }
     static __device__ __forceinline__ T Store(R f, int) { return f; }
     static __device__ __forceinline__ R Raw(T v) { return v; }
@@ -171,6 +172,7 @@ struct StoreF16 {
     using T = _Float16;
     using R = float;
     static constexpr bool kOffset = true;
+    static constexpr bool kStochastic = false;
     static constexpr float kScale = 4096.0f;
     static __device__ __forceinline__ R Raw(T v) { return static_cast<float>(v); }
     static __device__ __forceinline__ T Enc(R g) {
@@ -197,6 +199,7 @@ struct StoreBF16 {
     using T = uint16_t;
     using R = float;
     static constexpr bool kOffset = true;
+    static constexpr bool kStochastic = false;
     static constexpr float kScale = 1.0f;
     static __device__ __forceinline__ R Raw(T v) { return __uint_as_float(static_cast<uint32_t>(v) << 16); }
     static __device__ __forceinline__ T Enc(R g) {
@@ -217,12 +220,72 @@ struct StoreF64 {
     using T = double;
     using R = double;
     static constexpr bool kOffset = false;
+    static constexpr bool kStochastic = false;
     static __device__ __forceinline__ R Load(T v, int) { return v; }
     static __device__ __forceinline__ T Store(R f, int) { return f; }
     static __device__ __forceinline__ R Raw(T v) { return v; }
     static __device__ __forceinline__ T Enc(R v) { return v; }
     static __device__ __forceinline__ void StepMode() {}
     static __device__ __forceinline__ T EncStep(R v) { return v; }
+};
+
+// Stochastic rounding of the step's stores (FC_LBM_STORE_F16_SR / _BF16_SR).  Rounding
+// to nearest loses any change of a stored offset smaller than half a unit in the last
+// place, every step, in the same direction: a slow drift (the outlet's density relaxation
+// moves the outlet column by about one FP16 unit per step) stalls.  Stochastic rounding
+// rounds up with probability equal to the discarded fraction, so the stored value is
+// right on average and slow drifts survive.  The random bits are a pure function of
+// (cell, step, direction) — the same for both layouts, any z-chunking and the CPU oracle:
+// the cell's 32-bit hash (the "lowbias32" integer finaliser of C. Wellons' hash-prospector
+// applied to cell*0x9E3779B1 ^ step*0x85EBCA77), advanced by d golden-ratio increments
+// (a Weyl sequence) for direction d.  Seeding and state restore keep rounding to nearest.
+__host__ __device__ __forceinline__ uint32_t SrHash(uint32_t cell, uint32_t step) {
+    uint32_t h = (cell * 0x9E3779B1u) ^ (step * 0x85EBCA77u);
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+constexpr uint32_t kSrWeyl = 0x9E3779B9u;
+
+// FP16 offsets, stochastically rounded: the top 13 of the direction's random bits are
+// added to the float's bit pattern below the half's last mantissa bit and the result is
+// truncated (round toward zero, v_cvt_pkrtz_f16_f32, two populations per instruction) —
+// the magnitude rounds up with probability (discarded bits) / 2^13.  With half
+// denormals flushed (StepMode) a sum below 2^-14 in magnitude is stored as a zero of its
+// sign; truncation never carries a value across 2^-14, so that is the whole rule.
+template <int Q>
+struct StoreF16SR : StoreF16<Q> {
+    static constexpr bool kStochastic = true;
+    static __device__ __forceinline__ float Dither(float g, uint32_t r) {
+        return __uint_as_float(__float_as_uint(g) + (r >> 19));
+    }
+    static __device__ __forceinline__ void EncodeSR(const float (&fo)[Q], _Float16 (&enc)[Q], uint32_t h) {
+        StaticFor<0, (Q + 1) / 2>([&](auto K) {
+            constexpr int d = 2 * K;
+            const float a = Dither(fo[d], h + static_cast<uint32_t>(d) * kSrWeyl);
+            const float b = d + 1 < Q ? Dither(fo[d + 1 < Q ? d + 1 : d], h + static_cast<uint32_t>(d + 1) * kSrWeyl) : 0.0f;
+            const uint32_t p = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
+            enc[d] = __builtin_bit_cast(_Float16, static_cast<uint16_t>(p & 0xFFFFu));
+            if constexpr (d + 1 < Q) enc[d + 1] = __builtin_bit_cast(_Float16, static_cast<uint16_t>(p >> 16));
+        });
+    }
+};
+
+// BF16 offsets, stochastically rounded: the top 16 random bits added to the float's bit
+// pattern, then its high half kept (truncation).
+template <int Q>
+struct StoreBF16SR : StoreBF16<Q> {
+    static constexpr bool kStochastic = true;
+    static __device__ __forceinline__ void EncodeSR(const float (&fo)[Q], uint16_t (&enc)[Q], uint32_t h) {
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            const uint32_t r = h + static_cast<uint32_t>(d) * kSrWeyl;
+            enc[d] = static_cast<uint16_t>((__float_as_uint(fo[d]) + (r >> 16)) >> 16);
+        });
+    }
 };
 
 // ------------------------------------------------------ boundary records
@@ -260,6 +323,7 @@ struct StepArgs {
     int nx, ny, nz;
     int z0, zc;
     int parity;      // step index & 1
+    uint32_t step;   // step index (low 32 bits): the stochastic-rounding formats' random stream
     int reverse, fsy, fsz, pery, perz;
     float tau, u_in, cs;
     double tau_d, u_in_d, cs_d;
@@ -722,12 +786,25 @@ __device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
 
 // Stored form of a cell's post-collision working values (inside the step kernel, which
 // has set the storage format's StepMode).
+// `h`: the cell's stochastic-rounding hash (SrHash), used by the *SR formats only.
 template <int Q, class S>
-__device__ __forceinline__ void EncodeCell(const typename S::R (&fo)[Q], typename S::T (&enc)[Q]) {
-    StaticFor<0, Q>([&](auto I) {
-        constexpr int d = I;
-        enc[d] = S::EncStep(fo[d]);
-    });
+__device__ __forceinline__ void EncodeCell(const typename S::R (&fo)[Q], typename S::T (&enc)[Q], uint32_t h) {
+    if constexpr (S::kStochastic) {
+        S::EncodeSR(fo, enc, h);
+    } else {
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            enc[d] = S::EncStep(fo[d]);
+        });
+    }
+}
+
+template <class S>
+__device__ __forceinline__ uint32_t CellSrHash(const StepArgs& a, int x, int y, int z) {
+    if constexpr (S::kStochastic)
+        return SrHash(static_cast<uint32_t>((z * a.ny + y) * a.nx + x), a.step);
+    else
+        return 0u;
 }
 
 template <int Q, class S>
@@ -791,7 +868,15 @@ __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRs
     using T = typename S::T;
     using R = typename S::R;
     T enc[Q];
-    EncodeCell<Q, S>(fo, enc);
+    EncodeCell<Q, S>(fo, enc, CellSrHash<S>(a, x, y, z));
+    auto store_all = [&]() {
+        StaticFor<0, Q>([&](auto I) {
+            constexpr int d = I;
+            constexpr int dx = StoreDx<Q, Lay>(d);
+            constexpr int aux = dx != 0 ? kShiftedStoreAux : kStoreAux;
+            BufStore<T, aux>(dst_layer, LaneSel<dx>(ln), a.st_off[d], enc[d]);
+        });
+    };
     if constexpr (PER && Lay::kInPlace) {
         if (seam) {
             T* dst = static_cast<T*>(a.pop_dst);
@@ -800,18 +885,10 @@ __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRs
                 PutPop(&dst[SlotIndex<Q, true>(g, d, a.parity, x, y, z)], enc[d]);
             });
         } else {
-            StaticFor<0, Q>([&](auto I) {
-                constexpr int d = I;
-                constexpr int aux = StoreDx<Q, Lay>(d) != 0 ? kShiftedStoreAux : kStoreAux;
-                BufStore<T, aux>(dst_layer, LaneSel<StoreDx<Q, Lay>(d)>(ln), a.st_off[d], enc[d]);
-            });
+            store_all();
         }
     } else {
-        StaticFor<0, Q>([&](auto I) {
-            constexpr int d = I;
-            constexpr int aux = StoreDx<Q, Lay>(d) != 0 ? kShiftedStoreAux : kStoreAux;
-            BufStore<T, aux>(dst_layer, LaneSel<StoreDx<Q, Lay>(d)>(ln), a.st_off[d], enc[d]);
-        });
+        store_all();
     }
     if constexpr (Lay::kInPlace) {
         if (rec != nullptr) {
@@ -859,7 +936,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
         } else {
             if (x == 1 || x == a.nx - 2) {
                 T enc[Q];
-                EncodeCell<Q, S>(fo, enc);
+                EncodeCell<Q, S>(fo, enc, CellSrHash<S>(a, x, y, z));
                 WriteMoments<Q, S>(static_cast<R*>(a.side_mom), enc, par, x == 1 ? 0 : 1, a.ny * a.nz, z * a.ny + y);
             }
         }
@@ -1035,6 +1112,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
 #ifndef FC_LBM_PIN_PULLS
 #define FC_LBM_PIN_PULLS 1
 #endif
+
 // One workgroup = a kTileX by kTileY*CPT tile of one z layer; each thread updates CPT
 // cells kTileY rows apart.  The bulk pulls of all CPT cells are issued first.
 // The per-direction scalar offsets hold ~100 SGPRs, which caps the step at 7 waves per

@@ -1,5 +1,6 @@
 // Explicit instantiations of the step-kernel launcher for one lattice and storage
-// format, selected at compile time: -DFC_Q=19|15 -DFC_STORE=StoreF32|StoreF16|StoreBF16|StoreF64.
+// format, selected at compile time: -DFC_Q=19|15 -DFC_STORE=StoreF32|StoreF16|StoreBF16|StoreF64|
+// StoreF16SR|StoreBF16SR.
 #include "step_launch.hpp"
 
 #ifndef FC_Q

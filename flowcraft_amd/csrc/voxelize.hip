@@ -339,9 +339,11 @@ int fc_voxelize_frame(const float* positions, long long nverts, const uint32_t* 
 }
 
 // The whole voxelizer; results into host arrays (copied back) or, device_out, straight
-// into the caller's device buffers.
+// into the caller's device buffers of `capacity` cells each (checked against the frame on
+// the host, before any device work).
 static int Voxelize(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
-                    const fc_vox_params* params, uint8_t* solid, float* sdf, bool device_out) {
+                    const fc_vox_params* params, uint8_t* solid, float* sdf, bool device_out,
+                    long long capacity = 0) {
     int rc = CheckMesh(positions, nverts, indices, nindices, params);
     if (rc) return rc;
     if (nverts == 0 || nindices < 3) return FC_LBM_OK;
@@ -349,6 +351,9 @@ static int Voxelize(const float* positions, long long nverts, const uint32_t* in
     const Frame f = MakeFrame(positions, nverts, *params);
     const long long N = static_cast<long long>(f.nx) * f.ny * f.nz;
     if (N >= (1LL << 32) - 1) return VoxFail(FC_LBM_EINVAL, "grid too large (cells must fit 32-bit indices)");
+    if (device_out && capacity < N)
+        return VoxFail(FC_LBM_EINVAL, "output buffers hold " + std::to_string(capacity) + " cells, the frame has " +
+                                          std::to_string(N));
     const float radius = f.cs * 0.8660254f, tol = f.cs * 1e-3f;
 
     // triangle setup and brick index (host, triangle order)
@@ -453,8 +458,8 @@ int fc_voxelize(const float* positions, long long nverts, const uint32_t* indice
 }
 
 int fc_voxelize_device(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
-                       const fc_vox_params* params, uint8_t* solid_device, float* sdf_device) {
-    return Voxelize(positions, nverts, indices, nindices, params, solid_device, sdf_device, true);
+                       const fc_vox_params* params, uint8_t* solid_device, float* sdf_device, long long capacity) {
+    return Voxelize(positions, nverts, indices, nindices, params, solid_device, sdf_device, true, capacity);
 }
 
 }  // extern "C"

@@ -38,10 +38,14 @@ class LatticeModel(enum.IntEnum):
 
 
 class Storage(enum.IntEnum):
+    """Population storage (fc_lbm.h FC_LBM_STORE_*): the _SR formats round the step's
+    stores stochastically instead of to nearest."""
     F32 = L.STORE_F32
     F16 = L.STORE_F16
     BF16 = L.STORE_BF16
     F64 = L.STORE_F64
+    F16_SR = L.STORE_F16_SR
+    BF16_SR = L.STORE_BF16_SR
 
 
 class Layout(enum.IntEnum):
@@ -126,6 +130,46 @@ class VoxelGrid:
         return (z * self.ny + y) * self.nx + x
 
 
+def _check_tensor(t, name, dtypes, numel=None, min_numel=None):
+    """The library reads raw device pointers: anything but a contiguous GPU tensor of the
+    expected element type and size would be read as the wrong bytes or overrun."""
+    if not (hasattr(t, "data_ptr") and hasattr(t, "dtype") and hasattr(t, "is_cuda")):
+        raise ValueError(f"{name} must be a torch tensor")
+    if t.dtype not in dtypes:
+        raise ValueError(f"{name} must have dtype {' or '.join(str(d) for d in dtypes)}, not {t.dtype}")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} must hold {numel} values, not {t.numel()}")
+    if min_numel is not None and t.numel() < min_numel:
+        raise ValueError(f"{name} must hold at least {min_numel} values, not {t.numel()}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def _check_on_gpu(**tensors) -> None:
+    for name, t in tensors.items():
+        if t is not None and not t.is_cuda:
+            raise ValueError(f"{name.replace('_', ' ')} must be a GPU tensor")
+
+
+def check_geometry_tensors(cells: int, solid, sdf=None) -> None:
+    """Arguments of LbmSolver.SetGeometryDevice: a uint8/bool mask and an optional float32
+    signed distance of `cells` values each, contiguous, on the GPU."""
+    import torch
+    _check_tensor(solid, "solid mask", (torch.uint8, torch.bool), numel=cells)
+    if sdf is not None:
+        _check_tensor(sdf, "signed distance", (torch.float32,), numel=cells)
+    _check_on_gpu(solid_mask=solid, signed_distance=sdf)
+
+
+def check_field_at_tensors(cells, out) -> None:
+    """Arguments of LbmSolver.FieldAt: int32 cell indices and a float32 output of at least
+    4 values per cell, contiguous, on the GPU."""
+    import torch
+    _check_tensor(cells, "cell list", (torch.int32,))
+    _check_tensor(out, "field output", (torch.float32,), min_numel=4 * cells.numel())
+    _check_on_gpu(cell_list=cells, field_output=out)
+
+
 class LbmSolver:
     """Drop-in for FlowCraft's LbmSolver on one MI355X (see module docstring)."""
 
@@ -135,7 +179,6 @@ class LbmSolver:
         self._error = ""
         self._params = Params()
         self._solid = np.zeros(0, np.uint8)
-        self._solid_dev = None
         self._dims = (0, 0, 0)
         self._model = LatticeModel.D3Q19
         self._storage = Storage.F32
@@ -153,7 +196,7 @@ class LbmSolver:
         if grid is None or grid.CellCount() == 0:
             self._error = "empty voxel grid"
             return False
-        solid = np.ascontiguousarray(np.asarray(grid.solid, dtype=np.uint8).reshape(-1))
+        solid = np.array(grid.solid, dtype=np.uint8).reshape(-1)  # a copy: SolidMask() is the solver's own
         if solid.size != grid.CellCount():
             self._error = "voxel grid solid mask has the wrong size"
             return False
@@ -178,7 +221,6 @@ class LbmSolver:
         self._h = h
         self._error = ""
         self._solid = solid
-        self._solid_dev = None
         self._dims = (grid.nx, grid.ny, grid.nz)
         self._model = LatticeModel(model)
         self._storage = Storage(storage)
@@ -218,9 +260,12 @@ class LbmSolver:
         return self._dims[0] * self._dims[1] * self._dims[2]
 
     def SolidMask(self) -> np.ndarray:
-        if self._solid is None and getattr(self, "_solid_dev", None) is not None:
-            self._solid = self._solid_dev.reshape(-1).cpu().numpy().astype(np.uint8)
-            self._solid_dev = None
+        """LbmSolver::SolidMask (FlowCraft/LbmSolver.h:216-218).  After SetGeometryDevice the
+        host copy is made on first use from the library's own device copy of the mask."""
+        if self._solid is None and self.IsValid():
+            out = np.empty(self.CellCount(), np.uint8)
+            self._check(self._lib.fc_lbm_solid_mask(self._h, out.ctypes.data))
+            self._solid = out
         return self._solid
 
     # ---------------------------------------------------------------- state
@@ -316,26 +361,27 @@ class LbmSolver:
             self._check(self._lib.fc_lbm_field_device(self._h, out_tensor.data_ptr()))
 
     def SetGeometryDevice(self, solid_tensor, sdf_tensor=None) -> None:
-        """Extension: replace the geometry from device tensors (uint8 mask and float32 SDF
-        of CellCount() values, e.g. Voxelizer.VoxelizeDevice's); a reset must follow."""
+        """Extension: replace the geometry from device tensors (uint8/bool mask and float32
+        SDF of CellCount() values, contiguous, on the GPU; e.g. Voxelizer.VoxelizeDevice's);
+        a reset must follow.  Raises ValueError for any other tensor.  The library copies
+        both, so the tensors may be reused afterwards."""
         if not self.IsValid():
             return
-        n = self.CellCount()
-        if solid_tensor.numel() != n or (sdf_tensor is not None and sdf_tensor.numel() != n):
-            raise ValueError(f"geometry tensors must hold {n} values")
+        check_geometry_tensors(self.CellCount(), solid_tensor, sdf_tensor)
         import torch
         torch.cuda.current_stream(solid_tensor.device).synchronize()
         self._check(self._lib.fc_lbm_set_geometry_device(self._h, solid_tensor.data_ptr(),
                                                          None if sdf_tensor is None else sdf_tensor.data_ptr()))
         # the host copy of the mask is made when SolidMask() is first asked for
         self._solid = None
-        self._solid_dev = solid_tensor
 
     def FieldAt(self, cells_tensor, out_tensor) -> None:
-        """The field at listed cells (int32 device tensor of global indices) into a float32
-        device tensor of 4 values per cell: fluid cells as FieldDevice, solid cells zero.
-        Asynchronous on the solver's stream (Sync before reading on another)."""
+        """The field at listed cells (contiguous int32 device tensor of global indices) into a
+        contiguous float32 device tensor of at least 4 values per cell: fluid cells as
+        FieldDevice, solid cells zero.  Raises ValueError for any other tensor.  Asynchronous
+        on the solver's stream (Sync before reading on another)."""
         if self.IsValid():
+            check_field_at_tensors(cells_tensor, out_tensor)
             self._check(self._lib.fc_lbm_field_at(self._h, cells_tensor.data_ptr(), int(cells_tensor.numel()),
                                                   out_tensor.data_ptr()))
 

@@ -6,8 +6,10 @@
 
 precision — the 512x256x256 NACA 0012 wing (chord 128, leading edge at x = 128,
     angle of attack 4 deg, extruded over z; D3Q19 and D3Q15 Smagorinsky Cs = 0.1,
-    u = 0.08, tau = 0.5005, free-slip y/z walls), run from the uniform rest start
-    in FP64, FP32, FP16-offset and BF16-offset storage.  The FP64 run (the GPU
+    u = 0.08, tau = 0.5005, free-slip y/z walls; --dims scales the wing with the
+    lattice), run from the uniform rest start in FP64, FP32, FP16-offset and
+    BF16-offset storage (rounded to nearest, or stochastically: f16sr, bf16sr), and the
+    FP64 chaos control f64p (the inlet speed raised by one part in a million).  The FP64 run (the GPU
     twin of the FP64 oracle, equal to it within 1e-12 on the parity cases) is the
     reference.  Drag and lift coefficients Cd = Fx / (0.5 u^2 A), Cl = Fy /
     (0.5 u^2 A) with A = chord * nz (lift is along +y: the angle of attack rotates
@@ -45,8 +47,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 def c2_grid():
-    return geometry.naca_wing_grid(512, 256, 256, m=0.0, p=0.0, t=0.12, chord=128.0, le_x=128.0, le_y=128.0,
-                                   aoa_deg=4.0)
+    return wing_grid(512, 256, 256)
 
 
 def c3_grid():
@@ -81,18 +82,40 @@ def force_series(solver, steps, every):
 TAU_D3Q15 = 0.6
 
 
+def wing_grid(nx=512, ny=256, nz=256):
+    """The precision workload's NACA 0012 wing, scaled with the lattice: chord nx/4, leading
+    edge at x = nx/4 on the centre line, AoA 4 deg, extruded over z (512x256x256: the
+    benchmark wing)."""
+    return geometry.naca_wing_grid(nx, ny, nz, m=0.0, p=0.0, t=0.12, chord=nx / 4.0, le_x=nx / 4.0, le_y=ny / 2.0,
+                                   aoa_deg=4.0)
+
+
+# A control run for chaos: FP64 with the inlet speed raised by this fraction (a change the
+# size of FP32 rounding, acting at every step; the coefficients stay normalised by the
+# nominal speed, which moves them by 2e-6).  Where the flow is chaotic this run leaves the
+# unperturbed FP64 run as far as the storage formats do; where it is not, it stays with it.
+# (Noise added to the rest state alone is not enough: it leaves as acoustic waves before
+# the wake becomes unsteady.)
+PERTURBATION = 1e-6
+
+
 def precision_study(steps=5000, every=50, window=1000, models=("D3Q19", "D3Q15"), storages=("f64", "f32", "f16", "bf16"),
-                    layout=Layout.PINGPONG, log=print, tau=0.5005, tau15=TAU_D3Q15):
+                    layout=Layout.PINGPONG, log=print, tau=0.5005, tau15=TAU_D3Q15, dims=(512, 256, 256)):
+    """Cd/Cl per storage against FP64 on the wing (wing_grid(*dims)), from the uniform rest
+    start.  Storage names: f64, f32, f16, bf16, f16sr, bf16sr, and f64p — the FP64 chaos
+    control with the inlet speed raised by PERTURBATION."""
     import torch
-    grid = c2_grid()
-    u, chord, nz = 0.08, 128.0, 256
+    nx, ny, nz = dims
+    grid = wing_grid(nx, ny, nz)
+    u, chord = 0.08, nx / 4.0
     q = 0.5 * u * u * chord * nz
-    st_map = {"f64": Storage.F64, "f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16}
+    st_map = {"f64": Storage.F64, "f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16,
+              "f16sr": Storage.F16_SR, "bf16sr": Storage.BF16_SR, "f64p": Storage.F64}
     w = max(1, window // every)
-    out = {"workload": f"512x256x256 NACA 0012 chord 128 AoA 4 deg, Cs 0.1, u 0.08, free-slip y/z; tau {tau} "
-                       f"(D3Q15: {tau15})",
+    out = {"workload": f"{nx}x{ny}x{nz} NACA 0012 chord {chord:g} AoA 4 deg, Cs 0.1, u 0.08, free-slip y/z; "
+                       f"tau {tau} (D3Q15: {tau15})",
            "steps": steps, "every": every, "window": window, "area": chord * nz, "layout": layout.name.lower(),
-           "models": {}}
+           "perturbation": PERTURBATION, "models": {}}
     for mname in models:
         model = LatticeModel[mname]
         t_m = tau15 if model == LatticeModel.D3Q15 else tau
@@ -100,7 +123,12 @@ def precision_study(steps=5000, every=50, window=1000, models=("D3Q19", "D3Q15")
         res = {"tau": t_m}
         for sname in storages:
             s = _solver(grid, model, st_map[sname], layout)
-            s.Reset(params, False)
+            if sname == "f64p":
+                pp = Params(**vars(params))
+                pp.inletVelocity = params.inletVelocity * (1.0 + PERTURBATION)
+                s.Reset(pp, False)
+            else:
+                s.Reset(params, False)
             t0 = time.perf_counter()
             f = force_series(s, steps, every)
             dt = time.perf_counter() - t0
@@ -133,7 +161,8 @@ def large_lattice(steps=200, warmup=50, storages=("f32", "f16"), layout=Layout.I
     grid = c3_grid()
     log(f"  geometry {time.time() - t0:.1f}s, solid cells {int(grid.solid.sum())}")
     params = Params(tau=0.501, inletVelocity=0.06, smagorinskyCs=0.1)
-    st_map = {"f32": (Storage.F32, 4), "f16": (Storage.F16, 2), "bf16": (Storage.BF16, 2)}
+    st_map = {"f32": (Storage.F32, 4), "f16": (Storage.F16, 2), "bf16": (Storage.BF16, 2),
+              "f16sr": (Storage.F16_SR, 2), "bf16sr": (Storage.BF16_SR, 2)}
     out = {"workload": "768x512x576 (226M cells) ellipsoid (96,40,40) + 64 spheres r 4-12 seed 0, D3Q19 Cs 0.1, "
                        "u 0.06, tau 0.501, free-slip y/z", "steps": steps, "warmup": warmup,
            "layout": layout.name.lower(), "runs": {}}
@@ -171,7 +200,8 @@ def density_drain(steps=20000, every=1000, model="D3Q15", tau=TAU_D3Q15, storage
     outlet in each population format."""
     grid = c2_grid()
     q = 0.5 * 0.08 * 0.08 * 128.0 * 256
-    st_map = {"f64": Storage.F64, "f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16}
+    st_map = {"f64": Storage.F64, "f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16,
+              "f16sr": Storage.F16_SR, "bf16sr": Storage.BF16_SR}
     out = {"workload": f"512x256x256 NACA 0012 chord 128 AoA 4 deg, {model} tau {tau}, Cs 0.1, u 0.08, free-slip y/z",
            "every": every, "runs": {}}
     for sname in storages:
@@ -200,6 +230,7 @@ def main(argv=None):
     p.add_argument("--storages", nargs="+", default=["f64", "f32", "f16", "bf16"])
     p.add_argument("--tau", type=float, default=0.5005)
     p.add_argument("--tau15", type=float, default=TAU_D3Q15)
+    p.add_argument("--dims", type=int, nargs=3, default=[512, 256, 256])
     p.add_argument("--out", default=None)
     g = sub.add_parser("large")
     g.add_argument("--steps", type=int, default=200)
@@ -221,7 +252,7 @@ def main(argv=None):
 
     if a.cmd == "precision":
         r = precision_study(a.steps, a.every, a.window, tuple(a.models), tuple(a.storages), log=log, tau=a.tau,
-                            tau15=a.tau15)
+                            tau15=a.tau15, dims=tuple(a.dims))
     elif a.cmd == "drain":
         r = density_drain(a.steps, a.every, storages=tuple(a.storages), log=log)
     else:

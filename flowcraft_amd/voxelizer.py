@@ -76,7 +76,7 @@ class Voxelizer:
         if nx * ny * nz:
             torch.cuda.current_stream().synchronize()  # the buffers exist before the library writes them
             rc = lib.fc_voxelize_device(pos.ctypes.data, len(pos), idx.ctypes.data, len(idx),
-                                        ctypes.byref(params._c()), solid.data_ptr(), sdf.data_ptr())
+                                        ctypes.byref(params._c()), solid.data_ptr(), sdf.data_ptr(), solid.numel())
             if rc != L.FC_LBM_OK:
                 raise L.FcLbmError(rc, lib.fc_voxelize_last_error().decode())
         return (nx, ny, nz), cs, origin, solid, sdf

@@ -36,11 +36,17 @@ extern "C" {
 
 /* Population storage.  F32 mirrors LbmReference's float populations; F16/BF16 store
  * g = (f - w_i) * S (S = 4096 for F16, 1 for BF16) and run the collision in FP32;
- * F64 runs the whole step in double (the GPU twin of the FP64 oracle). */
+ * F64 runs the whole step in double (the GPU twin of the FP64 oracle).  The _SR
+ * variants store the same formats but round the step's stores stochastically (with
+ * deterministic random bits per cell, step and direction) instead of to nearest, so
+ * that changes smaller than the format's resolution survive on average; seeding,
+ * restore and readback are as in the round-to-nearest format. */
 #define FC_LBM_STORE_F32 0
 #define FC_LBM_STORE_F16 1
 #define FC_LBM_STORE_BF16 2
 #define FC_LBM_STORE_F64 3
+#define FC_LBM_STORE_F16_SR 4
+#define FC_LBM_STORE_BF16_SR 5
 
 /* Population layout: two SoA copies (pull, ping-pong) or one SoA copy updated in
  * place (esoteric pull). */
@@ -102,8 +108,17 @@ void fc_lbm_params_default(fc_lbm_params* p);
 int fc_lbm_set_geometry(fc_lbm* h, const uint8_t* solid, const float* sdf);
 
 /* Extension: the same from device buffers (N mask bytes, N SDF floats or NULL), e.g. the
- * outputs of fc_voxelize_device; the mask is also copied back for the host-side uses. */
+ * outputs of fc_voxelize_device.  The buffers must be complete when the call is made: work
+ * that writes them on another stream must have been synchronised (the call orders itself
+ * only after the handle's own stream).  The mask stays on the device; a host copy is made
+ * only if a host-side use asks for it (fc_lbm_solid_mask, fc_lbm_aero_forces). */
 int fc_lbm_set_geometry_device(fc_lbm* h, const uint8_t* solid_device, const float* sdf_device);
+
+/* Replaces LbmSolver::SolidMask (FlowCraft/LbmSolver.h:216-218): the geometry's solid mask
+ * (N bytes, 1 = solid) into a host array, copied from the library's own device copy (so a
+ * caller's later reuse of the buffers given to fc_lbm_set_geometry[_device] does not
+ * change it).  Blocking. */
+int fc_lbm_solid_mask(fc_lbm* h, uint8_t* out);
 
 /* Replaces LbmSolver::SetParams (FlowCraft/LbmSolver.h:135): state untouched. */
 int fc_lbm_set_params(fc_lbm* h, const fc_lbm_params* p);
@@ -253,10 +268,12 @@ int fc_voxelize_frame(const float* positions, long long nverts, const uint32_t* 
 int fc_voxelize(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
                 const fc_vox_params* params, uint8_t* solid, float* sdf);
 
-/* Extension: the same into device buffers (nx*ny*nz bytes and floats of the frame), for
- * fc_lbm_set_geometry_device: mesh to running solver without the grid crossing PCIe. */
+/* Extension: the same into device buffers of `capacity` cells each (capacity bytes of mask,
+ * capacity floats of distance), for fc_lbm_set_geometry_device: mesh to running solver
+ * without the grid crossing PCIe.  FC_LBM_EINVAL, before any device work, when capacity
+ * is below the frame's nx*ny*nz.  Blocking: the buffers are complete on return. */
 int fc_voxelize_device(const float* positions, long long nverts, const uint32_t* indices, long long nindices,
-                       const fc_vox_params* params, uint8_t* solid_device, float* sdf_device);
+                       const fc_vox_params* params, uint8_t* solid_device, float* sdf_device, long long capacity);
 
 /* Message of the calling thread's last voxelizer error. */
 const char* fc_voxelize_last_error(void);

@@ -28,7 +28,7 @@ _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_u8_p = ctypes.POINTER(ctypes.c_uint8)
 _c_u16_p = ctypes.POINTER(ctypes.c_uint16)
 
-STORAGE_NATIVE, STORAGE_F16, STORAGE_BF16 = 0, 1, 2
+STORAGE_NATIVE, STORAGE_F16, STORAGE_BF16, STORAGE_F16_SR, STORAGE_BF16_SR = 0, 1, 2, 3, 4
 STATS_FIELDS = ("avgDensity", "minDensity", "maxDensity", "avgSpeed", "maxSpeed", "forceX", "forceY",
                 "forceZ", "maxStress", "totalMass", "hasNaN")
 
@@ -83,6 +83,12 @@ def lib():
         L.orc_f32_to_f16_array.argtypes = [_c_float_p, _c_u16_p, ctypes.c_size_t]
         L.orc_f32_to_bf16_array.argtypes = [_c_float_p, _c_u16_p, ctypes.c_size_t]
         L.orc_f16_scale.restype = ctypes.c_float
+        L.orc_sr_hash.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_sr_hash.restype = ctypes.c_uint32
+        L.orc_f32_to_f16_sr.argtypes = [ctypes.c_float, ctypes.c_uint32]
+        L.orc_f32_to_f16_sr.restype = ctypes.c_uint16
+        L.orc_f32_to_bf16_sr.argtypes = [ctypes.c_float, ctypes.c_uint32]
+        L.orc_f32_to_bf16_sr.restype = ctypes.c_uint16
         L.orc_aero_forces.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_float_p, _c_u8_p,
                                       _c_float_p, _c_float_p, ctypes.POINTER(ctypes.c_int)]
         L.orc_vox_frame.argtypes = [_c_float_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, _c_float_p,

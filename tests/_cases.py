@@ -7,7 +7,8 @@ import numpy as np
 from flowcraft_amd import geometry
 from flowcraft_amd.lbm import LatticeModel, Layout, LbmSolver, Params, Storage, VoxelGrid
 
-STORAGE_TO_ORACLE = {Storage.F32: (32, 0), Storage.F16: (32, 1), Storage.BF16: (32, 2), Storage.F64: (64, 0)}
+STORAGE_TO_ORACLE = {Storage.F32: (32, 0), Storage.F16: (32, 1), Storage.BF16: (32, 2), Storage.F64: (64, 0),
+                     Storage.F16_SR: (32, 3), Storage.BF16_SR: (32, 4)}
 
 
 def case_a_grid() -> VoxelGrid:

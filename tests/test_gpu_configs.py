@@ -7,9 +7,10 @@ oracle finishes in seconds:
 
 * the 512x256x256 NACA 0012 wing (chord 128, AoA 4 deg, u 0.08, free-slip y/z): D3Q19
   in FP32 (tau 0.5005, Cs 0.1) against the FP32 oracle, which is bit-exact with
-  FlowCraft's LbmReference; FP16 and BF16 offset storage against the oracle's emulation
-  of the same format; D3Q15 in FP32 and FP16 at tau 0.6 (FlowCraft's own LbmReference
-  diverges with D3Q15 at this inlet for tau <= 0.55);
+  FlowCraft's LbmReference; FP16 and BF16 offset storage (and FP16 with stochastically
+  rounded stores) against the oracle's emulation of the same format; D3Q15 in FP32, FP16
+  and stochastically rounded FP16 at tau 0.6 (FlowCraft's own LbmReference diverges with
+  D3Q15 at this inlet for tau <= 0.55);
 * the 768x512x576 ellipsoid + 64 spheres lattice (226M cells, u 0.06, tau 0.501) in
   FP16 storage against the oracle's emulation;
 * in place == ping-pong, bit for bit, for the variants not covered elsewhere.
@@ -103,12 +104,12 @@ def _compare_with_oracle(grid, model, storage, params, steps, layout=Layout.PING
         assert abs(forces[k] - st[name]) <= 1e-9 * max(abs(st["forceX"]), 1e-12), (name, forces[k], st[name])
 
 
-@pytest.mark.parametrize("storage", [Storage.F32, Storage.F16, Storage.BF16], ids=lambda s: s.name)
+@pytest.mark.parametrize("storage", [Storage.F32, Storage.F16, Storage.BF16, Storage.F16_SR], ids=lambda s: s.name)
 def test_c2_wing_d3q19_matches_oracle(storage):
     _compare_with_oracle(_c2_wing(), LatticeModel.D3Q19, storage, C2_PARAMS, steps=20)
 
 
-@pytest.mark.parametrize("storage", [Storage.F32, Storage.F16], ids=lambda s: s.name)
+@pytest.mark.parametrize("storage", [Storage.F32, Storage.F16, Storage.F16_SR], ids=lambda s: s.name)
 def test_c2_wing_d3q15_matches_oracle(storage):
     _compare_with_oracle(_c2_wing(), LatticeModel.D3Q15, storage, C2_D3Q15_PARAMS, steps=20)
 

@@ -118,7 +118,7 @@ def test_ragged_lattices_bitexact(name, layout):
     grid = build()
     p = ragged_params(fsy, fsz)
     for model in MODELS:
-        for storage in (Storage.F32, Storage.F16):
+        for storage in (Storage.F32, Storage.F16, Storage.F16_SR, Storage.BF16_SR):
             eng = make_engine(grid, model, storage, layout)
             eng.Reset(p, True)
             orc = make_oracle(grid, model, storage)
@@ -134,15 +134,18 @@ def test_ragged_lattices_bitexact(name, layout):
 
 
 @pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: l.name)
-@pytest.mark.parametrize("storage", [Storage.F16, Storage.BF16], ids=lambda s: s.name)
+@pytest.mark.parametrize("storage", [Storage.F16, Storage.BF16, Storage.F16_SR, Storage.BF16_SR],
+                         ids=lambda s: s.name)
 @pytest.mark.parametrize("model", MODELS, ids=lambda m: m.name)
 def test_reduced_storage_matches_emulation(model, storage, layout):
-    """FP16/BF16 offset storage vs the oracle's exact emulation of the same format."""
+    """FP16/BF16 offset storage, rounded to nearest or stochastically, vs the oracle's
+    exact emulation of the same format: identical fields."""
     grid = case_a_grid()
     eng, orc = _run_pair(grid, case_a_params(), model, storage, layout, steps=40)
     g, o = eng.ReadbackField(), orc.field()
-    _fields_close(g, o, 1e-5, grid.solid)
     print(f"max ulp vs emulation {ulp_diff32(g, o)}")
+    _fields_close(g, o, 1e-5, grid.solid)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
 
 
 @pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: l.name)
@@ -171,7 +174,8 @@ def test_periodic_yz(layout):
         _fields_close(eng.ReadbackField(), orc.field(), 1e-5, grid.solid)
 
 
-@pytest.mark.parametrize("storage", [Storage.F32, Storage.F16], ids=lambda s: s.name)
+@pytest.mark.parametrize("storage", [Storage.F32, Storage.F16, Storage.F16_SR, Storage.BF16_SR],
+                         ids=lambda s: s.name)
 @pytest.mark.parametrize("model", MODELS, ids=lambda m: m.name)
 def test_inplace_equals_pingpong_bitwise(model, storage):
     grid = wing_grid()

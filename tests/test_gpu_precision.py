@@ -1,0 +1,48 @@
+"""The half-memory storage formats against the FP64 engine, in drag and lift.
+
+Workload: the benchmark wing scaled to 256x128x128 (NACA 0012, chord 64, leading edge at
+x = 64, angle of attack 4 deg, extruded over z; Cs 0.1, u 0.08, free-slip y/z walls), tau
+0.6, run from the uniform rest start for 20000 steps.  At tau 0.6 the flow is not chaotic:
+FP32 stays within 0.05% of FP64, so a storage format's distance from FP64 is its own.
+Cd = Fx / (0.5 u^2 A), Cl = Fy / (0.5 u^2 A), A = chord * nz, F the FP64 momentum-exchange
+sum (LbmReference::SolidCellForce convention, FlowCraft/LbmReference.cpp:127-148), as the
+mean over the trailing 10000 steps (sampled every 50).
+
+Bar: FP16 and BF16 offsets with stochastically rounded stores within 0.5% of FP64 in Cd
+and Cl (the precision target); FP32 as well.  The same formats rounded to nearest are
+reported and must be the worse ones: their stores lose the outlet's slow density drain
+(about one FP16 unit per step), which stochastic rounding keeps on average.
+"""
+from __future__ import annotations
+
+import pytest
+
+from flowcraft_amd.lbm import Layout
+from flowcraft_amd.studies import precision_study
+
+pytestmark = pytest.mark.gpu
+
+BAR = 0.005
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu-case] {request.node.nodeid}", flush=True)
+    yield
+
+
+@pytest.mark.parametrize("model", ["D3Q19", "D3Q15"])
+def test_stochastic_rounding_meets_the_drag_lift_bar(model):
+    r = precision_study(steps=20000, every=50, window=10000, models=(model,),
+                        storages=("f64", "f32", "f16", "f16sr", "bf16", "bf16sr"), layout=Layout.PINGPONG,
+                        log=lambda m: print(m, flush=True), tau=0.6, tau15=0.6, dims=(256, 128, 128))
+    res = r["models"][model]
+    err = {st: max(res[st]["cd_mean_relerr_vs_f64"], res[st]["cl_mean_relerr_vs_f64"])
+           for st in ("f32", "f16", "f16sr", "bf16", "bf16sr")}
+    print({k: f"{100 * v:.3f}%" for k, v in err.items()}, flush=True)
+    for st in ("f64", "f32", "f16", "f16sr", "bf16", "bf16sr"):
+        assert not res[st]["diverged"], st
+    assert err["f32"] <= BAR
+    assert err["f16sr"] <= BAR
+    assert err["bf16sr"] <= BAR
+    assert err["f16"] > err["f16sr"] and err["bf16"] > err["bf16sr"]

@@ -82,10 +82,20 @@ def test_mesh_to_solver_on_the_device():
     a, b = LbmSolver(), LbmSolver()
     assert a.Init(None, host, LatticeModel.D3Q19) and b.Init(None, host, LatticeModel.D3Q19)
     b.SetGeometryDevice(dsolid, dsdf)
+    # the solver keeps its own copy: reusing the caller's buffers changes neither the mask
+    # SolidMask() reports nor the run
+    dsolid.fill_(1)
+    dsdf.fill_(-1.0)
+    torch.cuda.synchronize()
     assert np.array_equal(b.SolidMask(), host.solid.reshape(-1))
     for e in (a, b):
         e.Reset(p, True)
         e.Step(None, 20)
     assert np.array_equal(a.ReadbackField().view(np.uint32), b.ReadbackField().view(np.uint32))
+    # and the aerodynamic summary (which reads a host copy of the mask made on first use)
+    # agrees between the two set-ups
+    fa, fb = a.ComputeAerodynamicForces(cs, org), b.ComputeAerodynamicForces(cs, org)
+    assert fa == fb
     a.Shutdown()
     b.Shutdown()
+

@@ -316,3 +316,90 @@ def test_f16_storage_flush_matches_device_conversion():
     normal = np.abs(v) >= 2.0 ** -14
     assert np.array_equal(f[normal], plain[normal])
     assert not np.any(((f & 0x7C00) == 0) & ((f & 0x3FF) != 0))
+
+
+# ------------------------------------------------------------- stochastic rounding
+def _f16_bits_to_f64(b):
+    return np.asarray(b, np.uint16).view(np.float16).astype(np.float64)
+
+
+def test_f16_sr_is_truncation_at_zero_bits_and_never_subnormal():
+    """With random bits 0 the FP16 stochastic rounding is truncation toward zero: the
+    stored half is the largest half not above |v| in magnitude (checked against numpy's
+    halves), and no result is a subnormal half."""
+    import oracle
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    v = (rng.standard_normal(20000) * 10.0 ** rng.uniform(-6, 4, 20000)).astype(np.float32)
+    v = v[np.abs(v) >= 2.0 ** -14]
+    h = np.array([L.orc_f32_to_f16_sr(float(x), 0) for x in v], np.uint16)
+    d = _f16_bits_to_f64(h)
+    assert np.all(np.abs(d) <= np.abs(v.astype(np.float64)))
+    up = (h.astype(np.uint32) + 1).astype(np.uint16)  # next half away from zero
+    assert np.all(np.abs(_f16_bits_to_f64(up)) > np.abs(v.astype(np.float64)))
+    assert np.all(np.sign(d[d != 0]) == np.sign(v[d != 0]))
+    rnd = rng.integers(0, 2 ** 32, 5000, dtype=np.uint64)
+    w = (rng.standard_normal(5000) * 10.0 ** rng.uniform(-12, 3, 5000)).astype(np.float32)
+    hb = np.array([L.orc_f32_to_f16_sr(float(x), int(r)) for x, r in zip(w, rnd)], np.uint16)
+    assert not np.any(((hb & 0x7C00) == 0) & ((hb & 0x3FF) != 0))
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_sr_is_exactly_unbiased_over_its_random_bits(fmt):
+    """Averaged over every value of the random bits it uses (2^13 for FP16, 2^16 for BF16),
+    the stochastically rounded value equals the input exactly: the expectation property
+    that lets slow drifts survive.  Dropping or shifting the added bits breaks it."""
+    import oracle
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    vals = (rng.standard_normal(12) * 10.0 ** rng.uniform(-3, 3, 12)).astype(np.float32)
+    vals = np.concatenate([vals, np.float32([1.0, -1.0, 2.0 ** -14 * 1.5, 65000.0, 0.333333])])
+    if fmt == "f16":
+        ks = np.arange(2 ** 13, dtype=np.uint64) << 19
+        conv, dec = L.orc_f32_to_f16_sr, _f16_bits_to_f64
+    else:
+        ks = np.arange(2 ** 16, dtype=np.uint64) << 16
+        conv = L.orc_f32_to_bf16_sr
+        dec = lambda b: (np.asarray(b, np.uint32) << 16).view(np.float32).astype(np.float64)
+    for v in vals:
+        out = np.array([conv(float(v), int(k)) for k in ks], np.uint16)
+        assert len(set(out.tolist())) <= 2  # the two neighbours of v in the format
+        assert dec(out).mean() == float(v), (fmt, v)
+
+
+def test_sr_flushes_below_the_smallest_normal_half():
+    """FP16 stochastic rounding with half denormals flushed: a sum below 2^-14 becomes a
+    zero of its sign; 2^-14 itself is kept."""
+    import oracle
+    L = oracle.lib()
+    t = np.float32(2.0 ** -14)
+    below = np.nextafter(t, np.float32(0))
+    assert L.orc_f32_to_f16_sr(float(below), 0) == 0x0000
+    assert L.orc_f32_to_f16_sr(float(-below), 0) == 0x8000
+    assert L.orc_f32_to_f16_sr(float(t), 0) == 0x0400
+    # a value within 2^-13 ulp of 2^-14 reaches it for large enough random bits
+    assert L.orc_f32_to_f16_sr(float(below), 0xFFFFFFFF) == 0x0400
+    assert L.orc_f32_to_f16_sr(0.0, 0xFFFFFFFF) == 0x0000
+    assert L.orc_f32_to_f16_sr(-0.0, 0xFFFFFFFF) == 0x8000
+
+
+def test_sr_hash_bits_are_uniform_and_decorrelated():
+    """The random stream of the SR formats: the 13 bits FP16 uses from each direction's
+    word are uniform over a block of cells (chi-square), and a cell's bits at consecutive
+    steps and neighbouring cells' bits at one step are uncorrelated."""
+    import oracle
+    L = oracle.lib()
+    cells = np.arange(0, 40000, dtype=np.uint32) * np.uint32(7) + np.uint32(12345)
+    for step in (0, 1, 977):
+        h = np.array([L.orc_sr_hash(int(c), step) for c in cells], np.uint64)
+        for d in (0, 5, 18):
+            bits = ((h + np.uint64(d) * np.uint64(0x9E3779B9)) & np.uint64(0xFFFFFFFF)) >> np.uint64(19)
+            counts = np.bincount((bits >> np.uint64(7)).astype(np.int64), minlength=64)  # 64 bins
+            e = len(bits) / 64
+            chi2 = float(((counts - e) ** 2 / e).sum())
+            assert chi2 < 120, (step, d, chi2)  # 63 degrees of freedom: p(chi2 > 120) < 1e-5
+    a = np.array([L.orc_sr_hash(int(c), 10) for c in cells[:20000]], np.float64)
+    b = np.array([L.orc_sr_hash(int(c), 11) for c in cells[:20000]], np.float64)
+    n = np.array([L.orc_sr_hash(int(c) + 1, 10) for c in cells[:20000]], np.float64)
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.03
+    assert abs(np.corrcoef(a, n)[0, 1]) < 0.03
