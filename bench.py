@@ -41,13 +41,15 @@ and lift coefficients (trailing 1000-step mean and last step) and the field.  At
 `error_vs_fp64_tau06` is the same wing at tau 0.6, not chaotic, over 20000 steps with
 trailing 10000-step means: there the 0.5% precision bar is a property of the format.
 
-`warm_start` is the metric's second half, measured live: on --warm-shapes held-out
+`warm_start` is the metric's second half, measured live: on --warm-shapes (64) held-out
 shapes of the warm-start distribution (256x128x128, u = 0.06, tau = 0.503, FP16
-storage), steps and wall time until the trailing-mean drag coefficient is within 1%
-of the plain start's value at 6000 steps, for the plain start, the learned warm start
-(committed surrogate weights, inference included) and FlowCraft's coarse-grid warm start.
-`warm_start_settled` is the same study against the plain start's value at 24000 steps,
-where its trailing mean has settled, with the weights trained on 15000-step fields.
+storage with stochastic rounding), steps and wall time until the trailing-mean drag
+coefficient is within 1% of a plain start's value at 6000 steps, for FlowCraft's two plain
+starts (rest; half the inlet speed, its default), the learned warm start (committed
+surrogate weights, inference included) and FlowCraft's coarse-grid warm start.
+`warm_start_settled` is the same study (--warm-shapes-settled, 16 shapes) against the
+plain starts' values at 24000 steps, where their trailing means have settled, with the
+weights trained on 15000-step fields.
 """
 from __future__ import annotations
 
@@ -410,12 +412,15 @@ def voxelize_block(cpu=True, reps=3, detail=2, res=640):
 
 def warm_start_block(torch, n_shapes, settled=False):
     """Live warm-vs-plain comparison on held-out shapes of the warm-start distribution
-    (256x128x128 D3Q19 Smagorinsky Cs = 0.1, u = 0.06, tau = 0.503, FP16 storage) with
-    committed surrogate weights: steps and wall time until the trailing-1000-step mean Cd
-    is within 1% of the plain start's value at the horizon, under the 'reach', 'hold'
-    (1000 steps) and 'horizon' (stays to the end) rules.  Default: the 6000-step horizon
-    and the weights trained on 3000-step fields.  settled=True: the 24000-step horizon,
-    where the plain run's Cd has settled, and the weights trained on 15000-step fields."""
+    (256x128x128 D3Q19 Smagorinsky Cs = 0.1, u = 0.06, tau = 0.503, FP16 storage with
+    stochastically rounded stores) with committed surrogate weights.  Four starts per
+    shape: FlowCraft's two plain seeds (at rest; at half the inlet speed, its default),
+    the learned warm start and FlowCraft's coarse-grid warm start (ApplyWarmStart).  Steps
+    and wall time until the trailing-1000-step mean Cd is within 1% of a plain start's
+    value at the horizon, under the 'reach', 'hold' (1000 steps) and 'horizon' (stays to
+    the end) rules, scored against each plain start.  Default: the 6000-step horizon and
+    the weights trained on 3000-step fields; settled=True: the 24000-step horizon, where
+    the plain runs' Cd has settled, and the weights trained on 15000-step fields."""
     from flowcraft_amd import surrogate as sg
     from flowcraft_amd import warmstart as ws
     net = sg.Surrogate(seed=0)
@@ -427,21 +432,27 @@ def warm_start_block(torch, n_shapes, settled=False):
     summ = ws.summarize(recs, horizon)
     trained = ("320 engine-generated fields (15000 steps, mean of the last 1000)" if settled else
                "600 engine-generated fields (3000 steps, mean of the last 1000)")
-    out = {"workload": "256x128x128 D3Q19 Cs 0.1, u 0.06, tau 0.503, FP16 storage; held-out shapes seed 1001; "
-                       f"surrogate trained on {trained}; horizon {horizon} steps",
+    out = {"workload": "256x128x128 D3Q19 Cs 0.1, u 0.06, tau 0.503, FP16 storage, stochastic rounding; held-out "
+                       f"shapes seed 1001; surrogate trained on {trained}; horizon {horizon} steps",
            "shapes": len(recs), "wall_s": round(time.time() - t0, 1),
-           "infer_ms_mean": round(float(np.mean([r["infer_ms"] for r in recs])), 2)}
-    for rule in ws.RULES:
-        o = summ[rule]
-        if "plain_steps" not in o:
-            out[rule] = o
-            continue
-        out[rule] = {"plain_steps_p50": o["plain_steps"]["p50"], "warm_steps_p50": o["warm_steps"]["p50"],
-                     "coarse_steps_p50": o["coarse_steps"]["p50"], "speedup_wall_p50": round(o["speedup_wall"]["p50"], 3),
-                     "speedup_wall_mean": round(o["speedup_wall"]["mean"], 3),
-                     "speedup_wall_of_means": round(o["speedup_wall_of_means"], 3),
-                     "coarse_speedup_wall_of_means": round(o["coarse_speedup_wall_of_means"], 3),
-                     "warm_fallbacks": o["warm_fallbacks"], "coarse_fallbacks": o["coarse_fallbacks"]}
+           "infer_ms_mean": round(float(np.mean([r["infer_ms"] for r in recs])), 2),
+           "coarse_ms_mean": round(float(np.mean([r["coarse_ms"] for r in recs])), 1)}
+    for base in ws.PLAIN_ARMS:
+        ob = {}
+        for rule in ws.RULES:
+            o = summ[base][rule]
+            if "plain_steps" not in o:
+                ob[rule] = o
+                continue
+            row = {"plain_steps_p50": o["plain_steps"]["p50"], "plain_unconverged": o["plain_unconverged"]}
+            for a in ws.SEEDED_ARMS:
+                row[a] = {"steps_p50": o[a]["steps"]["p50"], "speedup_wall_p50": round(o[a]["speedup_wall"]["p50"], 3),
+                          "speedup_wall_mean": round(o[a]["speedup_wall"]["mean"], 3),
+                          "speedup_wall_of_means": round(o[a]["speedup_wall_of_means"], 3),
+                          "fallbacks": o[a]["fallbacks"]}
+            ob[rule] = row
+        out["vs_" + base] = ob
+    out["final_cd_rel_diff_p50"] = {k: round(v["p50"], 4) for k, v in summ["final_cd_rel_diff"].items()}
     return out
 
 
@@ -463,8 +474,10 @@ def main():
     ap.add_argument("--layout", default="pingpong", choices=["inplace", "pingpong"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--warm-shapes", type=int, default=8,
-                    help="held-out shapes of the live warm-vs-plain comparison (0 skips it)")
+    ap.add_argument("--warm-shapes", type=int, default=64,
+                    help="held-out shapes of the live warm-vs-plain comparison, 6000-step horizon (0 skips it)")
+    ap.add_argument("--warm-shapes-settled", type=int, default=16,
+                    help="held-out shapes of the same comparison at the 24000-step horizon (0 skips it)")
     ap.add_argument("--profile", action="store_true",
                     help="only the FP32 D3Q19 step kernel for --steps steps (for rocprofv3 runs); "
                          "prints the HIP-event average step time")
@@ -576,7 +589,8 @@ def main():
     fp64_judged = None if args.no_variants else fp64_error_block(
         torch, grid, steps=20000, window=10000, tau=0.6, storages=("F32", "F16", "F16_SR", "BF16_SR"))
     warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
-    warm_settled = warm_start_block(torch, args.warm_shapes, settled=True) if args.warm_shapes > 0 else None
+    warm_settled = (warm_start_block(torch, args.warm_shapes_settled, settled=True)
+                    if args.warm_shapes_settled > 0 else None)
     cpu = None if args.no_cpu else cpu_baseline(grid)
     line = {
         "metric": METRIC, "value": round(main_r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1,

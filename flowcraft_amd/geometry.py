@@ -150,17 +150,24 @@ def _polygon_sdf(px: np.ndarray, py: np.ndarray, poly: np.ndarray) -> np.ndarray
     return np.where(inside, -d, d)
 
 
-def add_naca4_extruded(g: VoxelGrid, m: float, p: float, t: float, chord: float, le_x: float, le_y: float,
-                       aoa_deg: float, z_range=None) -> None:
-    """NACA mpt wing, chord ``chord`` cells, leading edge at (le_x, le_y), rotated by
-    the angle of attack about the leading edge (trailing edge down, lift toward +y
-    for flow in +x), extruded over z (all of it by default)."""
+def _naca_placed(m, p, t, chord, le_x, le_y, aoa_deg) -> np.ndarray:
+    """The NACA outline in grid x-y: chord `chord`, leading edge at (le_x, le_y), rotated
+    by the angle of attack about the leading edge (trailing edge down)."""
     poly = naca4_outline(m, p, t) * chord
     a = math.radians(aoa_deg)
     ca, sa = math.cos(a), math.sin(a)
     wx = le_x + poly[:, 0] * ca + poly[:, 1] * sa
     wy = le_y - poly[:, 0] * sa + poly[:, 1] * ca
-    wpoly = np.stack([wx, wy], axis=1)
+    return np.stack([wx, wy], axis=1)
+
+
+def add_naca4_extruded(g: VoxelGrid, m: float, p: float, t: float, chord: float, le_x: float, le_y: float,
+                       aoa_deg: float, z_range=None) -> None:
+    """NACA mpt wing, chord ``chord`` cells, leading edge at (le_x, le_y), rotated by
+    the angle of attack about the leading edge (trailing edge down, lift toward +y
+    for flow in +x), extruded over z (all of it by default)."""
+    wpoly = _naca_placed(m, p, t, chord, le_x, le_y, aoa_deg)
+    wx, wy = wpoly[:, 0], wpoly[:, 1]
     x0, x1 = max(0, int(wx.min()) - _MARGIN), min(g.nx, int(math.ceil(wx.max())) + _MARGIN + 1)
     y0, y1 = max(0, int(wy.min()) - _MARGIN), min(g.ny, int(math.ceil(wy.max())) + _MARGIN + 1)
     if x0 >= x1 or y0 >= y1:
@@ -210,10 +217,28 @@ def random_shape_grid(nx, ny, nz, rng: np.random.Generator, chord=64.0):
         m, p, t = rng.uniform(0, 0.06), rng.uniform(0.2, 0.6), rng.uniform(0.08, 0.18)
         aoa = rng.uniform(-4, 12)
         g = naca_wing_grid(nx, ny, nz, m, p, t, chord, le_x=nx * 0.25, le_y=ny / 2.0, aoa_deg=aoa)
-        return g, {"kind": "naca", "m": m, "p": p, "t": t, "aoa": aoa, "area": chord * nz}
+        return g, {"kind": "naca", "m": m, "p": p, "t": t, "aoa": aoa, "chord": chord, "area": chord * nz}
     a, b, c = rng.uniform(8, 32, size=3)
     g = ellipsoid_grid(nx, ny, nz, (nx * 0.25, ny / 2.0, nz / 2.0), (a, b, c))
     return g, {"kind": "ellipsoid", "axes": (a, b, c), "area": math.pi * b * c}
+
+
+def shape_sdf_at(desc: dict, dims, px, py, pz) -> np.ndarray:
+    """Signed distance (float64, grid-cell units) of a random_shape_grid shape at points
+    (px, py, pz) in grid coordinates (cell centres at i + 0.5) of the dims = (nx, ny, nz)
+    lattice it was placed in: the same surface random_shape_grid voxelizes, evaluated
+    anywhere (e.g. at the cell centres of a coarser grid over the same domain)."""
+    nx, ny, nz = dims
+    px, py, pz = (np.asarray(v, np.float64) for v in (px, py, pz))
+    if desc["kind"] == "naca":
+        poly = _naca_placed(desc["m"], desc["p"], desc["t"], desc.get("chord", 64.0), nx * 0.25, ny / 2.0,
+                            desc["aoa"])
+        return _polygon_sdf(px + 0 * pz, py + 0 * pz, poly)
+    a, b, c = desc["axes"]
+    dx, dy, dz = px - nx * 0.25, py - ny / 2.0, pz - nz / 2.0
+    k0 = np.sqrt((dx / a) ** 2 + (dy / b) ** 2 + (dz / c) ** 2)
+    k1 = np.maximum(np.sqrt((dx / (a * a)) ** 2 + (dy / (b * b)) ** 2 + (dz / (c * c)) ** 2), 1e-12)
+    return k0 * (k0 - 1.0) / k1
 
 
 def signed_distance_feature(g: VoxelGrid, clip: float = 16.0) -> np.ndarray:

@@ -6,15 +6,16 @@ ellipsoids; ``geometry.random_shape_grid``) on a D3Q19 Smagorinsky lattice,
 each run from the uniform rest start for a fixed number of steps; the target is
 the final (rho, u) field in normalised units ((rho-1)*3/u^2, u/u_in).
 
-Convergence study: for each held-out shape, a plain run (uniform rest start)
-to ``horizon`` steps sets the reference drag coefficient; a warm run starts from
-the surrogate's field.  Cd(t) is the momentum-exchange drag Fx / (0.5 U^2 A)
-averaged over the trailing ``window`` steps (sampled every ``every`` steps); a run
-has converged at the first sample after which Cd(t) stays within ``tol`` of the
-plain run's final Cd.  A warm run that diverges or never converges falls back to
-the plain start (counted).  The reference's own warm start — a coarse lattice at
-1/``coarse_div`` resolution run ``coarse_steps`` steps and nearest-sampled onto the
-fine grid (FlowCraft/main.cpp:1413-1564) — is the third arm.
+Convergence study: for each held-out shape, four runs of ``horizon`` steps — FlowCraft's
+two uniform seeds (at rest, and at half the inlet speed, its default: LbmSolver::Reset,
+FlowCraft/main.cpp:431), the surrogate's field, and FlowCraft's own coarse-grid warm start
+(ApplyWarmStart: the model at 1/``coarse_div`` resolution, ``coarse_steps`` coarse steps,
+nearest-sampled onto the fine grid; FlowCraft/main.cpp:1413-1564).  Cd(t) is the
+momentum-exchange drag Fx / (0.5 U^2 A) averaged over the trailing ``window`` steps
+(sampled every ``every`` steps).  Each plain run's value at the horizon is a reference;
+every run's steps to convergence to it are counted under three rules (first entry into the
+``tol`` band, staying in it for ``hold`` steps, staying to the end).  A seeded run that
+diverges or never converges falls back to the plain start (counted, charged its time).
 """
 from __future__ import annotations
 
@@ -25,7 +26,7 @@ import time
 import numpy as np
 
 from . import geometry
-from .lbm import LatticeModel, Layout, LbmSolver, Params, Storage
+from .lbm import LatticeModel, Layout, LbmSolver, Params, Storage, VoxelGrid
 from . import surrogate as sg
 
 
@@ -226,34 +227,76 @@ def converged_index(cd_avg, ref, tol, hold=None):
     return None
 
 
-def coarse_seed(desc, dims, div, steps, params, rng_state=None):
-    """The reference's warm start: the same shape at 1/div resolution, `steps` coarse
-    steps, nearest-sampled onto the fine grid with solid coarse cells -> (1,0,0,0)
-    (FlowCraft/main.cpp:1413-1461, 1496-1564)."""
+def flowcraft_coarse_grid(desc, dims, div=10):
+    """The coarse grid of FlowCraft's ApplyWarmStart (main.cpp:1496-1510) for a shape of
+    the study: the same model voxelized at 1/div of the fine grid's resolution over the
+    same padded domain.  Frame as Voxelizer::Voxelize builds it (Voxelizer.cpp:136-170):
+    the fine grid's origin, cell size = longest domain axis / max(resolution/div, 4),
+    nx = ceil(world-Y extent / cell), ny = ceil(world-X extent / cell), nz likewise; the
+    fine grid here is the study's lattice with cell size 1 at the origin.  Classification
+    by the voxelizer's rule: every cell whose centre lies within sqrt(3)/2 cells of the
+    surface is solid (the conservative shell, on both sides of it), so is everything
+    enclosed; the signed distance (world units) is kept for the Bouzidi links.  The
+    surface is the study's analytic one (geometry.shape_sdf_at), not a triangulation of
+    it."""
     nx, ny, nz = dims
-    cx, cy, cz = max(nx // div, 4), max(ny // div, 1), max(nz // div, 1)
-    sc = cx / nx
-    g = geometry.empty_grid(cx, cy, cz)
-    if desc["kind"] == "naca":
-        geometry.add_naca4_extruded(g, desc["m"], desc["p"], desc["t"], 64.0 * sc, nx * 0.25 * sc, ny / 2.0 * sc,
-                                    desc["aoa"])
-    else:
-        a, b, c = desc["axes"]
-        geometry.add_ellipsoid(g, (nx * 0.25 * sc, ny / 2.0 * sc, nz / 2.0 * sc), (a * sc, b * sc, c * sc))
-    g.solid = (g.signedDistance < 0).astype(np.uint8)
-    s = make_solver(g, Storage.F32)
-    s.Reset(params, False)
+    f32 = np.float32
+    res = max(nx, ny, nz)
+    res_c = max(res // max(div, 1), 4)
+    cs = max(f32(res) / f32(res_c), f32(1e-6))
+    nxc = max(int(math.ceil(f32(nx) / cs)), 1)  # grid-x spans world Y
+    nyc = max(int(math.ceil(f32(ny) / cs)), 1)
+    nzc = max(int(math.ceil(f32(nz) / cs)), 1)
+    z, y, x = np.meshgrid((np.arange(nzc, dtype=f32) + f32(0.5)) * cs, (np.arange(nyc, dtype=f32) + f32(0.5)) * cs,
+                          (np.arange(nxc, dtype=f32) + f32(0.5)) * cs, indexing="ij")
+    d = geometry.shape_sdf_at(desc, dims, x, y, z).astype(f32)
+    solid = (d <= cs * f32(0.8660254)).astype(np.uint8)
+    return VoxelGrid(nxc, nyc, nzc, solid, d, cellSize=float(cs), origin=(0.0, 0.0, 0.0))
+
+
+def sample_coarse_field(fine: VoxelGrid, coarse: VoxelGrid, coarse_field) -> np.ndarray:
+    """FlowCraft's SampleCoarseFieldOntoFineGrid (main.cpp:1413-1461): each fine cell takes
+    the field of the coarse cell that contains its centre in the world frame — world centre
+    by VoxelGrid::CellCenter (grid-y along world X, grid-x along world Y, VoxelGrid.h:53-55),
+    coarse indices by the inverse mapping with the same swap, each clamped to the grid;
+    a coarse solid cell gives quiescent fluid (1, 0, 0, 0).  Float32 arithmetic in the
+    reference's order.  Returns 4 floats per fine cell."""
+    f32 = np.float32
+    cs_f, cs_c = f32(fine.cellSize), f32(coarse.cellSize)
+    of, oc = np.asarray(fine.origin, f32), np.asarray(coarse.origin, f32)
+    inv = f32(1.0) / cs_c
+    gx = (np.arange(fine.nx, dtype=f32) + f32(0.5)) * cs_f  # world Y of grid-x
+    gy = (np.arange(fine.ny, dtype=f32) + f32(0.5)) * cs_f  # world X of grid-y
+    gz = (np.arange(fine.nz, dtype=f32) + f32(0.5)) * cs_f
+    # local = (worldCenter - coarse.origin) * invCoarseCellSize, per world axis
+    lx = ((of[0] + gy) - oc[0]) * inv   # world X: from the fine grid-y index
+    ly = ((of[1] + gx) - oc[1]) * inv   # world Y: from the fine grid-x index
+    lz = ((of[2] + gz) - oc[2]) * inv
+    cx = np.clip(np.floor(ly).astype(np.int64), 0, coarse.nx - 1)  # coarse grid-x from world Y
+    cy = np.clip(np.floor(lx).astype(np.int64), 0, coarse.ny - 1)  # coarse grid-y from world X
+    cz = np.clip(np.floor(lz).astype(np.int64), 0, coarse.nz - 1)
+    cell = (cz[:, None, None] * coarse.ny + cy[None, :, None]) * coarse.nx + cx[None, None, :]
+    cf = np.asarray(coarse_field, f32).reshape(-1, 4)
+    seed = cf[cell.reshape(-1)].copy()
+    seed[np.asarray(coarse.solid).reshape(-1)[cell.reshape(-1)] != 0] = np.array([1.0, 0.0, 0.0, 0.0], f32)
+    return seed.reshape(-1)
+
+
+def flowcraft_warm_seed(desc, dims, params, div=10, steps=1000, seed_mid=True):
+    """FlowCraft's ApplyWarmStart (main.cpp:1496-1564) on a shape of the study: the coarse
+    grid (flowcraft_coarse_grid), a coarse solver seeded as FlowCraft's Reset seeds
+    (mid-speed by default, AppState::lbmSeedMidSpeed, main.cpp:431) and stepped `steps`
+    times (warmStartSteps, main.cpp:444), its field read back and nearest-sampled onto the
+    fine grid (sample_coarse_field).  Returns the fine grid's seed field."""
+    nx, ny, nz = dims
+    cg = flowcraft_coarse_grid(desc, dims, div)
+    s = make_solver(cg, Storage.F32)
+    s.Reset(params, seed_mid)
     s.Step(None, steps)
-    cf = s.ReadbackField().reshape(cz, cy, cx, 4)
+    cf = s.ReadbackField()
     s.Shutdown()
-    # nearest coarse cell of each fine cell centre
-    ix = np.clip(np.floor((np.arange(nx) + 0.5) * cx / nx).astype(int), 0, cx - 1)
-    iy = np.clip(np.floor((np.arange(ny) + 0.5) * cy / ny).astype(int), 0, cy - 1)
-    iz = np.clip(np.floor((np.arange(nz) + 0.5) * cz / nz).astype(int), 0, cz - 1)
-    fine = cf[iz][:, iy][:, :, ix].copy()
-    csolid = g.solid[iz][:, iy][:, :, ix].astype(bool)
-    fine[csolid] = np.array([1.0, 0.0, 0.0, 0.0], np.float32)
-    return fine.reshape(-1)
+    fine = VoxelGrid(nx, ny, nz, np.zeros(0, np.uint8), None, cellSize=1.0, origin=(0.0, 0.0, 0.0))
+    return sample_coarse_field(fine, cg, cf)
 
 
 RULES = ("reach", "hold", "horizon")
@@ -275,13 +318,20 @@ def _arm_steps(cd, ref, tol, w, every, hold):
     return out, cda
 
 
+# The study's arms: two plain starts (FlowCraft's uniform seeds: at rest, and at half the
+# inlet speed with a Bernoulli density, its default — LbmSolver::Reset, main.cpp:431), the
+# learned warm start and FlowCraft's coarse-grid warm start.  The plain arms are the
+# baselines: each seeded arm is scored against each of them.
+PLAIN_ARMS = ("plain", "plain_mid")
+SEEDED_ARMS = ("warm", "coarse")
+
+
 def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, cs, coarse_div=10, coarse_steps=1000,
-             storage=Storage.F16, log=print, time_budget_s=None, hold=1000, series_every=50):
-    """The warm-vs-plain study on n_shapes held-out shapes.  Each shape is run from the
-    uniform rest start (plain), from the surrogate's field (warm) and from the
-    reference's coarse-grid warm start (coarse), `horizon` steps each.  The reference
-    value is the plain run's trailing-mean Cd at the horizon.  Returns per-shape records
-    with the steps to convergence under both rules and the trailing-mean Cd series."""
+             storage=Storage.F16_SR, log=print, time_budget_s=None, hold=1000, series_every=50):
+    """The warm-vs-plain study on n_shapes held-out shapes.  Each shape runs `horizon`
+    steps from each arm's start.  For each plain arm, its trailing-mean Cd at the horizon is
+    a reference value, and every arm's steps to convergence to that value are found under
+    the three rules (reach, hold, horizon).  Returns per-shape records."""
     import torch
     nx, ny, nz = dims
     V = nx * ny * nz
@@ -297,15 +347,15 @@ def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, 
         rec = {"kind": d["kind"], "area": area,
                "shape": {k: (list(v) if isinstance(v, tuple) else v) for k, v in d.items() if k not in ("kind", "area")}}
         s = make_solver(g, storage)
-        # plain arm
-        s.Reset(params, False)
-        cd, cl, ms = run_cd_series(s, horizon, every, area, u_in)
-        cda = trailing(cd, w)
-        ref = float(cda[-1])
-        steps_p, _ = _arm_steps(cd, ref, tol, w, every, hold)
-        rec.update(cd_ref=ref, cl_ref=float(trailing(cl, w)[-1]), ms_per_step=ms, plain=steps_p,
-                   plain_series=[float(v) for v in cda[ds - 1::ds]])
-        # learned arm
+        raw, cl_final = {}, {}
+        for arm in PLAIN_ARMS:
+            s.Reset(params, arm == "plain_mid")
+            cd, cl, ms = run_cd_series(s, horizon, every, area, u_in)
+            raw[arm] = cd
+            cl_final[arm] = float(trailing(cl, w)[-1]) if np.all(np.isfinite(cl)) else None
+            if arm == "plain":
+                rec["ms_per_step"] = ms
+        # learned arm: the surrogate's field, inference timed
         solid = torch.from_numpy(np.ascontiguousarray(g.solid.reshape(-1))).cuda()
         sdf = torch.from_numpy(np.ascontiguousarray(g.signedDistance.reshape(-1))).cuda()
         torch.cuda.synchronize()
@@ -314,93 +364,97 @@ def evaluate(net, dims, n_shapes, seed, horizon, every, window, tol, u_in, tau, 
         pred = net.predict(x, 1, (nz, ny, nx))
         seed_field = sg.field_from_prediction(pred, u_in, V)
         torch.cuda.synchronize()
-        infer_ms = (time.perf_counter() - t0) * 1e3
+        rec["infer_ms"] = (time.perf_counter() - t0) * 1e3
         s.SetParams(params)
         s.ResetFromField(seed_field)
-        cdw, clw, _ = run_cd_series(s, horizon, every, area, u_in)
-        steps_w, cdwa = _arm_steps(cdw, ref, tol, w, every, hold)
-        rec.update(infer_ms=infer_ms, warm=steps_w,
-                   warm_cd_final=None if cdwa is None else float(cdwa[-1]),
-                   warm_cl_final=float(trailing(clw, w)[-1]) if np.all(np.isfinite(clw)) else None,
-                   warm_series=None if cdwa is None else [float(v) for v in cdwa[ds - 1::ds]])
-        # the reference's coarse-grid warm start
-        try:
-            t0 = time.perf_counter()
-            cseed = coarse_seed(d, dims, coarse_div, coarse_steps, params)
-            coarse_ms = (time.perf_counter() - t0) * 1e3
-            s.SetParams(params)
-            s.ResetFromField(cseed)
-            cdc, _, _ = run_cd_series(s, horizon, every, area, u_in)
-            steps_c, cdca = _arm_steps(cdc, ref, tol, w, every, hold)
-            rec.update(coarse_ms=coarse_ms, coarse=steps_c,
-                       coarse_series=None if cdca is None else [float(v) for v in cdca[ds - 1::ds]])
-        except Exception as e:  # the comparison arm must not stop the study
-            rec.update(coarse_error=str(e), coarse={r: None for r in RULES})
+        raw["warm"], clw, _ = run_cd_series(s, horizon, every, area, u_in)
+        cl_final["warm"] = float(trailing(clw, w)[-1]) if np.all(np.isfinite(clw)) else None
+        # FlowCraft's coarse-grid warm start, its whole cost timed
+        t0 = time.perf_counter()
+        cseed = flowcraft_warm_seed(d, dims, params, coarse_div, coarse_steps)
+        rec["coarse_ms"] = (time.perf_counter() - t0) * 1e3
+        s.SetParams(params)
+        s.ResetFromField(cseed)
+        raw["coarse"], clc, _ = run_cd_series(s, horizon, every, area, u_in)
+        cl_final["coarse"] = float(trailing(clc, w)[-1]) if np.all(np.isfinite(clc)) else None
         s.Shutdown()
+        series = {arm: (trailing(cd, w) if np.all(np.isfinite(cd)) else None) for arm, cd in raw.items()}
+        rec["cd_final"] = {arm: (None if v is None else float(v[-1])) for arm, v in series.items()}
+        rec["cl_final"] = cl_final
+        rec["series"] = {arm: (None if v is None else [float(c) for c in v[ds - 1::ds]]) for arm, v in series.items()}
+        rec["steps"] = {}
+        for base in PLAIN_ARMS:
+            ref = rec["cd_final"][base]
+            rec["steps"][base] = {arm: (_arm_steps(raw[arm], ref, tol, w, every, hold)[0] if ref is not None
+                                        else {r: None for r in RULES}) for arm in raw}
         recs.append(rec)
-        log(f"  eval {i + 1}/{n_shapes} {d['kind']}: Cd_ref {ref:.4f}  steps plain/warm/coarse: reach "
-            f"{steps_p['reach']}/{steps_w['reach']}/{rec['coarse']['reach']}  hold {steps_p['hold']}/{steps_w['hold']}/"
-            f"{rec['coarse']['hold']}  horizon {steps_p['horizon']}/{steps_w['horizon']}/{rec['coarse']['horizon']}  "
-            f"(warm final Cd {rec['warm_cd_final']})")
+        st = rec["steps"]
+        log(f"  eval {i + 1}/{n_shapes} {d['kind']}: Cd rest {rec['cd_final']['plain']:.4f} mid "
+            f"{rec['cd_final']['plain_mid']:.4f} warm {rec['cd_final']['warm']} coarse {rec['cd_final']['coarse']}; "
+            + "; ".join(f"vs {b}: " + " ".join(f"{r} " + "/".join(str(st[b][a][r]) for a in ("plain", "plain_mid",
+                                                                                          "warm", "coarse"))
+                                              for r in RULES) for b in PLAIN_ARMS))
         if time_budget_s is not None and time.time() - t_start > time_budget_s:
             log(f"  evaluation time budget reached after {i + 1} shapes")
             break
     return recs
 
 
-def summarize(recs, horizon):
-    """Per rule: steps and wall time to convergence (mean, p50, p95) of the plain and
-    warm arms, the wall-time speedup, the number of warm runs that fell back to the plain
-    start (no convergence within the horizon: charged the plain run's time), and the
-    coarse-grid arm; plus the final Cd difference between warm and plain runs."""
-    def stats(v):
-        v = np.asarray(v, float)
-        return {"mean": float(v.mean()), "p50": float(np.percentile(v, 50)), "p95": float(np.percentile(v, 95))}
+def _stats(v):
+    v = np.asarray(v, float)
+    return {"mean": float(v.mean()), "p50": float(np.percentile(v, 50)), "p95": float(np.percentile(v, 95))}
 
+
+def summarize(recs, horizon):
+    """Per plain baseline and rule: steps and wall time to convergence (mean, p50, p95) of
+    the baseline and of each seeded arm, the wall-time speedups, and the seeded runs that
+    fell back to the plain start (no convergence within the horizon: charged the
+    baseline's time plus the seeding cost).  Also the spread of the final (trailing-mean)
+    Cd between arms: warm, coarse and the other plain start, each against each plain arm."""
+    cost = {"warm": "infer_ms", "coarse": "coarse_ms"}
     out = {"shapes": len(recs), "horizon_steps": horizon}
-    for rule in RULES:
-        plain_s, warm_s, plain_t, warm_t, coarse_s, coarse_t = [], [], [], [], [], []
-        fallbacks = unconverged_plain = coarse_fallbacks = 0
-        for r in recs:
-            ps = r["plain"][rule]
-            if ps is None:
-                unconverged_plain += 1
+    for base in PLAIN_ARMS:
+        ob = {}
+        for rule in RULES:
+            o = {"plain_unconverged": 0}
+            base_s, base_t = [], []
+            arm_s = {a: [] for a in SEEDED_ARMS}
+            arm_t = {a: [] for a in SEEDED_ARMS}
+            fb = {a: 0 for a in SEEDED_ARMS}
+            for r in recs:
+                st = r["steps"][base]
+                ps = st[base][rule]
+                if ps is None:
+                    o["plain_unconverged"] += 1
+                    continue
+                ms = r["ms_per_step"]
+                base_s.append(ps)
+                base_t.append(ps * ms)
+                for a in SEEDED_ARMS:
+                    sa = st[a][rule]
+                    if sa is None:
+                        fb[a] += 1
+                        sa = ps
+                    arm_s[a].append(sa)
+                    arm_t[a].append(sa * ms + r.get(cost[a], 0.0))
+            if base_s:
+                o.update(plain_steps=_stats(base_s), plain_ms=_stats(base_t))
+                for a in SEEDED_ARMS:
+                    o[a] = {"steps": _stats(arm_s[a]), "ms": _stats(arm_t[a]), "fallbacks": fb[a],
+                            "speedup_wall": _stats(np.asarray(base_t) / np.asarray(arm_t[a])),
+                            "speedup_wall_of_means": float(np.mean(base_t) / np.mean(arm_t[a]))}
+            ob[rule] = o
+        out[base] = ob
+    spread = {}
+    for base in PLAIN_ARMS:
+        for a in ("plain", "plain_mid") + SEEDED_ARMS:
+            if a == base:
                 continue
-            ms = r["ms_per_step"]
-            ws = r["warm"][rule]
-            if ws is None:
-                fallbacks += 1
-                ws_eff, wt = ps, ps * ms + r["infer_ms"]
-            else:
-                ws_eff, wt = ws, ws * ms + r["infer_ms"]
-            plain_s.append(ps)
-            warm_s.append(ws_eff)
-            plain_t.append(ps * ms)
-            warm_t.append(wt)
-            cs_ = r["coarse"][rule]
-            if cs_ is None:
-                coarse_fallbacks += 1
-                coarse_s.append(ps)
-                coarse_t.append(ps * ms + r.get("coarse_ms", 0.0))
-            else:
-                coarse_s.append(cs_)
-                coarse_t.append(cs_ * ms + r.get("coarse_ms", 0.0))
-        o = {"plain_unconverged": unconverged_plain, "warm_fallbacks": fallbacks,
-             "coarse_fallbacks": coarse_fallbacks}
-        if plain_s:
-            o.update(plain_steps=stats(plain_s), warm_steps=stats(warm_s), plain_ms=stats(plain_t),
-                     warm_ms=stats(warm_t), speedup_wall=stats(np.asarray(plain_t) / np.asarray(warm_t)),
-                     speedup_wall_of_means=float(np.mean(plain_t) / np.mean(warm_t)),
-                     coarse_steps=stats(coarse_s), coarse_ms=stats(coarse_t),
-                     coarse_speedup_wall_of_means=float(np.mean(plain_t) / np.mean(coarse_t)))
-        out[rule] = o
-    dcd = [abs(r["warm_cd_final"] - r["cd_ref"]) / abs(r["cd_ref"]) for r in recs
-           if r["cd_ref"] != 0 and r["warm_cd_final"] is not None]
-    if dcd:
-        out["final_cd_rel_diff_warm_vs_plain"] = stats(dcd)
-    dcl = [abs(r["warm_cl_final"] - r["cl_ref"]) for r in recs if r["warm_cl_final"] is not None]
-    if dcl:
-        out["final_cl_abs_diff_warm_vs_plain"] = stats(dcl)
+            v = [abs(r["cd_final"][a] - r["cd_final"][base]) / abs(r["cd_final"][base]) for r in recs
+                 if r["cd_final"][a] is not None and r["cd_final"][base] not in (None, 0.0)]
+            if v:
+                spread[f"{a}_vs_{base}"] = _stats(v)
+    out["final_cd_rel_diff"] = spread
     return out
 
 
@@ -489,7 +543,7 @@ def cmd_evaluate(args):
     torch.cuda.init()
     net = sg.Surrogate(seed=0)
     load_weights(net, args.weights)
-    storage = {"f16": Storage.F16, "bf16": Storage.BF16, "f32": Storage.F32}[args.storage]
+    storage = {"f16sr": Storage.F16_SR, "f16": Storage.F16, "bf16": Storage.BF16, "f32": Storage.F32}[args.storage]
     log(f"evaluate: {args.shapes} held-out shapes (seed {args.seed}) on {dims}, {args.storage} storage, "
         f"horizon {args.horizon}, Cd every {args.every} averaged over {args.window} steps, tol {args.tol}")
     recs = evaluate(net, dims, args.shapes, args.seed, args.horizon, args.every, args.window, args.tol, args.u,
@@ -541,7 +595,7 @@ def main(argv=None):
     ev.add_argument("--window", type=int, default=1000)
     ev.add_argument("--tol", type=float, default=0.01)
     ev.add_argument("--hold", type=int, default=1000, help="steps a run must stay in the band (hold rule)")
-    ev.add_argument("--storage", default="f16", choices=["f16", "bf16", "f32"])
+    ev.add_argument("--storage", default="f16sr", choices=["f16sr", "f16", "bf16", "f32"])
     ev.add_argument("--coarse-div", type=int, default=10)
     ev.add_argument("--coarse-steps", type=int, default=1000)
     ev.add_argument("--budget", type=float, default=None, help="seconds")
