@@ -522,6 +522,99 @@ constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHV = kHX * kHY * kHZ;  // 648 halo voxels
 constexpr int kCC = 32, kCS = kCC + 8;  // channels per chunk; LDS row stride (bf16), padded against bank conflicts
 
+// Stages the halo of channels [cb, cb + 8*R8) into sX: one 16-byte run per thread iteration
+// (zeros outside the volume; for upsample-concat the nearest-upsampled coarse channels and
+// the skip channels side by side).
+template <int MODE, int R8>
+__device__ __forceinline__ void StageHalo(const ConvArgs& a, u16* sX, int n, int x0, int y0, int z0, int cb) {
+    for (int e = threadIdx.x; e < kHV * R8; e += 256) {
+        const int hv = e / R8, r8 = e - hv * R8;
+        const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+        const int ix = x0 - 1 + hx, iy = y0 - 1 + hy, iz = z0 - 1 + hz;
+        uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+        if (ix >= 0 && iy >= 0 && iz >= 0 && ix < a.wi && iy < a.hi && iz < a.di) {
+            const int ci = cb + r8 * 8;
+            if (MODE == kUpcat && ci < a.c1) {
+                const int h2 = a.hi >> 1, w2 = a.wi >> 1, d2 = a.di >> 1;
+                const long long vi = ((static_cast<long long>(n) * d2 + (iz >> 1)) * h2 + (iy >> 1)) * w2 + (ix >> 1);
+                raw = *reinterpret_cast<const uint4*>(a.in2 + vi * a.c1 + ci);
+            } else {
+                const int cs = MODE == kUpcat ? a.kin - a.c1 : a.kin;
+                const int c0 = MODE == kUpcat ? ci - a.c1 : ci;
+                const long long vi = ((static_cast<long long>(n) * a.di + iz) * a.hi + iy) * a.wi + ix;
+                raw = *reinterpret_cast<const uint4*>(a.in + vi * cs + c0);
+            }
+        }
+        *reinterpret_cast<uint4*>(&sX[hv * kCS + r8 * 8]) = raw;
+    }
+}
+
+#ifndef FC_NN_CHUNK_UNROLL
+#define FC_NN_CHUNK_UNROLL 1
+#endif
+// The implicit GEMM of one staged chunk of 8*R8 channels: its 27*R8 (tap, 8-channel run)
+// pairs, four per 32-wide MFMA K step (lane group g takes run 4s + g).  The chunk width is
+// a compile-time constant, so the run -> (tap, channel) arithmetic is shifts and constant
+// divisions; the operands of step s+1 (weights from global memory, halo fragments from
+// LDS) are loaded before the MFMAs of step s (software pipelining), so their latency
+// overlaps one step of matrix work.
+template <int MODE, int TN, int R8>
+__device__ __forceinline__ void ChunkGemm(const ConvArgs& a, const u16* sX, int wave, int row, int grp, int nbase,
+                                          int K, int cb, f32x4 (&acc)[4][TN]) {
+    constexpr int kRuns = 27 * R8;
+    constexpr int kSteps = (kRuns + 3) / 4;
+    const u16* wrow[TN];
+    bool wok[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int nn = nbase + j * 16 + row;
+        wok[j] = nn < a.nout;
+        wrow[j] = a.w + static_cast<long long>(wok[j] ? nn : 0) * K + cb;
+    }
+    const int lbase = (wave * kHY * kHX + row) * kCS;  // the lane's halo row at tap (0,0,0), m = 0
+    // step s's operands: the weight fragments of its runs and the four halo fragments
+    auto load = [&](int s, uint4 (&bv)[TN], uint4 (&av)[4]) {
+        const int r = 4 * s + grp;
+        const bool rok = r < kRuns;
+        const int t = r / R8, c8 = r - t * R8;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            bv[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (rok && wok[j]) bv[j] = *reinterpret_cast<const uint4*>(wrow[j] + t * a.kin + c8 * 8);
+        }
+        const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+        // halo offsets of the tap: forward/upcat read o + k - 1, the adjoint o + 1 - k
+        const int oz = MODE == kAdj ? 2 - kz : kz, oy = MODE == kAdj ? 2 - ky : ky, ox = MODE == kAdj ? 2 - kx : kx;
+        const int tap = lbase + ((oz * kHY + oy) * kHX + ox) * kCS + c8 * 8;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            av[m] = make_uint4(0u, 0u, 0u, 0u);
+            if (rok) av[m] = *reinterpret_cast<const uint4*>(&sX[tap + m * kHX * kCS]);
+        }
+    };
+    uint4 bcur[TN], acur[4];
+    load(0, bcur, acur);
+#pragma unroll FC_NN_CHUNK_UNROLL
+    for (int s = 0; s < kSteps; ++s) {
+        uint4 bnext[TN], anext[4];
+        if (s + 1 < kSteps) load(s + 1, bnext, anext);  // issued ahead of this step's MFMAs
+        bf16x8 bfr[TN];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) __builtin_memcpy(&bfr[j], &bcur[j], 16);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            bf16x8 afr;
+            __builtin_memcpy(&afr, &acur[m], 16);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bcur[j] = bnext[j];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acur[m] = anext[m];
+    }
+}
+
 template <int MODE, int TN>
 __global__ __launch_bounds__(256) void KConvTile(const ConvArgs a) {
     __shared__ __attribute__((aligned(16))) u16 sX[kHV * kCS];
@@ -545,64 +638,18 @@ __global__ __launch_bounds__(256) void KConvTile(const ConvArgs a) {
     const int K = 27 * a.kin;
     for (int cb = 0; cb < a.kin; cb += kCC) {
         const int cc = a.kin - cb < kCC ? a.kin - cb : kCC;  // channels in this chunk (multiple of 8)
-        const int runs8 = cc / 8;
-        // ---- stage the halo of channels [cb, cb+cc): one 16-byte run per thread iteration
-        for (int e = threadIdx.x; e < kHV * runs8; e += 256) {
-            const int hv = e / runs8, r8 = e - hv * runs8;
-            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-            const int ix = x0 - 1 + hx, iy = y0 - 1 + hy, iz = z0 - 1 + hz;
-            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-            if (ix >= 0 && iy >= 0 && iz >= 0 && ix < a.wi && iy < a.hi && iz < a.di) {
-                const int ci = cb + r8 * 8;
-                if (MODE == kUpcat && ci < a.c1) {
-                    const int h2 = a.hi >> 1, w2 = a.wi >> 1, d2 = a.di >> 1;
-                    const long long vi = ((static_cast<long long>(n) * d2 + (iz >> 1)) * h2 + (iy >> 1)) * w2 + (ix >> 1);
-                    raw = *reinterpret_cast<const uint4*>(a.in2 + vi * a.c1 + ci);
-                } else {
-                    const int cs = MODE == kUpcat ? a.kin - a.c1 : a.kin;
-                    const int c0 = MODE == kUpcat ? ci - a.c1 : ci;
-                    const long long vi = ((static_cast<long long>(n) * a.di + iz) * a.hi + iy) * a.wi + ix;
-                    raw = *reinterpret_cast<const uint4*>(a.in + vi * cs + c0);
-                }
-            }
-            *reinterpret_cast<uint4*>(&sX[hv * kCS + r8 * 8]) = raw;
+        switch (cc / 8) {
+            case 1: StageHalo<MODE, 1>(a, sX, n, x0, y0, z0, cb); break;
+            case 2: StageHalo<MODE, 2>(a, sX, n, x0, y0, z0, cb); break;
+            case 3: StageHalo<MODE, 3>(a, sX, n, x0, y0, z0, cb); break;
+            default: StageHalo<MODE, 4>(a, sX, n, x0, y0, z0, cb); break;
         }
         __syncthreads();
-        // ---- implicit GEMM over the (tap, run) pairs of this chunk, 4 runs per K step
-        const int nruns = 27 * runs8;
-        for (int r0 = 0; r0 < nruns; r0 += 4) {
-            const int r = r0 + grp;  // this lane group's run
-            bf16x8 bfr[TN];
-            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-            int t = 0, c8 = 0;
-            if (r < nruns) {
-                t = r / runs8;
-                c8 = r - t * runs8;
-            }
-#pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                const int nn = nbase + j * 16 + row;
-                uint4 wv = make_uint4(0u, 0u, 0u, 0u);
-                if (r < nruns && nn < a.nout)
-                    wv = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(nn) * K + t * a.kin + cb + c8 * 8);
-                __builtin_memcpy(&bfr[j], &wv, 16);
-            }
-            const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
-            // halo offsets of the tap: forward/upcat read o + k - 1, the adjoint o + 1 - k
-            const int oz = MODE == kAdj ? 2 - kz : kz, oy = MODE == kAdj ? 2 - ky : ky, ox = MODE == kAdj ? 2 - kx : kx;
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                if (r < nruns) {
-                    const int hv = ((wave + oz) * kHY + (m + oy)) * kHX + (row + ox);
-                    raw = *reinterpret_cast<const uint4*>(&sX[hv * kCS + c8 * 8]);
-                } else {
-                    raw = make_uint4(0u, 0u, 0u, 0u);
-                }
-                bf16x8 afr;
-                __builtin_memcpy(&afr, &raw, 16);
-#pragma unroll
-                for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
-            }
+        switch (cc / 8) {
+            case 1: ChunkGemm<MODE, TN, 1>(a, sX, wave, row, grp, nbase, K, cb, acc); break;
+            case 2: ChunkGemm<MODE, TN, 2>(a, sX, wave, row, grp, nbase, K, cb, acc); break;
+            case 3: ChunkGemm<MODE, TN, 3>(a, sX, wave, row, grp, nbase, K, cb, acc); break;
+            default: ChunkGemm<MODE, TN, 4>(a, sX, wave, row, grp, nbase, K, cb, acc); break;
         }
         __syncthreads();
     }

@@ -365,6 +365,8 @@ def time_layers(nb=4, dims=(128, 128, 256), reps=10, train_reps=5, log=print):
         tot_f += flops
         rows.append((name, V, kin, cout, ms, flops / ms / 1e9))
         log(f"{name:4s} V={V:9d} K={27 * kin:5d} N={cout:4d}  {ms:8.3f} ms  {flops / ms / 1e9:8.1f} TFLOP/s")
+    if train_reps <= 0:
+        return {"layers": rows, "train_step_ms": None, "fwd_flops": tot_f}
     tgt = torch.zeros(V0 * 4, device=net.dev)
     wm = torch.ones(V0, device=net.dev)
     for _ in range(2):
@@ -388,11 +390,13 @@ if __name__ == "__main__":
                                  description="Time the surrogate's convolution kernels and a training step.")
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--dims", type=int, nargs=3, default=[128, 128, 256], metavar=("D", "H", "W"))
+    ap.add_argument("--reps", type=int, default=10, help="launches per layer timing")
+    ap.add_argument("--train-reps", type=int, default=5, help="training steps timed (0: layers only)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     import torch
     torch.cuda.init()
-    r = time_layers(a.batch, tuple(a.dims))
+    r = time_layers(a.batch, tuple(a.dims), reps=a.reps, train_reps=a.train_reps)
     if a.out:
         with open(a.out, "w") as f:
             json.dump(r, f)
