@@ -36,8 +36,10 @@ TARGETS = {
     "libfc_lbm.so": [("fc_lbm.hip", []), ("voxelize.hip", []), ("smoke.hip", [])] + _STEP_UNITS,
     "libfc_nn.so": [("nn_kernels.hip", [])],
 }
+# Standalone programs (not linked into the libraries): the copy kernel that calibrates
+# rocprofv3's traffic counters.
 EXECUTABLES = {
-    "hbm_probe": ["hbm_probe.hip"],
+    "hbm_copy": ["hbm_copy.hip"],
 }
 
 
