@@ -11,8 +11,8 @@ deterministic geometry.  A "step" is one fused stream-collide pass over all
 FlowCraft/main.cpp:3781-3783.
 
 `value` is the FP32-storage throughput; FP16 and BF16 offset storage, D3Q15, the
-other population layout and the 128x64x64 sphere (0.5M cells, launch-bound) are
-reported under `variants`.  Timing: W untimed steps, then K steps bracketed
+other population layout and the periodic 128x64x64 sphere (0.5M cells, latency-bound,
+populations held in the last-level cache) are reported under `variants`.  Timing: W untimed steps, then K steps bracketed
 by HIP events recorded on the solver's own stream, after a device synchronise on
 both sides.  The roofline uses the algorithmic 2*Q*s bytes per cell update.
 The CPU baseline times FlowCraft's LbmReference::Step (oracle/_ref) — or the
@@ -81,10 +81,10 @@ def wing_grid():
                                    aoa_deg=4.0)
 
 
-def time_variant(torch, grid, model, storage, layout, steps, warmup, tau=None, params=None):
+def time_variant(torch, grid, model, storage, layout, steps, warmup, tau=None, params=None, periodic=False):
     from flowcraft_amd.lbm import LbmSolver, Params
     s = LbmSolver()
-    if not s.Init(None, grid, model, storage=storage, layout=layout):
+    if not s.Init(None, grid, model, storage=storage, layout=layout, periodicY=periodic, periodicZ=periodic):
         raise RuntimeError(s.LastError())
     s.Reset(params or Params(tau=TAU if tau is None else tau, inletVelocity=0.08, smagorinskyCs=0.1), False)
     stream = torch.cuda.ExternalStream(s.Stream())
@@ -483,6 +483,9 @@ def main():
                          "prints the HIP-event average step time")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f16sr", "bf16sr"],
                     help="storage of the --profile run")
+    ap.add_argument("--profile-sphere", action="store_true",
+                    help="only the D3Q19 step kernel on the periodic 128x64x64 sphere (Cs 0, tau 0.56, u 0.05), "
+                         "--storage and --layout, for --steps steps (for rocprofv3 runs)")
     ap.add_argument("--profile-aero", action="store_true",
                     help="only the on-device aerodynamic-force summary, --steps calls (for rocprofv3 runs)")
     ap.add_argument("--profile-smoke", action="store_true",
@@ -514,6 +517,19 @@ def main():
     if args.profile_aero:
         print(json.dumps({"profile": "aero_forces", **aero_forces_block(torch, grid, cpu=False, reps=args.steps)}),
               flush=True)
+        return
+    if args.profile_sphere:
+        from flowcraft_amd import geometry
+        st = {"f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16, "f16sr": Storage.F16_SR,
+              "bf16sr": Storage.BF16_SR}[args.storage]
+        small = geometry.sphere_grid(128, 64, 64, (32, 32, 32), 8)
+        r = time_variant(torch, small, LatticeModel.D3Q19, st, layout, args.steps, args.warmup,
+                         params=Params(tau=0.56, inletVelocity=0.05, smagorinskyCs=0.0), periodic=True)
+        b = 4 if st == Storage.F32 else 2
+        print(json.dumps({"profile": f"D3Q19 {args.storage} {layout.name.lower()} periodic 128x64x64 sphere",
+                          "steps": args.steps, "us_per_step": r["ms_per_step"] * 1e3, "mlups": r["mlups"],
+                          "algorithmic_bytes_per_launch": 2 * 19 * b * small.CellCount(),
+                          "achieved_GBs": r["mlups"] * 1e6 * 2 * 19 * b / 1e9}), flush=True)
         return
     if args.profile:
         st = {"f32": Storage.F32, "f16": Storage.F16, "bf16": Storage.BF16, "f16sr": Storage.F16_SR,
@@ -559,17 +575,20 @@ def main():
                 "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
                 "achieved_GBs": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
                 "population_bytes": r["population_bytes"], "finite": r["finite"]}
-        # the 128x64x64 sphere (0.5M cells, one wave of workgroups): launch- and
-        # latency-bound, reported beside the bandwidth-bound lattices
+        # the 128x64x64 sphere, periodic in y and z (0.5M cells, 80 MB of populations, held
+        # in the last-level cache between steps): latency-bound, reported beside the
+        # bandwidth-bound lattices.  Kernels run back to back (no launch gaps), so a graph
+        # or a multi-step launch would not shorten a step.
         from flowcraft_amd import geometry
         small = geometry.sphere_grid(128, 64, 64, (32, 32, 32), 8)
-        r = time_variant(torch, small, LatticeModel.D3Q19, Storage.F32, layout, 2000, 100,
-                         params=Params(tau=0.56, inletVelocity=0.05, smagorinskyCs=0.0))
-        gbs = r["mlups"] * 1e6 * 152 / 1e9
-        variants["D3Q19_F32_128x64x64"] = {
-            "workload": "128x64x64 sphere r 8 at (32,32,32), Cs 0, tau 0.56, u 0.05", "mlups": round(r["mlups"], 1),
-            "us_per_step": round(r["ms_per_step"] * 1e3, 2), "achieved_GBs": round(gbs, 1),
-            "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "finite": r["finite"]}
+        for st in (Storage.F32, Storage.F16_SR):
+            r = time_variant(torch, small, LatticeModel.D3Q19, st, layout, 2000, 100,
+                             params=Params(tau=0.56, inletVelocity=0.05, smagorinskyCs=0.0), periodic=True)
+            gbs = r["mlups"] * 1e6 * (152 if st == Storage.F32 else 76) / 1e9
+            variants[f"D3Q19_{st.name}_128x64x64"] = {
+                "workload": "128x64x64 sphere r 8 at (32,32,32), periodic y/z, Cs 0, tau 0.56, u 0.05",
+                "mlups": round(r["mlups"], 1), "us_per_step": round(r["ms_per_step"] * 1e3, 2),
+                "achieved_GBs": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "finite": r["finite"]}
 
     bytes_per_cell = 2 * 19 * 4
     achieved = main_r["mlups"] * 1e6 * bytes_per_cell / 1e9

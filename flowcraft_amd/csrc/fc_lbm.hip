@@ -96,6 +96,7 @@ struct fc_lbm {
     bool has_geometry = false;
     bool has_state = false;
     int cells_per_thread = 1;  // cells each step-kernel thread updates (1 or 2)
+    bool resident = false;     // populations fit in the last-level cache: cached stores
     std::vector<uint8_t> solid_host;
     std::string err;
 };
@@ -103,6 +104,15 @@ struct fc_lbm {
 namespace {
 
 const char* kNoHandle = "null fc_lbm handle";
+
+// Lattices whose population buffers together fit in the 256 MB last-level cache (the
+// Infinity Cache) keep their stores cached, since the next step pulls them back; larger
+// ones stream the stores to HBM.  Measured on MI355X (ms per step, streamed -> cached):
+// the periodic 128x64x64 sphere FP32 ping-pong (80 MB) 21.7 -> 19.2 us, 128x64x128 (167
+// MB) 35.6 -> 29.3 us, 128^3 FP32 in place (164 MB) 57.1 -> 52.6 us; past the cache size
+// cached stores lose (192x96x96 FP32 ping-pong, 280 MB: 58.8 -> 60.0 us; 256x128x128
+// FP16 ping-pong, 328 MB: 67.3 -> 71.1 us).  FC_LBM_RESIDENT=0/1 overrides the choice.
+constexpr long long kResidentBytes = 256LL << 20;
 constexpr size_t kPopMargin = 256;
 
 // Elements of one population copy: the front margin, nz+2 padded layers of ny+2 rows of
@@ -401,6 +411,8 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
         FC_HIP(h, hipMemsetAsync(h->pops[i], 0, popbytes, h->stream));
     }
     if (copies == 1) h->pops[1] = h->pops[0];
+    h->resident = static_cast<long long>(popbytes) * copies <= kResidentBytes;
+    if (const char* e = std::getenv("FC_LBM_RESIDENT")) h->resident = std::atoi(e) != 0;
     FC_HIP(h, hipMalloc(&h->side_mom, 16ull * h->ny * h->nz * h->rbytes));
     FC_HIP(h, hipMemsetAsync(h->side_mom, 0, 16ull * h->ny * h->nz * h->rbytes, h->stream));
     FC_HIP(h, hipMalloc(&h->dforces, 4 * sizeof(double)));
@@ -680,7 +692,8 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     hipError_t e = hipSuccess;
     DispatchStep(h, [&]<int Q, class S, class Lay>() {
         InletPops<Q, S>(h, a);
-        e = LaunchStep<Q, S, Lay>(a, static_cast<unsigned>(tiles), cpt, (h->pery | h->perz) != 0, h->stream);
+        e = LaunchStep<Q, S, Lay>(a, static_cast<unsigned>(tiles), cpt, (h->pery | h->perz) != 0, h->resident,
+                                  h->stream);
     });
     FC_HIP(h, e);
     return FC_LBM_OK;

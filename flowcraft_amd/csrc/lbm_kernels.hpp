@@ -861,8 +861,10 @@ __device__ __forceinline__ void SpecLoad(const StepArgs& a, BufRsrc src, const P
 
 // Stores of a cell's post-collision populations (and, in place, the side data that
 // other cells read next step).  `seam`: a cell on a periodic y/z edge, whose anchors
-// wrap (only instantiated for periodic lattices).
-template <int Q, class S, class Lay, bool PER>
+// wrap (only instantiated for periodic lattices).  SA: cache policy of the stores that
+// stay on the block grid (kStoreAux streams them; 0 keeps them in the caches, for
+// lattices whose populations fit in the last-level cache).
+template <int Q, class S, class Lay, bool PER, int SA>
 __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRsrc dst_layer, const PopLane& ln, int x,
                                           int y, int z, bool seam, const typename S::R (&fo)[Q], BRec<typename S::R>* rec) {
     using T = typename S::T;
@@ -873,7 +875,7 @@ __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRs
         StaticFor<0, Q>([&](auto I) {
             constexpr int d = I;
             constexpr int dx = StoreDx<Q, Lay>(d);
-            constexpr int aux = dx != 0 ? kShiftedStoreAux : kStoreAux;
+            constexpr int aux = dx != 0 ? kShiftedStoreAux : SA;
             BufStore<T, aux>(dst_layer, LaneSel<dx>(ln), a.st_off[d], enc[d]);
         });
     };
@@ -882,7 +884,10 @@ __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRs
             T* dst = static_cast<T*>(a.pop_dst);
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                PutPop(&dst[SlotIndex<Q, true>(g, d, a.parity, x, y, z)], enc[d]);
+                if constexpr (SA == 0)
+                    dst[SlotIndex<Q, true>(g, d, a.parity, x, y, z)] = enc[d];
+                else
+                    PutPop(&dst[SlotIndex<Q, true>(g, d, a.parity, x, y, z)], enc[d]);
             });
         } else {
             store_all();
@@ -905,7 +910,7 @@ __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRs
 
 // Everything after the bulk pull for one cell: branch on the cell type, patch wall and
 // solid directions, collide, store.
-template <int Q, class S, class Lay, bool PER>
+template <int Q, class S, class Lay, bool PER, int SA>
 __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufRsrc src_layer, BufRsrc dst_layer,
                                            int x, int y, int z, const typename S::R (&fs)[Q], uint8_t fl) {
     using T = typename S::T;
@@ -931,7 +936,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
         if constexpr (!IP) {
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                BufStore<T>(dst_layer, ln.c, a.st_off[d], S::Enc(fo[d]));
+                BufStore<T, SA>(dst_layer, ln.c, a.st_off[d], S::Enc(fo[d]));
             });
         } else {
             if (x == 1 || x == a.nx - 2) {
@@ -1106,7 +1111,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
     }
     // One store sequence for every non-solid cell: inlet and outlet lanes write together
     // with the fluid lanes of their wave, so no cache line is written in two pieces.
-    StoreCell<Q, S, Lay, PER>(a, g, dst_layer, ln, x, y, z, seam, fo, rec);
+    StoreCell<Q, S, Lay, PER, SA>(a, g, dst_layer, ln, x, y, z, seam, fo, rec);
 }
 
 #ifndef FC_LBM_PIN_PULLS
@@ -1127,8 +1132,11 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
 #else
 #define FC_LBM_STEP_ATTR
 #endif
-template <int Q, class S, class Lay, int CPT, bool PER>
+// RESIDENT: the populations fit in the last-level cache, so the stores stay cached for
+// the next step's pulls instead of streaming to HBM.
+template <int Q, class S, class Lay, int CPT, bool PER, bool RESIDENT = false>
 __global__ __launch_bounds__(256) FC_LBM_STEP_ATTR void KStep(const StepArgs a) {
+    constexpr int SA = RESIDENT ? 0 : kStoreAux;
     using R = typename S::R;
     using T = typename S::T;
     S::StepMode();
@@ -1179,7 +1187,7 @@ __global__ __launch_bounds__(256) FC_LBM_STEP_ATTR void KStep(const StepArgs a) 
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int y = y0 + k * kTileY;
-        if (y < a.ny) UpdateCell<Q, S, Lay, PER>(a, g, src_layer, dst_layer, x, y, z, fs[k], fl[k]);
+        if (y < a.ny) UpdateCell<Q, S, Lay, PER, SA>(a, g, src_layer, dst_layer, x, y, z, fs[k], fl[k]);
     }
 }
 

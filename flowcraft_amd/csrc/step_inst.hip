@@ -13,23 +13,27 @@
 namespace fc {
 
 template <int Q, class S, class Lay>
-hipError_t LaunchStep(const StepArgs& a, unsigned tiles, int cpt, bool periodic, hipStream_t stream) {
+hipError_t LaunchStep(const StepArgs& a, unsigned tiles, int cpt, bool periodic, bool resident, hipStream_t stream) {
     const dim3 grid(tiles), block(256);
     if (periodic) {
         if (cpt == 2)
             hipLaunchKernelGGL((KStep<Q, S, Lay, 2, true>), grid, block, 0, stream, a);
+        else if (resident)
+            hipLaunchKernelGGL((KStep<Q, S, Lay, 1, true, true>), grid, block, 0, stream, a);
         else
             hipLaunchKernelGGL((KStep<Q, S, Lay, 1, true>), grid, block, 0, stream, a);
     } else {
         if (cpt == 2)
             hipLaunchKernelGGL((KStep<Q, S, Lay, 2, false>), grid, block, 0, stream, a);
+        else if (resident)
+            hipLaunchKernelGGL((KStep<Q, S, Lay, 1, false, true>), grid, block, 0, stream, a);
         else
             hipLaunchKernelGGL((KStep<Q, S, Lay, 1, false>), grid, block, 0, stream, a);
     }
     return hipGetLastError();
 }
 
-template hipError_t LaunchStep<FC_Q, FC_STORE<FC_Q>, PingPong>(const StepArgs&, unsigned, int, bool, hipStream_t);
-template hipError_t LaunchStep<FC_Q, FC_STORE<FC_Q>, InPlace>(const StepArgs&, unsigned, int, bool, hipStream_t);
+template hipError_t LaunchStep<FC_Q, FC_STORE<FC_Q>, PingPong>(const StepArgs&, unsigned, int, bool, bool, hipStream_t);
+template hipError_t LaunchStep<FC_Q, FC_STORE<FC_Q>, InPlace>(const StepArgs&, unsigned, int, bool, bool, hipStream_t);
 
 }  // namespace fc

@@ -10,8 +10,9 @@
 namespace fc {
 
 // Enqueues one KStep over `tiles` workgroups on `stream`: cpt cells per thread (1 or 2),
-// the periodic-seam variant when `periodic`.
+// the periodic-seam variant when `periodic`, cached population stores when `resident`
+// (one cell per thread only).
 template <int Q, class S, class Lay>
-hipError_t LaunchStep(const StepArgs& a, unsigned tiles, int cpt, bool periodic, hipStream_t stream);
+hipError_t LaunchStep(const StepArgs& a, unsigned tiles, int cpt, bool periodic, bool resident, hipStream_t stream);
 
 }  // namespace fc

@@ -320,6 +320,35 @@ def test_wing_cross_check_q19_vs_q15():
 
 
 @pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: l.name)
+@pytest.mark.parametrize("storage", [Storage.F32, Storage.F16_SR, Storage.F64], ids=lambda s: s.name)
+def test_store_cache_policies_agree(storage, layout, monkeypatch):
+    """Cached population stores (lattices that fit in the last-level cache, the default at
+    this size) and streamed ones (larger lattices) give identical bits, on a periodic
+    lattice (the in-place seam cells store through their own path), and the streamed
+    build matches the oracle."""
+    grid = small_sphere_grid()
+    p = Params(tau=0.6, inletVelocity=0.05, smagorinskyCs=0.12)
+    out = {}
+    for res in ("1", "0"):
+        monkeypatch.setenv("FC_LBM_RESIDENT", res)
+        eng = make_engine(grid, LatticeModel.D3Q19, storage, layout, periodic=(True, True))
+        eng.Reset(p, True)
+        eng.Step(None, 12)
+        out[res] = (eng.Populations(), eng.ReadbackField())
+        eng.Shutdown()
+    assert np.array_equal(out["1"][0], out["0"][0], equal_nan=True)
+    orc = make_oracle(grid, LatticeModel.D3Q19, storage, periodic=(True, True))
+    oracle_params(orc, p)
+    orc.reset(True)
+    orc.step(12)
+    if storage == Storage.F64:
+        fluid = grid.solid == 0
+        assert rel_err(out["0"][0][:, fluid], orc.pops()[:, fluid]) <= 1e-12
+    else:
+        _fields_close(out["0"][1], orc.field(), 1e-5, grid.solid)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: l.name)
 @pytest.mark.parametrize("storage", [Storage.F32, Storage.F16, Storage.F64], ids=lambda s: s.name)
 def test_cells_per_thread_variants_agree(storage, layout, monkeypatch):
     """The step kernel's one- and two-cells-per-thread builds give identical bits, on a
