@@ -89,7 +89,11 @@ typedef struct fc_lbm_stats {
 } fc_lbm_stats;
 
 /* Replaces LbmSolver::Init's allocation half (FlowCraft/LbmSolver.cpp:205-366).
- * The device is the HIP current device at the call. */
+ * The device is the HIP current device at the call.  Lattice limit: fewer than 2^31
+ * padded cells nx*(ny+2)*(nz+2) (about 1290^3), and three padded z layers of
+ * populations under 2 GiB; larger lattices fail with FC_LBM_EINVAL before any device
+ * call.  (FlowCraft's own ceiling is lower: its solid mask is one 4-byte-per-cell
+ * buffer under 4 GiB, about 1.07e9 cells, FlowCraft/LbmSolver.h:22-58.) */
 int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out);
 
 /* Replaces LbmSolver::Shutdown (FlowCraft/LbmSolver.cpp:368-387). */

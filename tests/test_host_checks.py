@@ -56,3 +56,28 @@ def test_device_voxelizer_rejects_small_output_buffers():
                                   ctypes.c_void_p(256), ctypes.c_void_p(512), cap)
         assert rc == _lib.FC_LBM_EINVAL
         assert f"output buffers hold {cap} cells, the frame has {nx * ny * nz}".encode() in L.fc_voxelize_last_error()
+
+
+def _create(nx, ny, nz, storage=_lib.STORE_F16, layout=_lib.LAYOUT_INPLACE):
+    L = _lib.load()
+    desc = _lib.FcLbmDesc(nx, ny, nz, _lib.D3Q19, storage, layout, 0, 0, None)
+    h = ctypes.c_void_p()
+    rc = L.fc_lbm_create(ctypes.byref(desc), ctypes.byref(h))
+    msg = (L.fc_lbm_last_error(h) or b"").decode()
+    L.fc_lbm_destroy(h)
+    return rc, msg
+
+
+def test_lattice_size_limit_is_checked_before_the_device():
+    """The engine's lattice limit: fewer than 2^31 padded cells nx*(ny+2)*(nz+2), and three
+    padded z layers of populations under 2 GiB (32-bit offsets from a per-workgroup base).
+    2048 x 1026 x 1022 = 2147475456 padded cells is the last nz that fits at this
+    cross-section; one more layer is refused with EINVAL before any device call."""
+    rc, msg = _create(2048, 1024, 1021)
+    assert rc == _lib.FC_LBM_EINVAL and "padded cells must be < 2^31" in msg
+    rc, msg = _create(65536, 8192, 1)
+    assert rc == _lib.FC_LBM_EINVAL and "cross-section too large" in msg
+    if torch.cuda.device_count() == 0:
+        # the size checks pass; without a device the next step (the device query) fails
+        rc, msg = _create(2048, 1024, 1020)
+        assert rc == _lib.FC_LBM_EDEVICE, (rc, msg)
