@@ -136,9 +136,27 @@ def _cpu_baseline_run(oracle, grid, p, cores, budget_s, max_steps):
     t = sim.time_steps(0, n)
     mlups = grid.CellCount() * n / t / 1e6
     del sim
+    online = os.cpu_count() or 0
+    # LbmReference starts std::thread::hardware_concurrency() workers, which counts the
+    # online CPUs, not the affinity mask: on a host with more CPUs than the mask it
+    # oversubscribes the pinned cores.
+    threads = online if kind == "reference" else cores
     return {"value": round(mlups, 2), "unit": "MLUPS", "cores": cores, "kind": kind,
+            "cpu_model": _cpu_model(), "cpus_online": online, "threads": threads,
             "sample": f"{n} steps (after 1 untimed) of the full {NX}x{NY}x{NZ} D3Q19 FP32 lattice, "
-                      f"process pinned to {cores} host cores"}
+                      f"process pinned to {cores} host cores (one GPU's share of the host), "
+                      f"{threads} worker threads" + (" (oversubscribed)" if threads > cores else "")}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def aero_forces_block(torch, grid, cpu=True, reps=20):
