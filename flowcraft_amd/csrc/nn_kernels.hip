@@ -509,23 +509,27 @@ unsigned Blk(long long n, int b) { return static_cast<unsigned>((n + b - 1) / b)
 // LDS-tiled convolution for the stride-1 modes (forward, upsample-concat forward, and
 // the adjoint of a stride-1 convolution).  A workgroup computes a 16 (x) x 4 (y) x 4 (z)
 // block of output voxels (wave w = z-slice w, 4 MFMA row tiles of 16 x-voxels each) for
-// 16*TN output channels.  For each chunk of up to 32 input channels it stages the
+// 16*TN output channels.  For each chunk of up to 8*RMAX input channels it stages the
 // 18 x 6 x 6 halo of the input in LDS once (zeros outside the volume; for upsample-concat
 // the nearest-upsampled coarse channels and the skip channels side by side), then runs
-// the implicit GEMM over (tap, 8-channel run) pairs, four runs per 32-wide MFMA K step:
-// every input voxel is read from global memory once per chunk instead of once per tap.
+// the implicit GEMM of the chunk: every input voxel is read from global memory once per
+// chunk instead of once per tap.  With 16 output channels the chunk's weights are staged
+// in LDS too (each wave would otherwise fetch all of them from L2, every K step a round
+// trip: 0.37 of the 0.99 ms of d0's matrix loop on MI355X).
 #ifndef FC_NN_CONV_TILED
 #define FC_NN_CONV_TILED 1
 #endif
 constexpr bool kConvTiled = FC_NN_CONV_TILED;
 constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHV = kHX * kHY * kHZ;  // 648 halo voxels
-constexpr int kCC = 32, kCS = kCC + 8;  // channels per chunk; LDS row stride (bf16), padded against bank conflicts
+// LDS row stride (bf16) of a halo voxel for chunks of up to 8*RMAX channels, padded
+// against bank conflicts (80 B for 32 channels, 48 B for 8 or 16)
+__host__ __device__ constexpr int HaloStride(int rmax) { return rmax >= 3 ? 8 * rmax + 8 : 24; }
 
 // Stages the halo of channels [cb, cb + 8*R8) into sX: one 16-byte run per thread iteration
 // (zeros outside the volume; for upsample-concat the nearest-upsampled coarse channels and
 // the skip channels side by side).
-template <int MODE, int R8>
+template <int MODE, int R8, int CS>
 __device__ __forceinline__ void StageHalo(const ConvArgs& a, u16* sX, int n, int x0, int y0, int z0, int cb) {
     for (int e = threadIdx.x; e < kHV * R8; e += 256) {
         const int hv = e / R8, r8 = e - hv * R8;
@@ -545,21 +549,36 @@ __device__ __forceinline__ void StageHalo(const ConvArgs& a, u16* sX, int n, int
                 raw = *reinterpret_cast<const uint4*>(a.in + vi * cs + c0);
             }
         }
-        *reinterpret_cast<uint4*>(&sX[hv * kCS + r8 * 8]) = raw;
+        *reinterpret_cast<uint4*>(&sX[hv * CS + r8 * 8]) = raw;
+    }
+}
+
+// Stages the weights of channels [cb, cb + 8*R8) of the 16 output channels into sW as
+// [tap][8-channel run][output channel][8]: a lane group's B fragment (16 output channels
+// of one run of one tap) is then 256 contiguous bytes.
+template <int R8>
+__device__ __forceinline__ void StageWeights16(const ConvArgs& a, u16* sW, int K, int cb) {
+    for (int e = threadIdx.x; e < 16 * 27 * R8; e += 256) {
+        const int nn = e & 15, tc = e >> 4;
+        const int t = tc / R8, c8 = tc - t * R8;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (nn < a.nout) v = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(nn) * K + t * a.kin + cb + c8 * 8);
+        *reinterpret_cast<uint4*>(&sW[e * 8]) = v;
     }
 }
 
 #ifndef FC_NN_CHUNK_UNROLL
 #define FC_NN_CHUNK_UNROLL 1
 #endif
-// The implicit GEMM of one staged chunk of 8*R8 channels: its 27*R8 (tap, 8-channel run)
+// The implicit GEMM of one staged chunk of 8*R8 channels, for wide output tiles (TN = 4,
+// whose 16*TN accumulators leave no room for the six-row schedule below): its 27*R8 (tap, 8-channel run)
 // pairs, four per 32-wide MFMA K step (lane group g takes run 4s + g).  The chunk width is
 // a compile-time constant, so the run -> (tap, channel) arithmetic is shifts and constant
 // divisions; the operands of step s+1 (weights from global memory, halo fragments from
 // LDS) are loaded before the MFMAs of step s (software pipelining), so their latency
 // overlaps one step of matrix work.
-template <int MODE, int TN, int R8>
-__device__ __forceinline__ void ChunkGemm(const ConvArgs& a, const u16* sX, int wave, int row, int grp, int nbase,
+template <int MODE, int TN, int R8, int CS>
+__device__ __forceinline__ void ChunkGemmRuns(const ConvArgs& a, const u16* sX, int wave, int row, int grp, int nbase,
                                           int K, int cb, f32x4 (&acc)[4][TN]) {
     constexpr int kRuns = 27 * R8;
     constexpr int kSteps = (kRuns + 3) / 4;
@@ -571,7 +590,7 @@ __device__ __forceinline__ void ChunkGemm(const ConvArgs& a, const u16* sX, int 
         wok[j] = nn < a.nout;
         wrow[j] = a.w + static_cast<long long>(wok[j] ? nn : 0) * K + cb;
     }
-    const int lbase = (wave * kHY * kHX + row) * kCS;  // the lane's halo row at tap (0,0,0), m = 0
+    const int lbase = (wave * kHY * kHX + row) * CS;  // the lane's halo row at tap (0,0,0), m = 0
     // step s's operands: the weight fragments of its runs and the four halo fragments
     auto load = [&](int s, uint4 (&bv)[TN], uint4 (&av)[4]) {
         const int r = 4 * s + grp;
@@ -585,11 +604,11 @@ __device__ __forceinline__ void ChunkGemm(const ConvArgs& a, const u16* sX, int 
         const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
         // halo offsets of the tap: forward/upcat read o + k - 1, the adjoint o + 1 - k
         const int oz = MODE == kAdj ? 2 - kz : kz, oy = MODE == kAdj ? 2 - ky : ky, ox = MODE == kAdj ? 2 - kx : kx;
-        const int tap = lbase + ((oz * kHY + oy) * kHX + ox) * kCS + c8 * 8;
+        const int tap = lbase + ((oz * kHY + oy) * kHX + ox) * CS + c8 * 8;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             av[m] = make_uint4(0u, 0u, 0u, 0u);
-            if (rok) av[m] = *reinterpret_cast<const uint4*>(&sX[tap + m * kHX * kCS]);
+            if (rok) av[m] = *reinterpret_cast<const uint4*>(&sX[tap + m * kHX * CS]);
         }
     };
     uint4 bcur[TN], acur[4];
@@ -615,9 +634,102 @@ __device__ __forceinline__ void ChunkGemm(const ConvArgs& a, const u16* sX, int 
     }
 }
 
-template <int MODE, int TN>
-__global__ __launch_bounds__(256) void KConvTile(const ConvArgs a) {
-    __shared__ __attribute__((aligned(16))) u16 sX[kHV * kCS];
+// The implicit GEMM of one staged chunk of 8*R8 channels for 16 output channels, weights
+// in LDS (LDSW: sW, as staged by StageWeights16) or read from global memory (L2).  Which
+// is faster depends on the LDS left for the halo: the weights of a 32-channel chunk (27.6
+// KB) would cut a CU to two workgroups, and on d0 (48 -> 16 channels, 16.8M voxels) that
+// costs more (2.33 ms) than the global reads do (1.84 ms); with 8- or 16-channel chunks
+// three or four workgroups still fit (e0: 0.75 -> 0.61 ms).  The wave's four output rows (y = y0 + m) read
+// halo rows m + oy(ky), so the three ky taps of one (kz, kx, 8-channel run) combination
+// share six halo fragments: each K step takes four such combinations (lane group g takes
+// combination 4s + g), loads the six halo fragments once and the weight fragments of the
+// three ky taps, and runs 12 MFMAs (acc[m] += A[m + oy(ky)] * B[ky]).  The chunk width is
+// a compile-time constant, so the combination -> (kz, kx, run) arithmetic is shifts and
+// constant divisions; the operands of step s+1 are loaded before the MFMAs of step s.
+template <int MODE, int R8, int CS, bool LDSW>
+__device__ __forceinline__ void ChunkGemmRows16(const ConvArgs& a, const u16* sX, const u16* sW, int wave, int row,
+                                                int grp, int K, int cb, f32x4 (&acc)[4][1]) {
+    constexpr int kCombos = 9 * R8;
+    constexpr int kSteps = (kCombos + 3) / 4;
+    const bool wok = row < a.nout;
+    const u16* wrow = a.w + static_cast<long long>(wok ? row : 0) * K + cb;
+    const int lbase = (wave * kHY * kHX + row) * CS;  // the lane's halo row at offset (0,0,0), m = 0
+    // step s's operands: the weight fragments of the three ky taps and six halo rows
+    auto load = [&](int s, uint4 (&bv)[3], uint4 (&av)[6]) {
+        const int c = 4 * s + grp;
+        const bool cok = c < kCombos;
+        const int kzx = c / R8, c8 = c - kzx * R8;
+        const int kz = kzx / 3, kx = kzx - kz * 3;
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+            const int t = (kz * 3 + ky) * 3 + kx;
+            bv[ky] = make_uint4(0u, 0u, 0u, 0u);
+            if constexpr (LDSW) {
+                if (cok) bv[ky] = *reinterpret_cast<const uint4*>(&sW[((t * R8 + c8) * 16 + row) * 8]);
+            } else {
+                if (cok && wok) bv[ky] = *reinterpret_cast<const uint4*>(wrow + t * a.kin + c8 * 8);
+            }
+        }
+        // halo offsets: forward/upcat read o + k - 1, the adjoint o + 1 - k
+        const int oz = MODE == kAdj ? 2 - kz : kz, ox = MODE == kAdj ? 2 - kx : kx;
+        const int base = lbase + (oz * kHY * kHX + ox) * CS + c8 * 8;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            av[r] = make_uint4(0u, 0u, 0u, 0u);
+            if (cok) av[r] = *reinterpret_cast<const uint4*>(&sX[base + r * kHX * CS]);
+        }
+    };
+    uint4 bcur[3], acur[6];
+    load(0, bcur, acur);
+#pragma unroll 1
+    for (int s = 0; s < kSteps; ++s) {
+        uint4 bnext[3], anext[6];
+        if (s + 1 < kSteps) load(s + 1, bnext, anext);  // issued ahead of this step's MFMAs
+        bf16x8 afr[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) __builtin_memcpy(&afr[r], &acur[r], 16);
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+            const int oy = MODE == kAdj ? 2 - ky : ky;
+            bf16x8 bfr;
+            __builtin_memcpy(&bfr, &bcur[ky], 16);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[m + oy], bfr, acc[m][0], 0, 0, 0);
+        }
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) bcur[ky] = bnext[ky];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) acur[r] = anext[r];
+    }
+}
+
+// Weights staged in LDS: 16 output channels and chunks of at most 16 channels.
+__host__ __device__ constexpr bool LdsWeights(int tn, int rmax) { return tn == 1 && rmax <= 2; }
+
+// One chunk: stage the halo (and the weights, when they go through LDS), then its GEMM.
+template <int MODE, int TN, int R8, int CS, bool LDSW>
+__device__ __forceinline__ void ConvChunk(const ConvArgs& a, u16* sX, u16* sW, int n, int x0, int y0, int z0, int cb,
+                                          int wave, int row, int grp, int nbase, int K, f32x4 (&acc)[4][TN]) {
+    StageHalo<MODE, R8, CS>(a, sX, n, x0, y0, z0, cb);
+    if constexpr (LDSW) StageWeights16<R8>(a, sW, K, cb);
+    __syncthreads();
+    if constexpr (TN == 1)
+        ChunkGemmRows16<MODE, R8, CS, LDSW>(a, sX, sW, wave, row, grp, K, cb, acc);
+    else
+        ChunkGemmRuns<MODE, TN, R8, CS>(a, sX, wave, row, grp, nbase, K, cb, acc);
+    __syncthreads();
+}
+
+// Chunks of up to 8*RMAX input channels.  With 16 output channels the register budget is
+// set to three waves per SIMD (170 VGPRs), as many as the LDS footprint allows.
+template <int MODE, int TN, int RMAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3 : 1))) void KConvTile(const ConvArgs a) {
+    constexpr int CS = HaloStride(RMAX);
+    constexpr bool LDSW = LdsWeights(TN, RMAX);
+    constexpr int kW = LDSW ? 16 * 27 * 8 * RMAX : 0;  // staged weights (bf16)
+    __shared__ __attribute__((aligned(16))) u16 sX[kHV * CS + kW];
+    u16* sW = sX + kHV * CS;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + kTZ - 1) / kTZ;
     int b = blockIdx.x;
@@ -636,22 +748,18 @@ __global__ __launch_bounds__(256) void KConvTile(const ConvArgs a) {
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int K = 27 * a.kin;
-    for (int cb = 0; cb < a.kin; cb += kCC) {
-        const int cc = a.kin - cb < kCC ? a.kin - cb : kCC;  // channels in this chunk (multiple of 8)
-        switch (cc / 8) {
-            case 1: StageHalo<MODE, 1>(a, sX, n, x0, y0, z0, cb); break;
-            case 2: StageHalo<MODE, 2>(a, sX, n, x0, y0, z0, cb); break;
-            case 3: StageHalo<MODE, 3>(a, sX, n, x0, y0, z0, cb); break;
-            default: StageHalo<MODE, 4>(a, sX, n, x0, y0, z0, cb); break;
-        }
-        __syncthreads();
-        switch (cc / 8) {
-            case 1: ChunkGemm<MODE, TN, 1>(a, sX, wave, row, grp, nbase, K, cb, acc); break;
-            case 2: ChunkGemm<MODE, TN, 2>(a, sX, wave, row, grp, nbase, K, cb, acc); break;
-            case 3: ChunkGemm<MODE, TN, 3>(a, sX, wave, row, grp, nbase, K, cb, acc); break;
-            default: ChunkGemm<MODE, TN, 4>(a, sX, wave, row, grp, nbase, K, cb, acc); break;
-        }
-        __syncthreads();
+    for (int cb = 0; cb < a.kin; cb += 8 * RMAX) {
+        const int r8 = (a.kin - cb < 8 * RMAX ? a.kin - cb : 8 * RMAX) / 8;  // runs in this chunk
+        // a ragged last chunk of r8 < RMAX runs takes its own instantiation
+        constexpr int R2 = RMAX < 2 ? RMAX : 2, R3 = RMAX < 3 ? RMAX : 3;
+        if (r8 >= RMAX)
+            ConvChunk<MODE, TN, RMAX, CS, LDSW>(a, sX, sW, n, x0, y0, z0, cb, wave, row, grp, nbase, K, acc);
+        else if (r8 == 1)
+            ConvChunk<MODE, TN, 1, CS, LDSW>(a, sX, sW, n, x0, y0, z0, cb, wave, row, grp, nbase, K, acc);
+        else if (r8 == 2)
+            ConvChunk<MODE, TN, R2, CS, LDSW>(a, sX, sW, n, x0, y0, z0, cb, wave, row, grp, nbase, K, acc);
+        else
+            ConvChunk<MODE, TN, R3, CS, LDSW>(a, sX, sW, n, x0, y0, z0, cb, wave, row, grp, nbase, K, acc);
     }
     // epilogue: lane holds output rows 4*(lane/16)+r (x within the tile row), column lane%16
     const int z = z0 + wave;
@@ -678,6 +786,17 @@ __global__ __launch_bounds__(256) void KConvTile(const ConvArgs a) {
         }
 }
 
+template <int MODE, int TN>
+void LaunchConvTile(const ConvArgs& a, dim3 grid, hipStream_t s) {
+    // chunk width: 8 input channels -> one run, 16 or 24 -> two, otherwise four
+    if (a.kin <= 8)
+        hipLaunchKernelGGL((KConvTile<MODE, TN, 1>), grid, dim3(256), 0, s, a);
+    else if (a.kin <= 24)
+        hipLaunchKernelGGL((KConvTile<MODE, TN, 2>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((KConvTile<MODE, TN, 4>), grid, dim3(256), 0, s, a);
+}
+
 template <int MODE>
 void LaunchConv(const ConvArgs& a, hipStream_t s) {
     const long long V = static_cast<long long>(a.nb) * a.dd * a.hd * a.wd;
@@ -685,11 +804,11 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
         const unsigned tiles = static_cast<unsigned>(a.nb) * ((a.dd + kTZ - 1) / kTZ) * ((a.hd + kTY - 1) / kTY) *
                                ((a.wd + kTX - 1) / kTX);
         if (a.nout <= 16)
-            hipLaunchKernelGGL((KConvTile<MODE, 1>), dim3(tiles, 1), dim3(256), 0, s, a);
+            LaunchConvTile<MODE, 1>(a, dim3(tiles, 1), s);
         else if (a.nout <= 32)
-            hipLaunchKernelGGL((KConvTile<MODE, 2>), dim3(tiles, 1), dim3(256), 0, s, a);
+            LaunchConvTile<MODE, 2>(a, dim3(tiles, 1), s);
         else
-            hipLaunchKernelGGL((KConvTile<MODE, 4>), dim3(tiles, Blk(a.nout, 64)), dim3(256), 0, s, a);
+            LaunchConvTile<MODE, 4>(a, dim3(tiles, Blk(a.nout, 64)), s);
         return;
     }
     if (a.nout <= 16) {
