@@ -178,6 +178,96 @@ __global__ __launch_bounds__(256) void KConv(const ConvArgs a) {
         }
 }
 
+// Adjoint (data gradient) of a stride-2 convolution, split by output parity.  Along one
+// axis an even output coordinate c receives only tap 1 (from input c/2) and an odd one taps
+// 0 and 2 (from (c+1)/2 and (c-1)/2), so a voxel has 1 to 8 valid taps of 27 (3.4 on
+// average); KConv's gather visits all 27 and zeroes the rest.  blockIdx.y is the parity
+// class (pz, py, px); the class's voxels are enumerated at half resolution, so the rows of
+// every wave share a class and its K loop visits only the class's valid taps, 32 gathered
+// channels per step (kin % 32 == 0).  blockIdx.z selects 16*TN output channels.
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void KConvAdj2(const ConvArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int cls = blockIdx.y;
+    const int px = cls & 1, py = (cls >> 1) & 1, pz = cls >> 2;
+    const int wx = (a.wd - px + 1) / 2, wy = (a.hd - py + 1) / 2, wz = (a.dd - pz + 1) / 2;
+    const long long Vc = static_cast<long long>(a.nb) * wz * wy * wx;
+    const long long vbase = (static_cast<long long>(blockIdx.x) * 4 + wave) * (16 * TM);
+    if (vbase >= Vc) return;  // wave-uniform: classes with odd extents hold fewer voxels
+    const int nbase = blockIdx.z * (16 * TN);
+    const int K = 27 * a.kin;
+    const int row = lane & 15, kq = (lane >> 4) * 8;
+    // the output voxel of class index v
+    auto coords = [&](long long v, int& n, int& z, int& y, int& x) {
+        x = 2 * static_cast<int>(v % wx) + px;
+        long long r = v / wx;
+        y = 2 * static_cast<int>(r % wy) + py;
+        r /= wy;
+        z = 2 * static_cast<int>(r % wz) + pz;
+        n = static_cast<int>(r / wz);
+    };
+    int vn[TM], vz[TM], vy[TM], vx[TM];
+    bool vok[TM];
+#pragma unroll
+    for (int m = 0; m < TM; ++m) {
+        const long long v = vbase + m * 16 + row;
+        vok[m] = v < Vc;
+        coords(vok[m] ? v : 0, vn[m], vz[m], vy[m], vx[m]);
+    }
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int m = 0; m < TM; ++m)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nkx = px ? 2 : 1, nky = py ? 2 : 1, nkz = pz ? 2 : 1;
+    const int ntaps = nkx * nky * nkz, kblocks = a.kin / 32;
+    for (int ti = 0; ti < ntaps; ++ti) {
+        const int jx = ti % nkx, jy = (ti / nkx) % nky, jz = ti / (nkx * nky);
+        const int kx = px ? 2 * jx : 1, ky = py ? 2 * jy : 1, kz = pz ? 2 * jz : 1;
+        const int t = (kz * 3 + ky) * 3 + kx;
+        for (int kc = 0; kc < kblocks; ++kc) {
+            const int k0 = t * a.kin + kc * 32 + kq;
+            bf16x8 bfr[TN];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+                const int n = nbase + j * 16 + row;
+                if (n < a.nout) raw = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(n) * K + k0);
+                __builtin_memcpy(&bfr[j], &raw, 16);
+            }
+#pragma unroll
+            for (int m = 0; m < TM; ++m) {
+                const bf16x8 afr = GatherA<kAdj>(a, vn[m], vz[m], vy[m], vx[m], k0, vok[m]);
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+            }
+        }
+    }
+    // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+#pragma unroll
+    for (int m = 0; m < TM; ++m)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int nn = nbase + j * 16 + row;
+            if (nn >= a.nout) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long v = vbase + m * 16 + (lane >> 4) * 4 + r;
+                if (v >= Vc) continue;
+                int n, z, y, x;
+                coords(v, n, z, y, x);
+                const long long vo = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
+                const float val = acc[m][j][r];
+                if (a.outf)
+                    a.outf[vo * a.nout + nn] = val;
+                else
+                    a.out[vo * a.nout + nn] = F2Bf(val);
+            }
+        }
+}
+
 // Weight gradient of a kFwd/kUpcat conv: dW[co][k] = sum_v dZ[v][co] * A[v][k], split over
 // voxel ranges (blockIdx.z) into fp32 partials part[split][co][K].  32 voxels per step,
 // staged transposed in LDS so fragments are single 16-byte reads.
@@ -529,26 +619,33 @@ __host__ __device__ constexpr int HaloStride(int rmax) { return rmax >= 3 ? 8 * 
 // Stages the halo of channels [cb, cb + 8*R8) into sX: one 16-byte run per thread iteration
 // (zeros outside the volume; for upsample-concat the nearest-upsampled coarse channels and
 // the skip channels side by side).
+// The 8 gathered channels [ci, ci+8) of input voxel (n, iz, iy, ix) of a stride-1 gather
+// (zeros outside the volume; for upsample-concat the coarse channels at the half-resolution
+// voxel, the skip channels at the full-resolution one).
+template <int MODE>
+__device__ __forceinline__ uint4 InputRun(const ConvArgs& a, int n, int iz, int iy, int ix, int ci) {
+    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+    if (ix >= 0 && iy >= 0 && iz >= 0 && ix < a.wi && iy < a.hi && iz < a.di) {
+        if (MODE == kUpcat && ci < a.c1) {
+            const int h2 = a.hi >> 1, w2 = a.wi >> 1, d2 = a.di >> 1;
+            const long long vi = ((static_cast<long long>(n) * d2 + (iz >> 1)) * h2 + (iy >> 1)) * w2 + (ix >> 1);
+            raw = *reinterpret_cast<const uint4*>(a.in2 + vi * a.c1 + ci);
+        } else {
+            const int cs = MODE == kUpcat ? a.kin - a.c1 : a.kin;
+            const int c0 = MODE == kUpcat ? ci - a.c1 : ci;
+            const long long vi = ((static_cast<long long>(n) * a.di + iz) * a.hi + iy) * a.wi + ix;
+            raw = *reinterpret_cast<const uint4*>(a.in + vi * cs + c0);
+        }
+    }
+    return raw;
+}
+
 template <int MODE, int R8, int CS>
 __device__ __forceinline__ void StageHalo(const ConvArgs& a, u16* sX, int n, int x0, int y0, int z0, int cb) {
     for (int e = threadIdx.x; e < kHV * R8; e += 256) {
         const int hv = e / R8, r8 = e - hv * R8;
         const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-        const int ix = x0 - 1 + hx, iy = y0 - 1 + hy, iz = z0 - 1 + hz;
-        uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-        if (ix >= 0 && iy >= 0 && iz >= 0 && ix < a.wi && iy < a.hi && iz < a.di) {
-            const int ci = cb + r8 * 8;
-            if (MODE == kUpcat && ci < a.c1) {
-                const int h2 = a.hi >> 1, w2 = a.wi >> 1, d2 = a.di >> 1;
-                const long long vi = ((static_cast<long long>(n) * d2 + (iz >> 1)) * h2 + (iy >> 1)) * w2 + (ix >> 1);
-                raw = *reinterpret_cast<const uint4*>(a.in2 + vi * a.c1 + ci);
-            } else {
-                const int cs = MODE == kUpcat ? a.kin - a.c1 : a.kin;
-                const int c0 = MODE == kUpcat ? ci - a.c1 : ci;
-                const long long vi = ((static_cast<long long>(n) * a.di + iz) * a.hi + iy) * a.wi + ix;
-                raw = *reinterpret_cast<const uint4*>(a.in + vi * cs + c0);
-            }
-        }
+        const uint4 raw = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + hx, cb + r8 * 8);
         *reinterpret_cast<uint4*>(&sX[hv * CS + r8 * 8]) = raw;
     }
 }
@@ -797,6 +894,125 @@ void LaunchConvTile(const ConvArgs& a, dim3 grid, hipStream_t s) {
         hipLaunchKernelGGL((KConvTile<MODE, TN, 4>), grid, dim3(256), 0, s, a);
 }
 
+// Weight gradient of a stride-1 convolution (forward or upsample-concat), LDS-tiled:
+//   dW[co][tap][ci] = sum_v dZ[v][co] * X[v + off(tap)][ci]
+// as MFMAs with M = 16 output channels, N = 16 input channels, K = 32 voxels.  A workgroup
+// owns one chunk of 16 input channels and TCO output channels and walks its share of the
+// 16 x 4 x 4 voxel tiles (one split).  Per tile it stages the dZ tile transposed (sZ:
+// [co][voxel]) and the 18 x 6 x 6 input halo transposed, in three copies shifted by
+// kx = 0, 1, 2 along x (sX: [kx][ci][hz][hy][x]), so that the B fragment of any tap — 8
+// consecutive x-voxels of one channel — is one aligned 16-byte LDS read.  The dZ fragments
+// of the tile's 8 K blocks stay in registers for all 27 taps; wave w accumulates taps w,
+// w+4, ... over the whole split and writes them to the split's partial sums.
+template <int MODE, int TCO>
+__global__ __launch_bounds__(256) void KWgradTile(const WgradArgs w) {
+    constexpr int TI = TCO / 16;               // output-channel MFMA tiles
+    constexpr int CIS = kHZ * kHY * kTX + 8;   // one channel of one shifted copy (bf16), padded
+    constexpr int ZS = kTX * kTY * kTZ + 8;    // one output channel of the dZ tile, padded
+    constexpr int kTaps = 7;                   // taps per wave (27 over 4 waves)
+    __shared__ __attribute__((aligned(16))) u16 sX[3 * 16 * CIS];
+    __shared__ __attribute__((aligned(16))) u16 sZ[TCO * ZS];
+    const ConvArgs& a = w.g;
+    const int K = 27 * a.kin;
+    const int cb = blockIdx.x * 16, co0 = blockIdx.y * TCO;
+    const int runs = a.kin - cb >= 16 ? 2 : 1;  // 8-channel runs in this chunk
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, grp = lane >> 4;
+    const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + kTZ - 1) / kTZ;
+    const long long ntiles = static_cast<long long>(a.nb) * tiles_z * tiles_y * tiles_x;
+    const long long tps = (ntiles + gridDim.z - 1) / gridDim.z;
+    const long long t0 = static_cast<long long>(blockIdx.z) * tps;
+    const long long t1 = t0 + tps < ntiles ? t0 + tps : ntiles;
+    if (runs == 1)  // channels 8..15 of the chunk stay zero
+        for (int e = threadIdx.x; e < 3 * 16 * CIS / 8; e += 256)
+            reinterpret_cast<uint4*>(sX)[e] = make_uint4(0u, 0u, 0u, 0u);
+    f32x4 acc[kTaps][TI];
+#pragma unroll
+    for (int tt = 0; tt < kTaps; ++tt)
+#pragma unroll
+        for (int i = 0; i < TI; ++i) acc[tt][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long long tile = t0; tile < t1; ++tile) {
+        long long b = tile;
+        const int x0 = static_cast<int>(b % tiles_x) * kTX;
+        b /= tiles_x;
+        const int y0 = static_cast<int>(b % tiles_y) * kTY;
+        b /= tiles_y;
+        const int z0 = static_cast<int>(b % tiles_z) * kTZ;
+        const int n = static_cast<int>(b / tiles_z);
+        // input halo, transposed, three x-shifted copies
+        for (int e = threadIdx.x; e < kHV * runs; e += 256) {
+            const int hv = e / runs, r8 = e - hv * runs;
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const uint4 raw = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + hx, cb + r8 * 8);
+            u16 v8[8];
+            __builtin_memcpy(v8, &raw, 16);
+            const int off = (hz * kHY + hy) * kTX;
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const int xx = hx - kx;
+                if (xx < 0 || xx >= kTX) continue;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) sX[(kx * 16 + r8 * 8 + c) * CIS + off + xx] = v8[c];
+            }
+        }
+        // dZ tile, transposed (zeros outside the volume)
+        for (int e = threadIdx.x; e < kTX * kTY * kTZ * TI * 2; e += 256) {
+            const int v = e / (TI * 2), q = e - v * (TI * 2);
+            const int x = x0 + (v & (kTX - 1)), y = y0 + (v / kTX) % kTY, z = z0 + v / (kTX * kTY);
+            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+            if (x < a.wd && y < a.hd && z < a.dd && co0 + q * 8 < w.cout) {
+                const long long vg = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
+                raw = *reinterpret_cast<const uint4*>(w.dz + vg * w.cout + co0 + q * 8);
+            }
+            u16 v8[8];
+            __builtin_memcpy(v8, &raw, 16);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) sZ[(q * 8 + c) * ZS + v] = v8[c];
+        }
+        __syncthreads();
+        // K block kb: voxels 32kb..32kb+31 (tile rows 2kb, 2kb+1 of 16 x-voxels); lane
+        // group g holds voxels 32kb + 8g .. +7: row 2kb + g/2, x = 8 (g & 1) ..
+        bf16x8 af[TI][8];
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb)
+                __builtin_memcpy(&af[i][kb], &sZ[(i * 16 + row) * ZS + 32 * kb + 8 * grp], 16);
+#pragma unroll
+        for (int tt = 0; tt < kTaps; ++tt) {
+            const int t = wave + 4 * tt;
+            if (t >= 27) break;
+            const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+            const u16* xb = sX + (kx * 16 + row) * CIS + 8 * (grp & 1);
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb) {
+                const int rv = 2 * kb + (grp >> 1);  // tile row: z = rv / 4, y = rv % 4
+                bf16x8 bfr;
+                __builtin_memcpy(&bfr, xb + (((rv >> 2) + kz) * kHY + (rv & 3) + ky) * kTX, 16);
+#pragma unroll
+                for (int i = 0; i < TI; ++i)
+                    acc[tt][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][kb], bfr, acc[tt][i], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // partial sums of this split: lane holds rows co = 4*grp + r, column ci = row
+    float* part = w.part + static_cast<long long>(blockIdx.z) * w.cout * K;
+#pragma unroll
+    for (int tt = 0; tt < kTaps; ++tt) {
+        const int t = wave + 4 * tt;
+        if (t >= 27) break;
+        if (row >= runs * 8) continue;
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = co0 + i * 16 + 4 * grp + r;
+                if (co < w.cout) part[static_cast<long long>(co) * K + t * a.kin + cb + row] = acc[tt][i][r];
+            }
+    }
+}
+
 template <int MODE>
 void LaunchConv(const ConvArgs& a, hipStream_t s) {
     const long long V = static_cast<long long>(a.nb) * a.dd * a.hd * a.wd;
@@ -809,6 +1025,17 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
             LaunchConvTile<MODE, 2>(a, dim3(tiles, 1), s);
         else
             LaunchConvTile<MODE, 4>(a, dim3(tiles, Blk(a.nout, 64)), s);
+        return;
+    }
+    if (MODE == kAdj && a.stride == 2 && a.kin % 32 == 0) {
+        // class 0 (all coordinates even) is the largest parity class
+        const long long V0 = static_cast<long long>(a.nb) * ((a.dd + 1) / 2) * ((a.hd + 1) / 2) * ((a.wd + 1) / 2);
+        if (a.nout <= 16)
+            hipLaunchKernelGGL((KConvAdj2<4, 1>), dim3(Blk(V0, 256), 8, 1), dim3(256), 0, s, a);
+        else if (a.nout <= 32)
+            hipLaunchKernelGGL((KConvAdj2<2, 2>), dim3(Blk(V0, 128), 8, 1), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((KConvAdj2<2, 4>), dim3(Blk(V0, 128), 8, Blk(a.nout, 64)), dim3(256), 0, s, a);
         return;
     }
     if (a.nout <= 16) {
@@ -860,6 +1087,24 @@ int fc_nn_wgrad(int mode, const void* in, const void* in2, const void* dz, float
     w.part = ws;
     const int K = 27 * kin;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((mode == kUpcat || stride == 1) && (kin == 8 || kin % 16 == 0) && cout % 16 == 0) {
+        // stride 1: the LDS-tiled kernel
+        auto launch_tile = [&](auto tco) {
+            constexpr int TCO = decltype(tco)::value;
+            const dim3 grid(Blk(kin, 16), cout / TCO, splits);
+            if (mode == kUpcat)
+                hipLaunchKernelGGL((KWgradTile<kUpcat, TCO>), grid, dim3(256), 0, s, w);
+            else
+                hipLaunchKernelGGL((KWgradTile<kFwd, TCO>), grid, dim3(256), 0, s, w);
+        };
+        if (cout % 32 == 0)
+            launch_tile(std::integral_constant<int, 32>{});
+        else
+            launch_tile(std::integral_constant<int, 16>{});
+        const long long n = static_cast<long long>(cout) * K;
+        hipLaunchKernelGGL(KReduceSplits, dim3(Blk(n, kRedOut)), dim3(256), 0, s, ws, splits, n, dw, accumulate);
+        return static_cast<int>(hipGetLastError());
+    }
     auto launch = [&](auto tco) {
         constexpr int TCO = decltype(tco)::value;
         const dim3 grid(Blk(K, 4096 / TCO), Blk(cout, TCO), splits);

@@ -116,6 +116,27 @@ def test_conv_forward_and_adjoint(stride, cin, cout):
     assert rel(gw.cpu().numpy().reshape(cout, 27, cin), no.conv_wgrad(x, dy, stride)) < 1e-4
 
 
+@pytest.mark.parametrize("cin,cout", [(16, 32), (32, 64), (64, 128)])
+def test_stride2_adjoint_odd_dims(cin, cout):
+    """The parity-split adjoint of a stride-2 conv on odd extents, where the eight parity
+    classes differ in size and some output voxels receive a single tap per axis."""
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    rng = np.random.default_rng(cin * cout)
+    n, D, H, W = 2, 7, 5, 9
+    Do, Ho, Wo = no.out_dims(D, 2), no.out_dims(H, 2), no.out_dims(W, 2)
+    w = bf(rng.normal(size=(cout, 27, cin)) / np.sqrt(27 * cin))
+    dy = bf(rng.normal(size=(n, Do, Ho, Wo, cout)))
+    wt = t.empty(cout * 27 * cin, dtype=t.bfloat16, device="cuda")
+    assert L.fc_nn_transpose_w(ptr(dev_bf16(w)), ptr(wt), cout, cin, stream()) == 0
+    dxd = t.empty(n * D * H * W * cin, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_conv(2, ptr(dev_bf16(dy)), None, ptr(wt), None, None, ptr(dxd), n, Do, Ho, Wo, D, H, W, cout, 0,
+                      cin, 2, 0, stream())
+    assert rc == 0
+    assert rel(dxd.cpu().numpy().reshape(n, D, H, W, cin), no.conv_adj(dy, w, (n, D, H, W, cin), 2)) < 1e-4
+
+
 @pytest.mark.parametrize("c1,c2,cout", [(32, 16, 16), (64, 32, 32), (128, 64, 64)])
 def test_upcat_conv_and_split(c1, c2, cout):
     from flowcraft_amd.surrogate import nn_lib
