@@ -116,6 +116,23 @@ def test_conv_forward_and_adjoint(stride, cin, cout):
     assert rel(gw.cpu().numpy().reshape(cout, 27, cin), no.conv_wgrad(x, dy, stride)) < 1e-4
 
 
+def test_leaky_backward_exact():
+    """dZ = dY * leaky'(Y), rounded once to bf16 (round to nearest even): the 8-wide path
+    (n % 8 == 0) and the scalar path (n = 4095) against torch's own fp32 -> bf16 rounding."""
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    rng = np.random.default_rng(7)
+    N = 4096
+    y = dev_bf16(rng.normal(size=N))
+    dy = dev_bf16(rng.normal(size=N) * 3.0)
+    ref = t.where(y.float() > 0, dy.float(), dy.float() * np.float32(0.05)).to(t.bfloat16)
+    for n in (N, N - 1):
+        dz = t.zeros(N, dtype=t.bfloat16, device="cuda")
+        assert L.fc_nn_leaky_backward(ptr(y), ptr(dy), ptr(dz), n, stream()) == 0
+        assert t.equal(dz[:n].cpu(), ref[:n].cpu()), n
+
+
 @pytest.mark.parametrize("cin,cout", [(16, 32), (32, 64), (64, 128)])
 def test_stride2_adjoint_odd_dims(cin, cout):
     """The parity-split adjoint of a stride-2 conv on odd extents, where the eight parity
