@@ -1093,6 +1093,8 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
             LaunchConvTile<MODE, 1>(a, dim3(tiles, 1), s);
         else if (a.nout <= 32)
             LaunchConvTile<MODE, 2>(a, dim3(tiles, 1), s);
+        else if (a.nout <= 48)  // the upsample-concat adjoints (48 = 32 + 16 channels)
+            LaunchConvTile<MODE, 3>(a, dim3(tiles, 1), s);
         else
             LaunchConvTile<MODE, 4>(a, dim3(tiles, Blk(a.nout, 64)), s);
         return;
