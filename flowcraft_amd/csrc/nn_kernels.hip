@@ -680,6 +680,10 @@ unsigned Blk(long long n, int b) { return static_cast<unsigned>((n + b - 1) / b)
 #define FC_NN_CONV_TILED 1
 #endif
 constexpr bool kConvTiled = FC_NN_CONV_TILED;
+#ifndef FC_NN_CONV_PERSISTENT
+#define FC_NN_CONV_PERSISTENT 1
+#endif
+constexpr bool kConvPersistent = FC_NN_CONV_PERSISTENT;
 constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHV = kHX * kHY * kHZ;  // 648 halo voxels
 // LDS row stride (bf16) of a halo voxel for chunks of up to 8*RMAX channels, padded
@@ -813,9 +817,11 @@ __device__ __forceinline__ void ChunkGemmRuns(const ConvArgs& a, const u16* sX, 
 // three ky taps, and runs 12 MFMAs (acc[m] += A[m + oy(ky)] * B[ky]).  The chunk width is
 // a compile-time constant, so the combination -> (kz, kx, run) arithmetic is shifts and
 // constant divisions; the operands of step s+1 are loaded before the MFMAs of step s.
-template <int MODE, int R8, int CS, bool LDSW>
+// sW holds WR runs per tap ([tap][WR runs][16 output channels][8]); the chunk's runs
+// start at run wrun0 (WR = R8, wrun0 = 0 when the chunk's own weights are staged).
+template <int MODE, int R8, int CS, bool LDSW, int WR = R8>
 __device__ __forceinline__ void ChunkGemmRows16(const ConvArgs& a, const u16* sX, const u16* sW, int wave, int row,
-                                                int grp, int K, int cb, f32x4 (&acc)[4][1]) {
+                                                int grp, int K, int cb, f32x4 (&acc)[4][1], int wrun0 = 0) {
     constexpr int kCombos = 9 * R8;
     constexpr int kSteps = (kCombos + 3) / 4;
     const bool wok = row < a.nout;
@@ -832,7 +838,7 @@ __device__ __forceinline__ void ChunkGemmRows16(const ConvArgs& a, const u16* sX
             const int t = (kz * 3 + ky) * 3 + kx;
             bv[ky] = make_uint4(0u, 0u, 0u, 0u);
             if constexpr (LDSW) {
-                if (cok) bv[ky] = *reinterpret_cast<const uint4*>(&sW[((t * R8 + c8) * 16 + row) * 8]);
+                if (cok) bv[ky] = *reinterpret_cast<const uint4*>(&sW[((t * WR + wrun0 + c8) * 16 + row) * 8]);
             } else {
                 if (cok && wok) bv[ky] = *reinterpret_cast<const uint4*>(wrow + t * a.kin + c8 * 8);
             }
@@ -951,6 +957,138 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3
                     a.out[v * a.nout + nn] = F2Bf(val);
             }
         }
+}
+
+// Persistent variant for 16 output channels and NRUNS = kin/8 input runs (at most 6).  The
+// matrix loop of KConvTile waits on its own halo staging: the workgroups of a CU stage in
+// lockstep, so on d0 (48 -> 16 channels) staging and MFMA work added up (0.17 + 0.99 ms of
+// 1.84 ms, the rest lost between them).  Here a workgroup loops over tiles; each
+// 16-channel chunk's halo is loaded into registers while the previous chunk's MFMAs run,
+// then written to LDS between two barriers, so global-memory latency overlaps matrix work
+// (d0 1.84 -> 1.57 ms).
+template <int NRUNS>
+constexpr int PrefetchRegs() { return (kHV * (NRUNS < 2 ? NRUNS : 2) + 255) / 256; }
+
+// Weights in LDS for the 8-input layer (6.9 KB: e0 0.63 -> 0.59 ms); d0's 41.5 KB would cut a
+// CU to two workgroups, and its weights are read from L2 instead (1.65 -> 1.57 ms).
+template <int MODE, int NRUNS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void KConvTileP16(const ConvArgs a) {
+    constexpr int CS = HaloStride(2);
+    constexpr int kPF = PrefetchRegs<NRUNS>();
+    constexpr bool LDSW = NRUNS <= 2;
+    __shared__ __attribute__((aligned(16))) u16 sX[kHV * CS];
+    __shared__ __attribute__((aligned(16))) u16 sW[LDSW ? 27 * NRUNS * 16 * 8 : 8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, grp = lane >> 4;
+    const int K = 27 * a.kin;
+    const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + kTZ - 1) / kTZ;
+    const int ntiles = a.nb * tiles_z * tiles_y * tiles_x;
+    constexpr int kChunks = (NRUNS + 1) / 2;
+    int tile = blockIdx.x;
+    if (tile >= ntiles) return;  // whole workgroup, before any barrier
+    auto decode = [&](int t, int& n, int& x0, int& y0, int& z0) {
+        x0 = (t % tiles_x) * kTX;
+        t /= tiles_x;
+        y0 = (t % tiles_y) * kTY;
+        t /= tiles_y;
+        z0 = (t % tiles_z) * kTZ;
+        n = t / tiles_z;
+    };
+    // the layer's weights, once
+    for (int e = threadIdx.x; LDSW && e < 16 * 27 * NRUNS; e += 256) {
+        const int nn = e & 15, tc = e >> 4;
+        const int t = tc / NRUNS, r = tc - t * NRUNS;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (nn < a.nout) v = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(nn) * K + t * a.kin + r * 8);
+        *reinterpret_cast<uint4*>(&sW[e * 8]) = v;
+    }
+    uint4 pf[kPF];
+    auto prefetch = [&](int t, int c) {
+        int n, x0, y0, z0;
+        decode(t, n, x0, y0, z0);
+        const int runs = NRUNS - 2 * c < 2 ? NRUNS - 2 * c : 2;
+#pragma unroll
+        for (int j = 0; j < kPF; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            pf[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (e < kHV * runs) {
+                const int hv = e / runs, r8 = e - hv * runs;
+                const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+                pf[j] = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + hx, 16 * c + r8 * 8);
+            }
+        }
+    };
+    auto commit = [&](int c) {
+        const int runs = NRUNS - 2 * c < 2 ? NRUNS - 2 * c : 2;
+#pragma unroll
+        for (int j = 0; j < kPF; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            if (e < kHV * runs) {
+                const int hv = e / runs, r8 = e - hv * runs;
+                *reinterpret_cast<uint4*>(&sX[hv * CS + r8 * 8]) = pf[j];
+            }
+        }
+    };
+    prefetch(tile, 0);
+    f32x4 acc[4][1];
+    for (;;) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[m][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < kChunks; ++c) {
+            __syncthreads();  // the previous chunk's matrix loop is done with sX (and sW is staged)
+            commit(c);
+            __syncthreads();
+            // the next chunk's halo: this tile's next chunk, else the next tile's first
+            const int nt = c + 1 < kChunks ? tile : tile + static_cast<int>(gridDim.x);
+            if (nt < ntiles) prefetch(nt, c + 1 < kChunks ? c + 1 : 0);
+            if (NRUNS - 2 * c >= 2)
+                ChunkGemmRows16<MODE, 2, CS, LDSW, NRUNS>(a, sX, sW, wave, row, grp, K, 16 * c, acc, 2 * c);
+            else
+                ChunkGemmRows16<MODE, 1, CS, LDSW, NRUNS>(a, sX, sW, wave, row, grp, K, 16 * c, acc, 2 * c);
+        }
+        // epilogue: lane holds output rows 4*(lane/16)+r (x within the tile row), column lane%16
+        int n, x0, y0, z0;
+        decode(tile, n, x0, y0, z0);
+        const int z = z0 + wave;
+        if (row < a.nout && z < a.dd) {
+            const float bb = a.bias ? a.bias[row] : 0.f;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int y = y0 + m;
+                if (y >= a.hd) continue;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int x = x0 + grp * 4 + q;
+                    if (x >= a.wd) continue;
+                    const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
+                    float val = acc[m][0][q] + bb;
+                    if (a.act == 1) val = val > 0.f ? val : kLeak * val;
+                    if (a.outf)
+                        a.outf[v * a.nout + row] = val;
+                    else
+                        a.out[v * a.nout + row] = F2Bf(val);
+                }
+            }
+        }
+        tile += gridDim.x;
+        if (tile >= ntiles) break;
+    }
+}
+
+// Workgroups of a persistent launch: the CUs times the kernel's resident workgroups per CU.
+template <class K>
+unsigned PersistentGrid(K kernel, long long work) {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu <= 0) per_cu = 1;
+    const long long g = static_cast<long long>(cus) * per_cu;
+    return static_cast<unsigned>(work < g ? work : g);
 }
 
 template <int MODE, int TN>
@@ -1089,7 +1227,16 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
     if ((MODE == kUpcat || a.stride == 1) && kConvTiled) {
         const unsigned tiles = static_cast<unsigned>(a.nb) * ((a.dd + kTZ - 1) / kTZ) * ((a.hd + kTY - 1) / kTY) *
                                ((a.wd + kTX - 1) / kTX);
-        if (a.nout <= 16)
+        if (tiles == 0) return;
+        if (a.nout <= 16 && MODE != kAdj && (a.kin == 8 || a.kin == 48) && kConvPersistent) {
+            // the persistent kernel for the network's 16-channel layers (e0: 8 inputs, d0: 48)
+            if (a.kin == 8)
+                hipLaunchKernelGGL((KConvTileP16<MODE, 1>), dim3(PersistentGrid(KConvTileP16<MODE, 1>, tiles)),
+                                   dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((KConvTileP16<MODE, 6>), dim3(PersistentGrid(KConvTileP16<MODE, 6>, tiles)),
+                                   dim3(256), 0, s, a);
+        } else if (a.nout <= 16)
             LaunchConvTile<MODE, 1>(a, dim3(tiles, 1), s);
         else if (a.nout <= 32)
             LaunchConvTile<MODE, 2>(a, dim3(tiles, 1), s);
