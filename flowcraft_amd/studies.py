@@ -3,6 +3,7 @@
     python -m flowcraft_amd.studies precision [--steps 5000] [--every 50] [--window 1000]
     python -m flowcraft_amd.studies large [--steps 200] [--warmup 50]
     python -m flowcraft_amd.studies drain [--steps 20000] [--every 1000]
+    python -m flowcraft_amd.studies sensitivity [--t0 1000] [--steps 4000] [--eps 1e-9] [--tau 0.5005]
 
 precision — the 512x256x256 NACA 0012 wing (chord 128, leading edge at x = 128,
     angle of attack 4 deg, extruded over z; D3Q19 and D3Q15 Smagorinsky Cs = 0.1,
@@ -25,6 +26,9 @@ large — the 768x512x576 bandwidth lattice (226M cells): an ellipsoid of semi-a
     Smagorinsky Cs = 0.1, u = 0.06, tau = 0.501, free-slip y/z; `warmup` untimed
     steps, then `steps` steps timed with HIP events on the solver's stream.  FP32 in
     place (one 17 GB population copy) and FP16 offsets.
+
+sensitivity — whether the developed wing flow amplifies a small local perturbation (and
+    FP32 rounding) in FP64: sensitivity_study.
 
 drain — the precision case's D3Q15 run (tau 0.6) from rest in FP32 and FP16: mean
     density, mean speed and Cd every 1000 steps, showing how the channel's excess mass
@@ -90,10 +94,11 @@ def wing_grid(nx=512, ny=256, nz=256):
                                    aoa_deg=4.0)
 
 
-# A control run for chaos: FP64 with the inlet speed raised by this fraction (a change the
-# size of FP32 rounding, acting at every step; the coefficients stay normalised by the
-# nominal speed, which moves them by 2e-6).  Where the flow is chaotic this run leaves the
-# unperturbed FP64 run as far as the storage formats do; where it is not, it stays with it.
+# A control run: FP64 with the inlet speed raised by this fraction (a change the size of
+# FP32 rounding, acting at every step; the coefficients stay normalised by the nominal
+# speed, which moves them by 2e-6).  If the flow amplified small perturbations this run
+# would leave the unperturbed FP64 run as far as the storage formats do; on the wing it
+# stays with it (Cd differences of 1e-8 to 2e-6 over 5000 steps at tau 0.5005).
 # (Noise added to the rest state alone is not enough: it leaves as acoustic waves before
 # the wake becomes unsteady.)
 PERTURBATION = 1e-6
@@ -219,6 +224,60 @@ def density_drain(steps=20000, every=1000, model="D3Q15", tau=TAU_D3Q15, storage
     return out
 
 
+def sensitivity_study(t0=1000, steps=4000, every=50, eps=1e-9, tau=0.5005, dims=(512, 256, 256), log=print):
+    """How the developed wing flow answers a small local perturbation, in FP64.  From rest the
+    FP64 engine runs t0 steps; the exact state is dumped and continued `steps` steps three
+    ways: unperturbed; with the populations of a 4x4x4-cell box in the wake (half a chord
+    behind the trailing edge, at mid-height and mid-span) scaled by (1 + eps); and in FP32
+    from the same state (rounded once).  Reports Cd/Cl of each and their relative distance
+    from the unperturbed run over time: a distance that grows exponentially means the flow
+    amplifies small differences on this horizon (chaos); one that stays near its initial
+    size means it does not."""
+    import torch
+    nx, ny, nz = dims
+    grid = wing_grid(nx, ny, nz)
+    u, chord = 0.08, nx / 4.0
+    q = 0.5 * u * u * chord * nz
+    params = Params(tau=tau, inletVelocity=u, smagorinskyCs=0.1)
+    s = _solver(grid, LatticeModel.D3Q19, Storage.F64, Layout.PINGPONG)
+    s.Reset(params, False)
+    s.Step(None, t0)
+    s.Sync()
+    pops = s.Populations()
+    s.Shutdown()
+    x0 = int(nx / 4 + chord * 1.5)
+    box = (slice(None), slice(nz // 2 - 2, nz // 2 + 2), slice(ny // 2 - 2, ny // 2 + 2), slice(x0, x0 + 4))
+    runs = {}
+    for name, st in (("base", Storage.F64), ("perturbed", Storage.F64), ("f32", Storage.F32)):
+        r = _solver(grid, LatticeModel.D3Q19, st, Layout.PINGPONG)
+        r.Reset(params, False)
+        if name == "perturbed":
+            p2 = pops.copy()
+            p2[box] *= 1.0 + eps
+            r.SetPopulations(p2)
+            del p2
+        else:
+            r.SetPopulations(pops)
+        f = force_series(r, steps, every)
+        r.Shutdown()
+        runs[name] = (f[:, 0] / q, f[:, 1] / q)
+        log(f"  {name}: Cd {runs[name][0][-1]:.6f} Cl {runs[name][1][-1]:.6f}")
+    base_cd, base_cl = runs["base"]
+    out = {"workload": f"{nx}x{ny}x{nz} NACA 0012 wing, D3Q19 FP64, tau {tau}, Cs 0.1, u 0.08; state after {t0} "
+                       f"steps from rest, continued {steps} steps", "eps": eps, "every": every,
+           "box": [x0, ny // 2 - 2, nz // 2 - 2, 4], "cd_base": [float(v) for v in base_cd], "series": {}}
+    for name in ("perturbed", "f32"):
+        cd, cl = runs[name]
+        dcd = np.abs(cd - base_cd) / np.abs(base_cd)
+        dcl = np.abs(cl - base_cl) / np.abs(base_cl)
+        out["series"][name] = {"rel_cd": [float(v) for v in dcd], "rel_cl": [float(v) for v in dcl]}
+        k = len(dcd)
+        marks = {str((i + 1) * every): [float(dcd[i]), float(dcl[i])] for i in (0, k // 8, k // 4, k // 2, k - 1)}
+        out["series"][name]["marks"] = marks
+        log(f"  {name}: relative |dCd|, |dCl| at steps " + ", ".join(f"{m}: {v[0]:.2e} {v[1]:.2e}" for m, v in marks.items()))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m flowcraft_amd.studies")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -238,6 +297,14 @@ def main(argv=None):
     g.add_argument("--storages", nargs="+", default=["f32", "f16"])
     g.add_argument("--layout", default="inplace", choices=["inplace", "pingpong"])
     g.add_argument("--out", default=None)
+    sv = sub.add_parser("sensitivity")
+    sv.add_argument("--t0", type=int, default=1000)
+    sv.add_argument("--steps", type=int, default=4000)
+    sv.add_argument("--every", type=int, default=50)
+    sv.add_argument("--eps", type=float, default=1e-9)
+    sv.add_argument("--tau", type=float, default=0.5005)
+    sv.add_argument("--dims", type=int, nargs=3, default=[512, 256, 256])
+    sv.add_argument("--out", default=None)
     d = sub.add_parser("drain")
     d.add_argument("--steps", type=int, default=20000)
     d.add_argument("--every", type=int, default=1000)
@@ -253,6 +320,8 @@ def main(argv=None):
     if a.cmd == "precision":
         r = precision_study(a.steps, a.every, a.window, tuple(a.models), tuple(a.storages), log=log, tau=a.tau,
                             tau15=a.tau15, dims=tuple(a.dims))
+    elif a.cmd == "sensitivity":
+        r = sensitivity_study(a.t0, a.steps, a.every, a.eps, a.tau, tuple(a.dims), log=log)
     elif a.cmd == "drain":
         r = density_drain(a.steps, a.every, storages=tuple(a.storages), log=log)
     else:

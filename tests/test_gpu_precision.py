@@ -2,8 +2,8 @@
 
 Workload: the benchmark wing scaled to 256x128x128 (NACA 0012, chord 64, leading edge at
 x = 64, angle of attack 4 deg, extruded over z; Cs 0.1, u 0.08, free-slip y/z walls), tau
-0.6, run from the uniform rest start for 20000 steps.  At tau 0.6 the flow is not chaotic:
-FP32 stays within 0.05% of FP64, so a storage format's distance from FP64 is its own.
+0.6, run from the uniform rest start for 20000 steps.  At tau 0.6 FP32 stays within 0.05%
+of FP64, so a storage format's distance from FP64 is its own.
 Cd = Fx / (0.5 u^2 A), Cl = Fy / (0.5 u^2 A), A = chord * nz, F the FP64 momentum-exchange
 sum (LbmReference::SolidCellForce convention, FlowCraft/LbmReference.cpp:127-148), as the
 mean over the trailing 10000 steps (sampled every 50).
