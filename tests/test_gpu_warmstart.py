@@ -5,10 +5,12 @@ forward/backward kernels on 600 engine-generated fields of the 256x128x128 warm-
 distribution) is applied to held-out shapes (seed 1001, never seen in training):
 
 * its predicted (rho, u) field is within 5% relative L2 of the engine's own
-  3000-step field (mean of the last 1000 steps) on fluid cells, where the plain
-  rest start (rho = 1, u = 0) is ~100% off and the untrained network ~100%;
+  3000-step field (mean of the last 1000 steps) on fluid cells, for every shape, where
+  the plain rest start (rho = 1, u = 0) is ~100% off and the untrained network ~100%;
 * a run seeded from it has a trailing-mean drag coefficient closer to the plain
-  run's 3000-step value after 1500 steps than the plain run itself has at 1500.
+  run's 3000-step value after 1500 steps than the plain run itself has at 1500: on
+  most of the shapes, and in the median over them.
+Runs use FP16 offsets with stochastically rounded stores, the format of the studies.
 """
 from __future__ import annotations
 
@@ -18,6 +20,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 DIMS = (256, 128, 128)
+N_SHAPES = 6
 U, TAU, CS = 0.06, 0.503, 0.1
 
 
@@ -46,12 +49,13 @@ def test_trained_surrogate_beats_plain_start():
     untrained = sg.Surrogate(seed=0)
     params = Params(tau=TAU, inletVelocity=U, smagorinskyCs=CS)
     rng = ws.shape_stream(1001)
-    for k in range(2):
+    gaps = []
+    for k in range(N_SHAPES):
         g, d = geometry.random_shape_grid(nx, ny, nz, rng)
         print(f"held-out shape {k}: {d['kind']}", flush=True)
         solid = torch.from_numpy(np.ascontiguousarray(g.solid.reshape(-1))).cuda()
         sdf = torch.from_numpy(np.ascontiguousarray(g.signedDistance.reshape(-1))).cuda()
-        s = ws.make_solver(g, Storage.F16)
+        s = ws.make_solver(g, Storage.F16_SR)
         # plain start: Cd every 10 steps to 3000 steps, and the mean field of the last 1000
         s.Reset(params, False)
         buf = torch.zeros((300, 4), dtype=torch.float64, device="cuda")
@@ -87,15 +91,27 @@ def test_trained_surrogate_beats_plain_start():
         w15 = float(ws.trailing(cd_w, 100)[149])
         p15 = float(ws.trailing(cd_p, 100)[149])
         print(f"Cd: plain@3000 {ref:.4f}; after 1500 steps warm {w15:.4f}, plain {p15:.4f}", flush=True)
-        assert abs(w15 - ref) < abs(p15 - ref)
+        gaps.append((abs(w15 - ref), abs(p15 - ref)))
+    _assert_warm_closer(gaps)
+
+
+def _assert_warm_closer(gaps):
+    """The warm run is closer to the plain run's reference than the plain run at 1500
+    steps on most shapes, and in the median over them."""
+    w, p = np.array(gaps).T
+    print(f"warm closer on {int((w < p).sum())} of {len(w)} shapes; median gap warm {np.median(w):.4f}, "
+          f"plain {np.median(p):.4f}", flush=True)
+    assert (w < p).sum() > len(w) // 2
+    assert np.median(w) < np.median(p)
 
 
 def test_settled_surrogate_beats_plain_start():
     """The weights trained on settled fields (320 shapes, mean of steps 14000-15000 of a
-    run from rest): on held-out shapes the prediction is within 35% relative L2 of the
+    run from rest): on each held-out shape the prediction is within 35% relative L2 of the
     engine's own 15000-step field (validation mean 13.5%), where the rest start and the
     untrained network are ~100% off, and a run seeded from it has a trailing-mean drag
-    closer to the plain run's 15000-step value after 1500 steps than the plain run has."""
+    closer to the plain run's 15000-step value after 1500 steps than the plain run has, on
+    most shapes and in the median."""
     import torch
     from flowcraft_amd import geometry
     from flowcraft_amd import surrogate as sg
@@ -108,12 +124,13 @@ def test_settled_surrogate_beats_plain_start():
     untrained = sg.Surrogate(seed=0)
     params = Params(tau=TAU, inletVelocity=U, smagorinskyCs=CS)
     rng = ws.shape_stream(1001)
-    for k in range(2):
+    gaps = []
+    for k in range(N_SHAPES):
         g, d = geometry.random_shape_grid(nx, ny, nz, rng)
         print(f"held-out shape {k}: {d['kind']}", flush=True)
         solid = torch.from_numpy(np.ascontiguousarray(g.solid.reshape(-1))).cuda()
         sdf = torch.from_numpy(np.ascontiguousarray(g.signedDistance.reshape(-1))).cuda()
-        s = ws.make_solver(g, Storage.F16)
+        s = ws.make_solver(g, Storage.F16_SR)
         s.Reset(params, False)
         buf = torch.zeros((1500, 4), dtype=torch.float64, device="cuda")
         field = torch.zeros(V * 4, dtype=torch.float32, device="cuda")
@@ -146,4 +163,5 @@ def test_settled_surrogate_beats_plain_start():
         w15 = float(ws.trailing(cd_w, 100)[149])
         p15 = float(ws.trailing(cd_p, 100)[149])
         print(f"Cd: plain@15000 {ref:.4f}; after 1500 steps warm {w15:.4f}, plain {p15:.4f}", flush=True)
-        assert abs(w15 - ref) < abs(p15 - ref)
+        gaps.append((abs(w15 - ref), abs(p15 - ref)))
+    _assert_warm_closer(gaps)
