@@ -877,6 +877,69 @@ __device__ __forceinline__ void ChunkGemmRows16(const ConvArgs& a, const u16* sX
     }
 }
 
+// ChunkGemmRows16 for a wave that owns ZW z slices of the tile (halo depth 4*ZW + 2): each
+// weight fragment feeds 4*ZW MFMAs instead of 4.  The weights of step s+1 are prefetched;
+// the halo fragments (LDS, short latency) are loaded at the start of their step.
+template <int MODE, int R8, int CS, bool LDSW, int WR, int ZW>
+__device__ __forceinline__ void ChunkGemmRows16Z(const ConvArgs& a, const u16* sX, const u16* sW, int wave, int row,
+                                                 int grp, int K, int cb, f32x4 (&acc)[ZW][4], int wrun0) {
+    constexpr int kCombos = 9 * R8;
+    constexpr int kSteps = (kCombos + 3) / 4;
+    const bool wok = row < a.nout;
+    const u16* wrow = a.w + static_cast<long long>(wok ? row : 0) * K + cb;
+    const int lbase = (wave * ZW * kHY * kHX + row) * CS;  // the lane's halo row at offset (0,0,0) of slice 0
+    auto loadw = [&](int s, uint4 (&bv)[3]) {
+        const int c = 4 * s + grp;
+        const bool cok = c < kCombos;
+        const int kzx = c / R8, c8 = c - kzx * R8;
+        const int kz = kzx / 3, kx = kzx - kz * 3;
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+            const int t = (kz * 3 + ky) * 3 + kx;
+            bv[ky] = make_uint4(0u, 0u, 0u, 0u);
+            if constexpr (LDSW) {
+                if (cok) bv[ky] = *reinterpret_cast<const uint4*>(&sW[((t * WR + wrun0 + c8) * 16 + row) * 8]);
+            } else {
+                if (cok && wok) bv[ky] = *reinterpret_cast<const uint4*>(wrow + t * a.kin + c8 * 8);
+            }
+        }
+    };
+    uint4 bcur[3];
+    loadw(0, bcur);
+#pragma unroll 1
+    for (int s = 0; s < kSteps; ++s) {
+        uint4 bnext[3];
+        if (s + 1 < kSteps) loadw(s + 1, bnext);  // issued ahead of this step's MFMAs
+        const int c = 4 * s + grp;
+        const bool cok = c < kCombos;
+        const int kzx = c / R8, c8 = c - kzx * R8;
+        const int kz = kzx / 3, kx = kzx - kz * 3;
+        const int oz = MODE == kAdj ? 2 - kz : kz, ox = MODE == kAdj ? 2 - kx : kx;
+        const int base = lbase + (oz * kHY * kHX + ox) * CS + c8 * 8;
+#pragma unroll
+        for (int j = 0; j < ZW; ++j) {
+            bf16x8 afr[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (cok) v = *reinterpret_cast<const uint4*>(&sX[base + (j * kHY * kHX + r * kHX) * CS]);
+                __builtin_memcpy(&afr[r], &v, 16);
+            }
+#pragma unroll
+            for (int ky = 0; ky < 3; ++ky) {
+                const int oy = MODE == kAdj ? 2 - ky : ky;
+                bf16x8 bfr;
+                __builtin_memcpy(&bfr, &bcur[ky], 16);
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    acc[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[m + oy], bfr, acc[j][m], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) bcur[ky] = bnext[ky];
+    }
+}
+
 // Weights staged in LDS: 16 output channels and chunks of at most 16 channels.
 __host__ __device__ constexpr bool LdsWeights(int tn, int rmax) { return tn == 1 && rmax <= 2; }
 
@@ -966,22 +1029,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3
 // 16-channel chunk's halo is loaded into registers while the previous chunk's MFMAs run,
 // then written to LDS between two barriers, so global-memory latency overlaps matrix work
 // (d0 1.84 -> 1.57 ms).
-template <int NRUNS>
-constexpr int PrefetchRegs() { return (kHV * (NRUNS < 2 ? NRUNS : 2) + 255) / 256; }
+template <int NRUNS, int HV>
+constexpr int PrefetchRegs() { return (HV * (NRUNS < 2 ? NRUNS : 2) + 255) / 256; }
 
 // Weights in LDS for the 8-input layer (6.9 KB: e0 0.63 -> 0.59 ms); d0's 41.5 KB would cut a
 // CU to two workgroups, and its weights are read from L2 instead (1.65 -> 1.57 ms).
-template <int MODE, int NRUNS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void KConvTileP16(const ConvArgs a) {
+// d0 (NRUNS = 6): each wave owns two z slices of a 16x4x8 tile (ChunkGemmRows16Z), so each
+// weight fragment feeds 8 MFMAs; its registers allow two waves per SIMD (1.57 -> 1.49 ms;
+// held to three, it spills and takes 1.62 ms).
+template <int MODE, int NRUNS, int ZW = (NRUNS > 2 ? 2 : 1)>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZW == 1 ? 3 : 2))) void KConvTileP16(const ConvArgs a) {
     constexpr int CS = HaloStride(2);
-    constexpr int kPF = PrefetchRegs<NRUNS>();
+    constexpr int TZ = 4 * ZW, HZ = TZ + 2, HV = kHX * kHY * HZ;  // the tile's z extent and its halo
+    constexpr int kPF = PrefetchRegs<NRUNS, HV>();
     constexpr bool LDSW = NRUNS <= 2;
-    __shared__ __attribute__((aligned(16))) u16 sX[kHV * CS];
+    __shared__ __attribute__((aligned(16))) u16 sX[HV * CS];
     __shared__ __attribute__((aligned(16))) u16 sW[LDSW ? 27 * NRUNS * 16 * 8 : 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, grp = lane >> 4;
     const int K = 27 * a.kin;
-    const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + kTZ - 1) / kTZ;
+    const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + TZ - 1) / TZ;
     const int ntiles = a.nb * tiles_z * tiles_y * tiles_x;
     constexpr int kChunks = (NRUNS + 1) / 2;
     int tile = blockIdx.x;
@@ -991,7 +1058,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void K
         t /= tiles_x;
         y0 = (t % tiles_y) * kTY;
         t /= tiles_y;
-        z0 = (t % tiles_z) * kTZ;
+        z0 = (t % tiles_z) * TZ;
         n = t / tiles_z;
     };
     // the layer's weights, once
@@ -1011,7 +1078,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void K
         for (int j = 0; j < kPF; ++j) {
             const int e = threadIdx.x + 256 * j;
             pf[j] = make_uint4(0u, 0u, 0u, 0u);
-            if (e < kHV * runs) {
+            if (e < HV * runs) {
                 const int hv = e / runs, r8 = e - hv * runs;
                 const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
                 pf[j] = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + hx, 16 * c + r8 * 8);
@@ -1023,17 +1090,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void K
 #pragma unroll
         for (int j = 0; j < kPF; ++j) {
             const int e = threadIdx.x + 256 * j;
-            if (e < kHV * runs) {
+            if (e < HV * runs) {
                 const int hv = e / runs, r8 = e - hv * runs;
                 *reinterpret_cast<uint4*>(&sX[hv * CS + r8 * 8]) = pf[j];
             }
         }
     };
     prefetch(tile, 0);
-    f32x4 acc[4][1];
+    f32x4 acc[ZW][4];
     for (;;) {
 #pragma unroll
-        for (int m = 0; m < 4; ++m) acc[m][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < ZW; ++j)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) acc[j][m] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int c = 0; c < kChunks; ++c) {
             __syncthreads();  // the previous chunk's matrix loop is done with sX (and sW is staged)
             commit(c);
@@ -1041,16 +1110,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void K
             // the next chunk's halo: this tile's next chunk, else the next tile's first
             const int nt = c + 1 < kChunks ? tile : tile + static_cast<int>(gridDim.x);
             if (nt < ntiles) prefetch(nt, c + 1 < kChunks ? c + 1 : 0);
-            if (NRUNS - 2 * c >= 2)
-                ChunkGemmRows16<MODE, 2, CS, LDSW, NRUNS>(a, sX, sW, wave, row, grp, K, 16 * c, acc, 2 * c);
-            else
-                ChunkGemmRows16<MODE, 1, CS, LDSW, NRUNS>(a, sX, sW, wave, row, grp, K, 16 * c, acc, 2 * c);
+            if constexpr (ZW == 1) {
+                f32x4(&acc1)[4][1] = reinterpret_cast<f32x4(&)[4][1]>(acc[0]);
+                if (NRUNS - 2 * c >= 2)
+                    ChunkGemmRows16<MODE, 2, CS, LDSW, NRUNS>(a, sX, sW, wave, row, grp, K, 16 * c, acc1, 2 * c);
+                else
+                    ChunkGemmRows16<MODE, 1, CS, LDSW, NRUNS>(a, sX, sW, wave, row, grp, K, 16 * c, acc1, 2 * c);
+            } else {
+                if (NRUNS - 2 * c >= 2)
+                    ChunkGemmRows16Z<MODE, 2, CS, LDSW, NRUNS, ZW>(a, sX, sW, wave, row, grp, K, 16 * c, acc, 2 * c);
+                else
+                    ChunkGemmRows16Z<MODE, 1, CS, LDSW, NRUNS, ZW>(a, sX, sW, wave, row, grp, K, 16 * c, acc, 2 * c);
+            }
         }
         // epilogue: lane holds output rows 4*(lane/16)+r (x within the tile row), column lane%16
         int n, x0, y0, z0;
         decode(tile, n, x0, y0, z0);
-        const int z = z0 + wave;
-        if (row < a.nout && z < a.dd) {
+#pragma unroll
+        for (int j = 0; j < ZW; ++j) {
+            const int z = z0 + wave * ZW + j;
+            if (row >= a.nout || z >= a.dd) continue;
             const float bb = a.bias ? a.bias[row] : 0.f;
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
@@ -1061,7 +1140,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void K
                     const int x = x0 + grp * 4 + q;
                     if (x >= a.wd) continue;
                     const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
-                    float val = acc[m][0][q] + bb;
+                    float val = acc[j][m][q] + bb;
                     if (a.act == 1) val = val > 0.f ? val : kLeak * val;
                     if (a.outf)
                         a.outf[v * a.nout + row] = val;
@@ -1235,7 +1314,7 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
                                    dim3(256), 0, s, a);
             else
                 hipLaunchKernelGGL((KConvTileP16<MODE, 6>), dim3(PersistentGrid(KConvTileP16<MODE, 6>, tiles)),
-                                   dim3(256), 0, s, a);
+                                   dim3(256), 0, s, a);  // at most `tiles` workgroups: enough for any ZW
         } else if (a.nout <= 16)
             LaunchConvTile<MODE, 1>(a, dim3(tiles, 1), s);
         else if (a.nout <= 32)

@@ -154,6 +154,29 @@ def test_stride2_adjoint_odd_dims(cin, cout):
     assert rel(dxd.cpu().numpy().reshape(n, D, H, W, cin), no.conv_adj(dy, w, (n, D, H, W, cin), 2)) < 1e-4
 
 
+@pytest.mark.parametrize("dims", [(1, 10, 4, 6), (2, 6, 6, 18)])
+def test_upcat_conv_ragged_tiles(dims):
+    """The 48 -> 16 upsample-concat layer (d0's shape) on extents that leave ragged tiles:
+    D = 10 and 6 against an 8-deep tile, W = 18 against a 16-wide one."""
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    n, D, H, W = dims
+    c1, c2, cout = 32, 16, 16
+    rng = np.random.default_rng(D * W)
+    xc = bf(rng.normal(size=(n, D // 2, H // 2, W // 2, c1)))
+    xs = bf(rng.normal(size=(n, D, H, W, c2)))
+    kin = c1 + c2
+    w = bf(rng.normal(size=(cout, 27, kin)) / np.sqrt(27 * kin))
+    b = rng.normal(size=cout).astype(np.float32).astype(np.float64)
+    ref = no.conv_fwd(np.concatenate([no.up2(xc), xs], axis=-1), w, b, 1)
+    yd = t.empty(n * D * H * W * cout, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_conv(1, ptr(dev_bf16(xs)), ptr(dev_bf16(xc)), ptr(dev_bf16(w)), ptr(dev_f32(b)), None, ptr(yd), n, D,
+                      H, W, D, H, W, kin, c1, cout, 1, 0, stream())
+    assert rc == 0
+    assert rel(yd.cpu().numpy().reshape(ref.shape), ref) < 1e-4
+
+
 @pytest.mark.parametrize("c1,c2,cout", [(32, 16, 16), (64, 32, 32), (128, 64, 64)])
 def test_upcat_conv_and_split(c1, c2, cout):
     from flowcraft_amd.surrogate import nn_lib
