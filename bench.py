@@ -623,7 +623,7 @@ def main():
     fp64_err = None if args.no_variants else fp64_error_block(torch, grid)
     # the precision target on a flow where FP32 tracks FP64: the same wing at tau 0.6, where FP32
     # stays within 0.03% of FP64, over 20000 steps (trailing 10000-step means)
-    fp64_judged = None if args.no_variants else fp64_error_block(
+    fp64_tau06 = None if args.no_variants else fp64_error_block(
         torch, grid, steps=20000, window=10000, tau=0.6, storages=("F32", "F16", "F16_SR", "BF16_SR"))
     warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
     warm_settled = (warm_start_block(torch, args.warm_shapes_settled, settled=True)
@@ -638,7 +638,7 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "error_vs_fp64": fp64_err, "error_vs_fp64_tau06": fp64_judged, "warm_start": warm, "warm_start_settled": warm_settled,
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "error_vs_fp64": fp64_err, "error_vs_fp64_tau06": fp64_tau06, "warm_start": warm, "warm_start_settled": warm_settled,
         "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)
