@@ -877,69 +877,6 @@ __device__ __forceinline__ void ChunkGemmRows16(const ConvArgs& a, const u16* sX
     }
 }
 
-// ChunkGemmRows16 for a wave that owns ZW z slices of the tile (halo depth 4*ZW + 2): each
-// weight fragment feeds 4*ZW MFMAs instead of 4.  The weights of step s+1 are prefetched;
-// the halo fragments (LDS, short latency) are loaded at the start of their step.
-template <int MODE, int R8, int CS, bool LDSW, int WR, int ZW>
-__device__ __forceinline__ void ChunkGemmRows16Z(const ConvArgs& a, const u16* sX, const u16* sW, int wave, int row,
-                                                 int grp, int K, int cb, f32x4 (&acc)[ZW][4], int wrun0) {
-    constexpr int kCombos = 9 * R8;
-    constexpr int kSteps = (kCombos + 3) / 4;
-    const bool wok = row < a.nout;
-    const u16* wrow = a.w + static_cast<long long>(wok ? row : 0) * K + cb;
-    const int lbase = (wave * ZW * kHY * kHX + row) * CS;  // the lane's halo row at offset (0,0,0) of slice 0
-    auto loadw = [&](int s, uint4 (&bv)[3]) {
-        const int c = 4 * s + grp;
-        const bool cok = c < kCombos;
-        const int kzx = c / R8, c8 = c - kzx * R8;
-        const int kz = kzx / 3, kx = kzx - kz * 3;
-#pragma unroll
-        for (int ky = 0; ky < 3; ++ky) {
-            const int t = (kz * 3 + ky) * 3 + kx;
-            bv[ky] = make_uint4(0u, 0u, 0u, 0u);
-            if constexpr (LDSW) {
-                if (cok) bv[ky] = *reinterpret_cast<const uint4*>(&sW[((t * WR + wrun0 + c8) * 16 + row) * 8]);
-            } else {
-                if (cok && wok) bv[ky] = *reinterpret_cast<const uint4*>(wrow + t * a.kin + c8 * 8);
-            }
-        }
-    };
-    uint4 bcur[3];
-    loadw(0, bcur);
-#pragma unroll 1
-    for (int s = 0; s < kSteps; ++s) {
-        uint4 bnext[3];
-        if (s + 1 < kSteps) loadw(s + 1, bnext);  // issued ahead of this step's MFMAs
-        const int c = 4 * s + grp;
-        const bool cok = c < kCombos;
-        const int kzx = c / R8, c8 = c - kzx * R8;
-        const int kz = kzx / 3, kx = kzx - kz * 3;
-        const int oz = MODE == kAdj ? 2 - kz : kz, ox = MODE == kAdj ? 2 - kx : kx;
-        const int base = lbase + (oz * kHY * kHX + ox) * CS + c8 * 8;
-#pragma unroll
-        for (int j = 0; j < ZW; ++j) {
-            bf16x8 afr[6];
-#pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                if (cok) v = *reinterpret_cast<const uint4*>(&sX[base + (j * kHY * kHX + r * kHX) * CS]);
-                __builtin_memcpy(&afr[r], &v, 16);
-            }
-#pragma unroll
-            for (int ky = 0; ky < 3; ++ky) {
-                const int oy = MODE == kAdj ? 2 - ky : ky;
-                bf16x8 bfr;
-                __builtin_memcpy(&bfr, &bcur[ky], 16);
-#pragma unroll
-                for (int m = 0; m < 4; ++m)
-                    acc[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[m + oy], bfr, acc[j][m], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int ky = 0; ky < 3; ++ky) bcur[ky] = bnext[ky];
-    }
-}
-
 // Weights staged in LDS: 16 output channels and chunks of at most 16 channels.
 __host__ __device__ constexpr bool LdsWeights(int tn, int rmax) { return tn == 1 && rmax <= 2; }
 
@@ -1022,35 +959,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3
         }
 }
 
-// Persistent variant for 16 output channels and NRUNS = kin/8 input runs (at most 6).  The
-// matrix loop of KConvTile waits on its own halo staging: the workgroups of a CU stage in
-// lockstep, so on d0 (48 -> 16 channels) staging and MFMA work added up (0.17 + 0.99 ms of
-// 1.84 ms, the rest lost between them).  Here a workgroup loops over tiles; each
-// 16-channel chunk's halo is loaded into registers while the previous chunk's MFMAs run,
-// then written to LDS between two barriers, so global-memory latency overlaps matrix work
-// (d0 1.84 -> 1.57 ms).
-template <int NRUNS, int HV>
-constexpr int PrefetchRegs() { return (HV * (NRUNS < 2 ? NRUNS : 2) + 255) / 256; }
-
-// Weights in LDS for the 8-input layer (6.9 KB: e0 0.63 -> 0.59 ms); d0's 41.5 KB would cut a
-// CU to two workgroups, and its weights are read from L2 instead (1.65 -> 1.57 ms).
-// d0 (NRUNS = 6): each wave owns two z slices of a 16x4x8 tile (ChunkGemmRows16Z), so each
-// weight fragment feeds 8 MFMAs; its registers allow two waves per SIMD (1.57 -> 1.49 ms;
-// held to three, it spills and takes 1.62 ms).
-template <int MODE, int NRUNS, int ZW = (NRUNS > 2 ? 2 : 1)>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZW == 1 ? 3 : 2))) void KConvTileP16(const ConvArgs a) {
-    constexpr int CS = HaloStride(2);
-    constexpr int TZ = 4 * ZW, HZ = TZ + 2, HV = kHX * kHY * HZ;  // the tile's z extent and its halo
-    constexpr int kPF = PrefetchRegs<NRUNS, HV>();
-    constexpr bool LDSW = NRUNS <= 2;
+// Persistent convolution for the network's 16-output-channel stride-1 layers (e0: 8
+// inputs; d0: 48 = 32 upsampled + 16 skip channels), forward and upsample-concat modes.
+// A workgroup loops over 16 (x) x 4 (y) x 4*ZW (z) output tiles; wave w owns z slices
+// w*ZW .. w*ZW+ZW-1, each as four MFMA tiles of 16 x-voxels (one per y row) by the 16
+// output channels.  Input channels go in chunks of RC 8-channel runs (RC = 2 unless the
+// layer has one run).  Each chunk's halo (18 x 6 x (4*ZW+2) voxels) and weights are
+// loaded into registers while the previous chunk's matrix loop runs and written to LDS
+// between two barriers; the matrix loop reads only LDS.  (Weights read from global memory
+// inside the loop, issued after the halo prefetch, make every wait on them a wait on the
+// prefetch too: memory counters retire in order.)  A layer with one chunk stages its
+// weights once.
+// Matrix loop: K step s gives lane group g the combination c = 4s + g of (kz, kx, run);
+// the wave's four output rows y = m read halo rows m + ky, so the three ky taps share six
+// halo fragments per z slice (12 MFMAs per slice and step, each weight fragment feeding
+// 4*ZW of them).  Combinations past the chunk's 9*RC take zero weights and a valid halo
+// address, so the loop has no branches.  The instruction's operands are the weights (rows
+// = output channels) and the input (columns = voxels), so a lane ends with 4 consecutive
+// output channels of one voxel and the epilogue writes 8 bytes (bf16) or 16 bytes (fp32)
+// per lane, each y row 16 voxels x 16 channels contiguous.
+template <int MODE, int NRUNS, int ZW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void KConvTileP16(const ConvArgs a) {
+    static_assert(MODE == kFwd || MODE == kUpcat, "forward gathers only");
+    constexpr int CS = HaloStride(2);                  // 24 bf16 (48 B) per halo voxel
+    constexpr int TZ = 4 * ZW, HZ = TZ + 2, HV = kHX * kHY * HZ;
+    constexpr int RC = NRUNS < 2 ? NRUNS : 2;          // 8-channel runs per chunk
+    static_assert(NRUNS % RC == 0, "whole chunks");
+    constexpr int kChunks = NRUNS / RC;
+    constexpr int kPF = (HV * RC + 255) / 256;         // halo prefetch registers (16 B each)
+    constexpr int WE = 27 * RC * 16;                   // weight fragments of a chunk (16 B each)
+    constexpr int kPW = (WE + 255) / 256;
+    constexpr int kCombos = 9 * RC, kSteps = (kCombos + 3) / 4;
+    constexpr int kPlane = kHY * kHX * CS;             // one halo z layer (bf16)
+    constexpr int kTapW = RC * 16 * 8;                 // one tap's weights (bf16)
+    constexpr int kZeroW = WE * 8;                     // zero weights for padded combinations:
+    constexpr int kZeroLen = 6 * kTapW + 128;          // reached at offsets 0, 3, 6 taps (+16 rows)
     __shared__ __attribute__((aligned(16))) u16 sX[HV * CS];
-    __shared__ __attribute__((aligned(16))) u16 sW[LDSW ? 27 * NRUNS * 16 * 8 : 8];
+    __shared__ __attribute__((aligned(16))) u16 sW[WE * 8 + kZeroLen];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, grp = lane >> 4;
     const int K = 27 * a.kin;
     const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + TZ - 1) / TZ;
     const int ntiles = a.nb * tiles_z * tiles_y * tiles_x;
-    constexpr int kChunks = (NRUNS + 1) / 2;
     int tile = blockIdx.x;
     if (tile >= ntiles) return;  // whole workgroup, before any barrier
     auto decode = [&](int t, int& n, int& x0, int& y0, int& z0) {
@@ -1061,42 +1011,88 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZW == 1 ? 3
         z0 = (t % tiles_z) * TZ;
         n = t / tiles_z;
     };
-    // the layer's weights, once
-    for (int e = threadIdx.x; LDSW && e < 16 * 27 * NRUNS; e += 256) {
-        const int nn = e & 15, tc = e >> 4;
-        const int t = tc / NRUNS, r = tc - t * NRUNS;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (nn < a.nout) v = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(nn) * K + t * a.kin + r * 8);
-        *reinterpret_cast<uint4*>(&sW[e * 8]) = v;
-    }
-    uint4 pf[kPF];
-    auto prefetch = [&](int t, int c) {
+    uint4 pf[kPF], pw[kPW];
+    auto prefetch_halo = [&](int t, int c) {
         int n, x0, y0, z0;
         decode(t, n, x0, y0, z0);
-        const int runs = NRUNS - 2 * c < 2 ? NRUNS - 2 * c : 2;
 #pragma unroll
         for (int j = 0; j < kPF; ++j) {
             const int e = threadIdx.x + 256 * j;
             pf[j] = make_uint4(0u, 0u, 0u, 0u);
-            if (e < HV * runs) {
-                const int hv = e / runs, r8 = e - hv * runs;
+            if (e < HV * RC) {
+                const int hv = e / RC, r8 = e - hv * RC;
                 const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-                pf[j] = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + hx, 16 * c + r8 * 8);
+                pf[j] = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + hx, 8 * RC * c + r8 * 8);
             }
         }
     };
-    auto commit = [&](int c) {
-        const int runs = NRUNS - 2 * c < 2 ? NRUNS - 2 * c : 2;
+    // sW = [tap][run][output channel][8 channels]
+    auto prefetch_weights = [&](int c) {
+#pragma unroll
+        for (int j = 0; j < kPW; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            pw[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (e < WE) {
+                const int co = e & 15, tr = e >> 4;
+                const int t = tr / RC, r = tr - t * RC;
+                if (co < a.nout) pw[j] = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(co) * K + t * a.kin +
+                                                                         8 * RC * c + r * 8);
+            }
+        }
+    };
+    auto commit_halo = [&]() {
 #pragma unroll
         for (int j = 0; j < kPF; ++j) {
             const int e = threadIdx.x + 256 * j;
-            if (e < HV * runs) {
-                const int hv = e / runs, r8 = e - hv * runs;
+            if (e < HV * RC) {
+                const int hv = e / RC, r8 = e - hv * RC;
                 *reinterpret_cast<uint4*>(&sX[hv * CS + r8 * 8]) = pf[j];
             }
         }
     };
-    prefetch(tile, 0);
+    auto commit_weights = [&]() {
+#pragma unroll
+        for (int j = 0; j < kPW; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            if (e < WE) *reinterpret_cast<uint4*>(&sW[e * 8]) = pw[j];
+        }
+    };
+    // LDS element offsets of step s for this lane: its halo fragment at (slice 0, row 0) and
+    // its weight fragment at ky = 0; past the chunk's combinations, a zero block.
+    auto step_offsets = [&](int s, int& xo, int& wo) {
+        const int c = 4 * s + grp;
+        const bool ok = c < kCombos;
+        const int cc = ok ? c : 0;
+        const int kzx = cc / RC, run = cc - kzx * RC;
+        const int kz = kzx / 3, kx = kzx - kz * 3;
+        xo = (wave * ZW + kz) * kPlane + (row + kx) * CS + run * 8;
+        wo = (ok ? ((kz * 9 + kx) * RC + run) * 128 : kZeroW) + row * 8;
+    };
+    auto load_w = [&](int wo, bf16x8 (&w)[3]) {
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+            const uint4 v = *reinterpret_cast<const uint4*>(&sW[wo + ky * 3 * kTapW]);
+            __builtin_memcpy(&w[ky], &v, 16);
+        }
+    };
+    auto load_x = [&](int xo, int j, bf16x8 (&x)[6]) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const uint4 v = *reinterpret_cast<const uint4*>(&sX[xo + j * kPlane + r * kHX * CS]);
+            __builtin_memcpy(&x[r], &v, 16);
+        }
+    };
+    // the zero block: written once, never overwritten by the chunk weights
+    for (int e = threadIdx.x; e < kZeroLen / 8; e += 256)
+        *reinterpret_cast<uint4*>(&sW[kZeroW + e * 8]) = make_uint4(0u, 0u, 0u, 0u);
+    float bias[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bias[q] = a.bias ? a.bias[4 * grp + q] : 0.f;
+    prefetch_halo(tile, 0);
+    prefetch_weights(0);
+    if constexpr (kChunks == 1) {
+        commit_weights();  // read after the first barrier below
+    }
     f32x4 acc[ZW][4];
     for (;;) {
 #pragma unroll
@@ -1104,48 +1100,79 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZW == 1 ? 3
 #pragma unroll
             for (int m = 0; m < 4; ++m) acc[j][m] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int c = 0; c < kChunks; ++c) {
-            __syncthreads();  // the previous chunk's matrix loop is done with sX (and sW is staged)
-            commit(c);
+            __syncthreads();  // the previous chunk's matrix loop is done with sX and sW
+            commit_halo();
+            if constexpr (kChunks > 1) commit_weights();
             __syncthreads();
-            // the next chunk's halo: this tile's next chunk, else the next tile's first
+            // the next chunk: this tile's next one, else the next tile's first
             const int nt = c + 1 < kChunks ? tile : tile + static_cast<int>(gridDim.x);
-            if (nt < ntiles) prefetch(nt, c + 1 < kChunks ? c + 1 : 0);
-            if constexpr (ZW == 1) {
-                f32x4(&acc1)[4][1] = reinterpret_cast<f32x4(&)[4][1]>(acc[0]);
-                if (NRUNS - 2 * c >= 2)
-                    ChunkGemmRows16<MODE, 2, CS, LDSW, NRUNS>(a, sX, sW, wave, row, grp, K, 16 * c, acc1, 2 * c);
-                else
-                    ChunkGemmRows16<MODE, 1, CS, LDSW, NRUNS>(a, sX, sW, wave, row, grp, K, 16 * c, acc1, 2 * c);
-            } else {
-                if (NRUNS - 2 * c >= 2)
-                    ChunkGemmRows16Z<MODE, 2, CS, LDSW, NRUNS, ZW>(a, sX, sW, wave, row, grp, K, 16 * c, acc, 2 * c);
-                else
-                    ChunkGemmRows16Z<MODE, 1, CS, LDSW, NRUNS, ZW>(a, sX, sW, wave, row, grp, K, 16 * c, acc, 2 * c);
+            if (nt < ntiles) {
+                prefetch_halo(nt, c + 1 < kChunks ? c + 1 : 0);
+                if constexpr (kChunks > 1) prefetch_weights(c + 1 < kChunks ? c + 1 : 0);
+            }
+            // matrix loop, unrolled: the operands of the next (step, slice) are read before this
+            // one's MFMAs
+            int xo, wo;
+            step_offsets(0, xo, wo);
+            bf16x8 wc[3], xc[6];
+            load_w(wo, wc);
+            load_x(xo, 0, xc);
+#pragma unroll
+            for (int s = 0; s < kSteps; ++s) {
+#pragma unroll
+                for (int j = 0; j < ZW; ++j) {
+                    const bool last = j + 1 == ZW;
+                    bf16x8 wn[3], xn[6];
+                    int xo2 = xo, wo2 = wo;
+                    if (!last) {
+                        load_x(xo, j + 1, xn);
+                    } else if (s + 1 < kSteps) {
+                        step_offsets(s + 1, xo2, wo2);
+                        load_w(wo2, wn);
+                        load_x(xo2, 0, xn);
+                    }
+#pragma unroll
+                    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                        for (int m = 0; m < 4; ++m)
+                            acc[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wc[ky], xc[m + ky], acc[j][m], 0, 0, 0);
+                    if (!last || s + 1 < kSteps) {
+#pragma unroll
+                        for (int r = 0; r < 6; ++r) xc[r] = xn[r];
+                    }
+                    if (last && s + 1 < kSteps) {
+#pragma unroll
+                        for (int ky = 0; ky < 3; ++ky) wc[ky] = wn[ky];
+                        xo = xo2;
+                        wo = wo2;
+                    }
+                }
             }
         }
-        // epilogue: lane holds output rows 4*(lane/16)+r (x within the tile row), column lane%16
+        // epilogue: lane holds output channels 4*grp .. 4*grp+3 of voxel x0 + row
         int n, x0, y0, z0;
         decode(tile, n, x0, y0, z0);
+        const int x = x0 + row;
 #pragma unroll
         for (int j = 0; j < ZW; ++j) {
             const int z = z0 + wave * ZW + j;
-            if (row >= a.nout || z >= a.dd) continue;
-            const float bb = a.bias ? a.bias[row] : 0.f;
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const int y = y0 + m;
-                if (y >= a.hd) continue;
+                if (x >= a.wd || y >= a.hd || z >= a.dd) continue;
+                const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
+                float val[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int x = x0 + grp * 4 + q;
-                    if (x >= a.wd) continue;
-                    const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
-                    float val = acc[j][m][q] + bb;
-                    if (a.act == 1) val = val > 0.f ? val : kLeak * val;
-                    if (a.outf)
-                        a.outf[v * a.nout + row] = val;
-                    else
-                        a.out[v * a.nout + row] = F2Bf(val);
+                    val[q] = acc[j][m][q] + bias[q];
+                    if (a.act == 1) val[q] = val[q] > 0.f ? val[q] : kLeak * val[q];
+                }
+                if (a.outf) {
+                    *reinterpret_cast<float4*>(a.outf + v * 16 + 4 * grp) = make_float4(val[0], val[1], val[2], val[3]);
+                } else {
+                    const uint2 pk = make_uint2(static_cast<uint32_t>(F2Bf(val[0])) | (static_cast<uint32_t>(F2Bf(val[1])) << 16),
+                                                static_cast<uint32_t>(F2Bf(val[2])) | (static_cast<uint32_t>(F2Bf(val[3])) << 16));
+                    *reinterpret_cast<uint2*>(a.out + v * 16 + 4 * grp) = pk;
                 }
             }
         }
@@ -1153,6 +1180,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZW == 1 ? 3
         if (tile >= ntiles) break;
     }
 }
+
 
 // Workgroups of a persistent launch: the CUs times the kernel's resident workgroups per CU.
 template <class K>
@@ -1307,14 +1335,24 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
         const unsigned tiles = static_cast<unsigned>(a.nb) * ((a.dd + kTZ - 1) / kTZ) * ((a.hd + kTY - 1) / kTY) *
                                ((a.wd + kTX - 1) / kTX);
         if (tiles == 0) return;
-        if (a.nout <= 16 && MODE != kAdj && (a.kin == 8 || a.kin == 48) && kConvPersistent) {
-            // the persistent kernel for the network's 16-channel layers (e0: 8 inputs, d0: 48)
-            if (a.kin == 8)
-                hipLaunchKernelGGL((KConvTileP16<MODE, 1>), dim3(PersistentGrid(KConvTileP16<MODE, 1>, tiles)),
-                                   dim3(256), 0, s, a);
-            else
-                hipLaunchKernelGGL((KConvTileP16<MODE, 6>), dim3(PersistentGrid(KConvTileP16<MODE, 6>, tiles)),
-                                   dim3(256), 0, s, a);  // at most `tiles` workgroups: enough for any ZW
+        bool done = false;
+        if constexpr (MODE != kAdj && kConvPersistent) {
+            if (a.nout == 16 && (a.kin == 8 || a.kin == 48)) {
+                // the persistent kernel for the network's 16-channel layers (e0: 8 inputs, d0: 48);
+                // at most one workgroup per tile of 16 x 4 x 8 voxels
+                constexpr int ZW = 2;
+                const unsigned ptiles = static_cast<unsigned>(a.nb) * ((a.dd + 4 * ZW - 1) / (4 * ZW)) *
+                                        ((a.hd + kTY - 1) / kTY) * ((a.wd + kTX - 1) / kTX);
+                if (a.kin == 8)
+                    hipLaunchKernelGGL((KConvTileP16<MODE, 1, ZW>),
+                                       dim3(PersistentGrid(KConvTileP16<MODE, 1, ZW>, ptiles)), dim3(256), 0, s, a);
+                else
+                    hipLaunchKernelGGL((KConvTileP16<MODE, 6, ZW>),
+                                       dim3(PersistentGrid(KConvTileP16<MODE, 6, ZW>, ptiles)), dim3(256), 0, s, a);
+                done = true;
+            }
+        }
+        if (done) {
         } else if (a.nout <= 16)
             LaunchConvTile<MODE, 1>(a, dim3(tiles, 1), s);
         else if (a.nout <= 32)
