@@ -47,7 +47,7 @@ storage with stochastic rounding), steps and wall time until the trailing-mean d
 coefficient is within 1% of a plain start's value at 6000 steps, for FlowCraft's two plain
 starts (rest; half the inlet speed, its default), the learned warm start (committed
 surrogate weights, inference included) and FlowCraft's coarse-grid warm start.
-`warm_start_settled` is the same study (--warm-shapes-settled, 16 shapes) against the
+`warm_start_settled` is the same study (--warm-shapes-settled, 8 shapes) against the
 plain starts' values at 24000 steps, where their trailing means have settled, with the
 weights trained on 15000-step fields.
 """
@@ -494,7 +494,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--warm-shapes", type=int, default=64,
                     help="held-out shapes of the live warm-vs-plain comparison, 6000-step horizon (0 skips it)")
-    ap.add_argument("--warm-shapes-settled", type=int, default=16,
+    ap.add_argument("--warm-shapes-settled", type=int, default=8,
                     help="held-out shapes of the same comparison at the 24000-step horizon (0 skips it)")
     ap.add_argument("--profile", action="store_true",
                     help="only the FP32 D3Q19 step kernel for --steps steps (for rocprofv3 runs); "
@@ -560,7 +560,8 @@ def main():
                           "achieved_GBs": r["mlups"] * 1e6 * 2 * 19 * b / 1e9}), flush=True)
         return
     main_r = time_variant(torch, grid, LatticeModel.D3Q19, Storage.F32, layout, args.steps, args.warmup)
-    print(f"# D3Q19 F32 {layout.name}: {main_r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    print(f"# D3Q19 F32 {layout.name}: {main_r['mlups']:.0f} MLUPS ({time.time() - t0:.1f} s)", file=sys.stderr,
+          flush=True)
     variants = {}
     if not args.no_variants:
         # variants are timed over at least VARIANT_STEPS steps: a 0.5 ms step timed 20 times
@@ -608,6 +609,7 @@ def main():
                 "mlups": round(r["mlups"], 1), "us_per_step": round(r["ms_per_step"] * 1e3, 2),
                 "achieved_GBs": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "finite": r["finite"]}
 
+    print(f"# block variants: {time.time() - t0:.1f} s since start", file=sys.stderr, flush=True)
     bytes_per_cell = 2 * 19 * 4
     achieved = main_r["mlups"] * 1e6 * bytes_per_cell / 1e9
     traffic = load_traffic(f"lbm_step_d3q19_f32_{layout.name.lower()}_512x256x256")
@@ -616,19 +618,25 @@ def main():
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "algorithmic_bytes_per_launch": bytes_per_cell * grid.CellCount(),
                 "kernel": f"fc::KStep<19, StoreF32, {layout.name.title().replace('place', 'Place')}>"}
-    aero = None if args.no_variants else aero_forces_block(torch, grid, cpu=not args.no_cpu)
-    vox = None if args.no_variants else voxelize_block(cpu=not args.no_cpu)
-    smoke = None if args.no_variants else smoke_block(torch, grid, cpu=not args.no_cpu)
-    pacing = None if args.no_variants else pacing_block(torch, grid)
-    fp64_err = None if args.no_variants else fp64_error_block(torch, grid)
+    def timed(name, fn):
+        t = time.time()
+        r = fn()
+        print(f"# block {name}: {time.time() - t:.1f} s", file=sys.stderr, flush=True)
+        return r
+
+    aero = None if args.no_variants else timed("aero_forces", lambda: aero_forces_block(torch, grid, cpu=not args.no_cpu))
+    vox = None if args.no_variants else timed("voxelize", lambda: voxelize_block(cpu=not args.no_cpu))
+    smoke = None if args.no_variants else timed("smoke_trails", lambda: smoke_block(torch, grid, cpu=not args.no_cpu))
+    pacing = None if args.no_variants else timed("pacing", lambda: pacing_block(torch, grid))
+    fp64_err = None if args.no_variants else timed("error_vs_fp64", lambda: fp64_error_block(torch, grid))
     # the precision target on a flow where FP32 tracks FP64: the same wing at tau 0.6, where FP32
     # stays within 0.03% of FP64, over 20000 steps (trailing 10000-step means)
-    fp64_tau06 = None if args.no_variants else fp64_error_block(
-        torch, grid, steps=20000, window=10000, tau=0.6, storages=("F32", "F16", "F16_SR", "BF16_SR"))
-    warm = warm_start_block(torch, args.warm_shapes) if args.warm_shapes > 0 else None
-    warm_settled = (warm_start_block(torch, args.warm_shapes_settled, settled=True)
+    fp64_tau06 = None if args.no_variants else timed("error_vs_fp64_tau06", lambda: fp64_error_block(
+        torch, grid, steps=20000, window=10000, tau=0.6, storages=("F32", "F16_SR", "BF16_SR")))
+    warm = timed("warm_start", lambda: warm_start_block(torch, args.warm_shapes)) if args.warm_shapes > 0 else None
+    warm_settled = (timed("warm_start_settled", lambda: warm_start_block(torch, args.warm_shapes_settled, settled=True))
                     if args.warm_shapes_settled > 0 else None)
-    cpu = None if args.no_cpu else cpu_baseline(grid)
+    cpu = None if args.no_cpu else timed("cpu_baseline", lambda: cpu_baseline(grid))
     line = {
         "metric": METRIC, "value": round(main_r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(main_r["ms_per_step"], 4),

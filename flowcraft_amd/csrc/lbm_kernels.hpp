@@ -784,6 +784,21 @@ __device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
     return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
+// Chunked remap (FC_LBM_XCD_REMAP=2, FC_LBM_XCD_CHUNK=C): each XCD receives runs of C
+// consecutive tiles, and the eight XCDs take neighbouring runs, so the whole device works
+// on one compact band of the lattice while tiles that share cache lines within a run stay
+// in one L2.  Ids past the last whole group of 8*C map to themselves.  Measured on the
+// 512x256x256 wing, D3Q19 ping-pong, FP32 / stochastically rounded FP16 / FP32 in place
+// (GLUPS): contiguous ranges (the default) 37.1 / 69.2 / 32.1, C = 4 36.5 / 65.2 / 29.8,
+// C = 16 35.0 / 58.2 / 30.9, hardware order (FC_LBM_XCD_REMAP=0) 34.3 / 66.0 / 27.5.
+template <unsigned C>
+__device__ __forceinline__ unsigned XcdChunked(unsigned hw, unsigned n) {
+    const unsigned full = n / (8u * C) * (8u * C);
+    if (hw >= full) return hw;
+    const unsigned xcd = hw & 7u, slot = hw >> 3;
+    return ((slot / C) * 8u + xcd) * C + slot % C;
+}
+
 // Stored form of a cell's post-collision working values (inside the step kernel, which
 // has set the storage format's StepMode).
 // `h`: the cell's stochastic-rounding hash (SrHash), used by the *SR formats only.
@@ -1143,7 +1158,12 @@ __global__ __launch_bounds__(256) FC_LBM_STEP_ATTR void KStep(const StepArgs a) 
 #ifndef FC_LBM_XCD_REMAP
 #define FC_LBM_XCD_REMAP 1
 #endif
-    const unsigned tile = FC_LBM_XCD_REMAP ? XcdContiguous(blockIdx.x, gridDim.x) : blockIdx.x;
+#ifndef FC_LBM_XCD_CHUNK
+#define FC_LBM_XCD_CHUNK 4
+#endif
+    const unsigned tile = FC_LBM_XCD_REMAP == 1   ? XcdContiguous(blockIdx.x, gridDim.x)
+                          : FC_LBM_XCD_REMAP == 2 ? XcdChunked<FC_LBM_XCD_CHUNK>(blockIdx.x, gridDim.x)
+                                                  : blockIdx.x;
     constexpr int kTileX = TileX(sizeof(T)), kTileY = TileY(sizeof(T));
     const unsigned tiles_x = static_cast<unsigned>((a.nx + kTileX - 1) / kTileX);
     const unsigned tiles_y = static_cast<unsigned>((a.ny + kTileY * CPT - 1) / (kTileY * CPT));
