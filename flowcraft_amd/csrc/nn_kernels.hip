@@ -959,46 +959,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3
         }
 }
 
-// Persistent convolution for the network's 16-output-channel stride-1 layers (e0: 8
-// inputs; d0: 48 = 32 upsampled + 16 skip channels), forward and upsample-concat modes.
-// A workgroup loops over 16 (x) x 4 (y) x 4*ZW (z) output tiles; wave w owns z slices
-// w*ZW .. w*ZW+ZW-1, each as four MFMA tiles of 16 x-voxels (one per y row) by the 16
-// output channels.  Input channels go in chunks of RC 8-channel runs (RC = 2 unless the
-// layer has one run).  Each chunk's halo (18 x 6 x (4*ZW+2) voxels) and weights are
-// loaded into registers while the previous chunk's matrix loop runs and written to LDS
-// between two barriers; the matrix loop reads only LDS.  (Weights read from global memory
-// inside the loop, issued after the halo prefetch, make every wait on them a wait on the
-// prefetch too: memory counters retire in order.)  A layer with one chunk stages its
-// weights once.
+// Persistent convolution for the network's stride-1 layers of 16 to 64 output channels
+// (e0: 8 inputs -> 16; e1b: 32 -> 32; e2b: 64 -> 64; d2: 192 = 128 upsampled + 64 skip ->
+// 64; d1: 96 = 64 upsampled + 32 skip -> 32; d0: 48 = 32 upsampled + 16 skip -> 16),
+// forward and upsample-concat modes, CO = 16*NT output channels per workgroup (blockIdx.y
+// selects the block of CO channels).  A workgroup loops over 16 (x) x 4 (y) x 4*ZW (z) output
+// tiles; wave w owns z slices w*ZW .. w*ZW+ZW-1, each as four MFMA tiles of 16 x-voxels
+// (one per y row) by the CO output channels.  Input channels go in chunks of RC 8-channel
+// runs (RC = 2 unless the layer has one run).  Each chunk's halo (18 x 6 x (4*ZW+2) voxels)
+// and weights are loaded into registers while the previous chunk's matrix loop runs and
+// written to LDS between two barriers; the matrix loop reads only LDS.  (Weights read from
+// global memory inside the loop, issued after the halo prefetch, make every wait on them a
+// wait on the prefetch too: memory counters retire in order.)  A layer with one chunk
+// stages its weights once.
 // Matrix loop: K step s gives lane group g the combination c = 4s + g of (kz, kx, run);
 // the wave's four output rows y = m read halo rows m + ky, so the three ky taps share six
-// halo fragments per z slice (12 MFMAs per slice and step, each weight fragment feeding
-// 4*ZW of them).  Combinations past the chunk's 9*RC take zero weights and a valid halo
-// address, so the loop has no branches.  The instruction's operands are the weights (rows
-// = output channels) and the input (columns = voxels), so a lane ends with 4 consecutive
-// output channels of one voxel and the epilogue writes 8 bytes (bf16) or 16 bytes (fp32)
-// per lane, each y row 16 voxels x 16 channels contiguous.
-template <int MODE, int NRUNS, int ZW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void KConvTileP16(const ConvArgs a) {
+// halo fragments per z slice (12*NT MFMAs per slice and step, each weight fragment feeding
+// 4*ZW of them).  Combinations past the chunk's 9*RC (only in the last step) read a valid
+// halo address and zero weights, so the loop has no branches: with 16 output channels the
+// weights come from a zero block in LDS (no extra instructions: d0 0.97 against 1.00 ms
+// with selects), with 32 the fragments are replaced by zeros (the block would take LDS
+// that the second workgroup per CU needs).  The instruction's operands are the weights (rows = output channels) and the
+// input (columns = voxels), so a lane ends with 4 consecutive output channels of one voxel
+// per 16-channel tile and the epilogue writes 8 bytes (bf16) or 16 bytes (fp32) per lane
+// and tile.
+template <int MODE, int NRUNS, int ZW, int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void KConvTileP(const ConvArgs a) {
     static_assert(MODE == kFwd || MODE == kUpcat, "forward gathers only");
+    constexpr int CO = 16 * NT;                        // output channels of the workgroup
     constexpr int CS = HaloStride(2);                  // 24 bf16 (48 B) per halo voxel
     constexpr int TZ = 4 * ZW, HZ = TZ + 2, HV = kHX * kHY * HZ;
     constexpr int RC = NRUNS < 2 ? NRUNS : 2;          // 8-channel runs per chunk
     static_assert(NRUNS % RC == 0, "whole chunks");
     constexpr int kChunks = NRUNS / RC;
     constexpr int kPF = (HV * RC + 255) / 256;         // halo prefetch registers (16 B each)
-    constexpr int WE = 27 * RC * 16;                   // weight fragments of a chunk (16 B each)
+    constexpr int WE = 27 * RC * CO;                   // weight fragments of a chunk (16 B each)
     constexpr int kPW = (WE + 255) / 256;
     constexpr int kCombos = 9 * RC, kSteps = (kCombos + 3) / 4;
     constexpr int kPlane = kHY * kHX * CS;             // one halo z layer (bf16)
-    constexpr int kTapW = RC * 16 * 8;                 // one tap's weights (bf16)
-    constexpr int kZeroW = WE * 8;                     // zero weights for padded combinations:
-    constexpr int kZeroLen = 6 * kTapW + 128;          // reached at offsets 0, 3, 6 taps (+16 rows)
+    constexpr int kTapW = RC * CO * 8;                 // one tap's weights (bf16)
+    constexpr bool ZB = NT == 1;                       // zero weights from an LDS block
+    constexpr int kZeroW = WE * 8;                     // the zero block: reached at offsets
+    constexpr int kZeroLen = ZB ? 6 * kTapW + CO * 8 : 0;  // 0, 3, 6 taps (+ CO rows)
     __shared__ __attribute__((aligned(16))) u16 sX[HV * CS];
     __shared__ __attribute__((aligned(16))) u16 sW[WE * 8 + kZeroLen];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, grp = lane >> 4;
     const int K = 27 * a.kin;
+    const int nbase = blockIdx.y * CO;
     const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + TZ - 1) / TZ;
     const int ntiles = a.nb * tiles_z * tiles_y * tiles_x;
     int tile = blockIdx.x;
@@ -1033,7 +1041,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
             const int e = threadIdx.x + 256 * j;
             pw[j] = make_uint4(0u, 0u, 0u, 0u);
             if (e < WE) {
-                const int co = e & 15, tr = e >> 4;
+                const int co = nbase + e % CO, tr = e / CO;
                 const int t = tr / RC, r = tr - t * RC;
                 if (co < a.nout) pw[j] = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(co) * K + t * a.kin +
                                                                          8 * RC * c + r * 8);
@@ -1058,22 +1066,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
         }
     };
     // LDS element offsets of step s for this lane: its halo fragment at (slice 0, row 0) and
-    // its weight fragment at ky = 0; past the chunk's combinations, a zero block.
-    auto step_offsets = [&](int s, int& xo, int& wo) {
+    // its weight fragment at ky = 0, tile 0; past the chunk's combinations, combination 0's
+    // halo and the zero block (ZB) or combination 0's weights, replaced by zeros in load_w.
+    auto step_offsets = [&](int s, int& xo, int& wo, bool& ok) {
         const int c = 4 * s + grp;
-        const bool ok = c < kCombos;
+        ok = c < kCombos;
         const int cc = ok ? c : 0;
         const int kzx = cc / RC, run = cc - kzx * RC;
         const int kz = kzx / 3, kx = kzx - kz * 3;
         xo = (wave * ZW + kz) * kPlane + (row + kx) * CS + run * 8;
-        wo = (ok ? ((kz * 9 + kx) * RC + run) * 128 : kZeroW) + row * 8;
+        wo = (ok || !ZB ? ((kz * 9 + kx) * RC + run) * CO * 8 : kZeroW) + row * 8;
     };
-    auto load_w = [&](int wo, bf16x8 (&w)[3]) {
+    auto load_w = [&](int wo, bool ok, bf16x8 (&w)[3][NT]) {
 #pragma unroll
-        for (int ky = 0; ky < 3; ++ky) {
-            const uint4 v = *reinterpret_cast<const uint4*>(&sW[wo + ky * 3 * kTapW]);
-            __builtin_memcpy(&w[ky], &v, 16);
-        }
+        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+            for (int jt = 0; jt < NT; ++jt) {
+                uint4 v = *reinterpret_cast<const uint4*>(&sW[wo + ky * 3 * kTapW + jt * 128]);
+                if (!ZB && !ok) v = make_uint4(0u, 0u, 0u, 0u);
+                __builtin_memcpy(&w[ky][jt], &v, 16);
+            }
     };
     auto load_x = [&](int xo, int j, bf16x8 (&x)[6]) {
 #pragma unroll
@@ -1083,22 +1095,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
         }
     };
     // the zero block: written once, never overwritten by the chunk weights
-    for (int e = threadIdx.x; e < kZeroLen / 8; e += 256)
-        *reinterpret_cast<uint4*>(&sW[kZeroW + e * 8]) = make_uint4(0u, 0u, 0u, 0u);
-    float bias[4];
+    if constexpr (ZB)
+        for (int e = threadIdx.x; e < kZeroLen / 8; e += 256)
+            *reinterpret_cast<uint4*>(&sW[kZeroW + e * 8]) = make_uint4(0u, 0u, 0u, 0u);
+    float bias[NT][4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bias[q] = a.bias ? a.bias[4 * grp + q] : 0.f;
+    for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bias[jt][q] = a.bias ? a.bias[nbase + 16 * jt + 4 * grp + q] : 0.f;
     prefetch_halo(tile, 0);
     prefetch_weights(0);
     if constexpr (kChunks == 1) {
         commit_weights();  // read after the first barrier below
     }
-    f32x4 acc[ZW][4];
+    f32x4 acc[ZW][4][NT];
     for (;;) {
 #pragma unroll
         for (int j = 0; j < ZW; ++j)
 #pragma unroll
-            for (int m = 0; m < 4; ++m) acc[j][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int jt = 0; jt < NT; ++jt) acc[j][m][jt] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int c = 0; c < kChunks; ++c) {
             __syncthreads();  // the previous chunk's matrix loop is done with sX and sW
             commit_halo();
@@ -1113,43 +1130,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
             // matrix loop, unrolled: the operands of the next (step, slice) are read before this
             // one's MFMAs
             int xo, wo;
-            step_offsets(0, xo, wo);
-            bf16x8 wc[3], xc[6];
-            load_w(wo, wc);
+            bool ok;
+            step_offsets(0, xo, wo, ok);
+            bf16x8 wc[3][NT], xc[6];
+            load_w(wo, ok, wc);
             load_x(xo, 0, xc);
 #pragma unroll
             for (int s = 0; s < kSteps; ++s) {
 #pragma unroll
                 for (int j = 0; j < ZW; ++j) {
                     const bool last = j + 1 == ZW;
-                    bf16x8 wn[3], xn[6];
+                    bf16x8 wn[3][NT], xn[6];
                     int xo2 = xo, wo2 = wo;
+                    bool ok2 = ok;
                     if (!last) {
                         load_x(xo, j + 1, xn);
                     } else if (s + 1 < kSteps) {
-                        step_offsets(s + 1, xo2, wo2);
-                        load_w(wo2, wn);
+                        step_offsets(s + 1, xo2, wo2, ok2);
+                        load_w(wo2, ok2, wn);
                         load_x(xo2, 0, xn);
                     }
 #pragma unroll
                     for (int ky = 0; ky < 3; ++ky)
 #pragma unroll
                         for (int m = 0; m < 4; ++m)
-                            acc[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wc[ky], xc[m + ky], acc[j][m], 0, 0, 0);
+#pragma unroll
+                            for (int jt = 0; jt < NT; ++jt)
+                                acc[j][m][jt] =
+                                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(wc[ky][jt], xc[m + ky], acc[j][m][jt], 0, 0, 0);
                     if (!last || s + 1 < kSteps) {
 #pragma unroll
                         for (int r = 0; r < 6; ++r) xc[r] = xn[r];
                     }
                     if (last && s + 1 < kSteps) {
 #pragma unroll
-                        for (int ky = 0; ky < 3; ++ky) wc[ky] = wn[ky];
+                        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                            for (int jt = 0; jt < NT; ++jt) wc[ky][jt] = wn[ky][jt];
                         xo = xo2;
                         wo = wo2;
+                        ok = ok2;
                     }
                 }
             }
         }
-        // epilogue: lane holds output channels 4*grp .. 4*grp+3 of voxel x0 + row
+        // epilogue: lane holds output channels 16*jt + 4*grp .. +3 of voxel x0 + row
         int n, x0, y0, z0;
         decode(tile, n, x0, y0, z0);
         const int x = x0 + row;
@@ -1161,18 +1186,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
                 const int y = y0 + m;
                 if (x >= a.wd || y >= a.hd || z >= a.dd) continue;
                 const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
-                float val[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    val[q] = acc[j][m][q] + bias[q];
-                    if (a.act == 1) val[q] = val[q] > 0.f ? val[q] : kLeak * val[q];
-                }
-                if (a.outf) {
-                    *reinterpret_cast<float4*>(a.outf + v * 16 + 4 * grp) = make_float4(val[0], val[1], val[2], val[3]);
-                } else {
-                    const uint2 pk = make_uint2(static_cast<uint32_t>(F2Bf(val[0])) | (static_cast<uint32_t>(F2Bf(val[1])) << 16),
-                                                static_cast<uint32_t>(F2Bf(val[2])) | (static_cast<uint32_t>(F2Bf(val[3])) << 16));
-                    *reinterpret_cast<uint2*>(a.out + v * 16 + 4 * grp) = pk;
+                for (int jt = 0; jt < NT; ++jt) {
+                    float val[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        val[q] = acc[j][m][jt][q] + bias[jt][q];
+                        if (a.act == 1) val[q] = val[q] > 0.f ? val[q] : kLeak * val[q];
+                    }
+                    const long long o = v * a.nout + nbase + 16 * jt + 4 * grp;
+                    if (a.outf) {
+                        *reinterpret_cast<float4*>(a.outf + o) = make_float4(val[0], val[1], val[2], val[3]);
+                    } else {
+                        const uint2 pk = make_uint2(static_cast<uint32_t>(F2Bf(val[0])) | (static_cast<uint32_t>(F2Bf(val[1])) << 16),
+                                                    static_cast<uint32_t>(F2Bf(val[2])) | (static_cast<uint32_t>(F2Bf(val[3])) << 16));
+                        *reinterpret_cast<uint2*>(a.out + o) = pk;
+                    }
                 }
             }
         }
@@ -1181,6 +1210,165 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
     }
 }
 
+
+// LDS-tiled forward convolution with stride 2 (the encoder's downsampling layers).  A
+// workgroup computes a 16 (x) x 4 (y) x 4 (z) block of output voxels (wave w = z-slice w,
+// 4 MFMA row tiles of 16 x-voxels, one per y row) for 16*TN output channels, one 8-channel
+// run of the input at a time.  Per run it holds in LDS the 33 x 9 x 9 input halo of the
+// block, 16 bytes per voxel, each halo row split by x parity (even x at slots 0..16, odd x
+// at 17..32): the three kx taps of 16 consecutive output voxels (input x = 2x + kx - 1) then
+// read 16 consecutive slots, 256 contiguous bytes and no bank conflicts, where the unsplit
+// row would be read at a 32-byte stride; and the run's weights as [tap][output channel][8].
+// The next run's halo and weights are loaded into registers while the current run's matrix
+// loop runs (which reads only LDS, so no wait on those loads stalls it) and written to LDS
+// between two barriers.  The implicit GEMM of a run takes four taps per 32-wide K step (lane
+// group g takes tap 4s + g; the 28th is zero).  Every input voxel is read from global
+// memory about 1.45 times per run (the halo overlap) instead of once per tap.
+constexpr int kS2X = 2 * kTX + 1, kS2Y = 2 * kTY + 1, kS2Z = 2 * kTZ + 1;  // 33 x 9 x 9 input halo
+constexpr int kS2V = kS2X * kS2Y * kS2Z;
+__device__ __forceinline__ int S2Slot(int hx) { return (hx & 1) ? (kS2X + 1) / 2 + (hx >> 1) : (hx >> 1); }
+
+template <int TN>
+__global__ __launch_bounds__(256) void KConvTileS2(const ConvArgs a) {
+    constexpr int kPF = (kS2V + 255) / 256;  // halo prefetch registers (16 B each)
+    constexpr int WE = 27 * 16 * TN;         // weight fragments of a run (16 B each)
+    constexpr int kPW = (WE + 255) / 256;
+    __shared__ __attribute__((aligned(16))) u16 sX[kS2V * 8];
+    __shared__ __attribute__((aligned(16))) u16 sW[WE * 8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_x = (a.wd + kTX - 1) / kTX, tiles_y = (a.hd + kTY - 1) / kTY, tiles_z = (a.dd + kTZ - 1) / kTZ;
+    int b = blockIdx.x;
+    const int tx = b % tiles_x;
+    b /= tiles_x;
+    const int ty = b % tiles_y;
+    b /= tiles_y;
+    const int tz = b % tiles_z;
+    const int n = b / tiles_z;
+    const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+    const int nbase = blockIdx.y * (16 * TN);
+    const int row = lane & 15, grp = lane >> 4;
+    const int K = 27 * a.kin;
+    uint4 pf[kPF], pw[kPW];
+    auto prefetch = [&](int cb) {
+#pragma unroll
+        for (int j = 0; j < kPF; ++j) {
+            const int hv = threadIdx.x + 256 * j;
+            pf[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (hv < kS2V) {
+                const int hx = hv % kS2X, hy = (hv / kS2X) % kS2Y, hz = hv / (kS2X * kS2Y);
+                const int ix = 2 * x0 - 1 + hx, iy = 2 * y0 - 1 + hy, iz = 2 * z0 - 1 + hz;
+                if (ix >= 0 && iy >= 0 && iz >= 0 && ix < a.wi && iy < a.hi && iz < a.di) {
+                    const long long vi = ((static_cast<long long>(n) * a.di + iz) * a.hi + iy) * a.wi + ix;
+                    pf[j] = *reinterpret_cast<const uint4*>(a.in + vi * a.kin + cb);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPW; ++j) {
+            const int e = threadIdx.x + 256 * j;  // e = t * 16TN + co
+            pw[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (e < WE) {
+                const int t = e / (16 * TN), co = nbase + (e - t * (16 * TN));
+                if (co < a.nout) pw[j] = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(co) * K + t * a.kin + cb);
+            }
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int j = 0; j < kPF; ++j) {
+            const int hv = threadIdx.x + 256 * j;
+            if (hv < kS2V) {
+                const int hx = hv % kS2X, hyz = hv / kS2X;
+                *reinterpret_cast<uint4*>(&sX[(hyz * kS2X + S2Slot(hx)) * 8]) = pf[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPW; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            if (e < WE) *reinterpret_cast<uint4*>(&sW[e * 8]) = pw[j];
+        }
+    };
+    f32x4 acc[4][TN];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    constexpr int kSteps = 7;  // 27 taps, four per step
+    // step s's operands: weights of tap 4s + grp for the TN channel tiles, and the halo
+    // fragments of the wave's four y rows (zeros for the 28th tap)
+    auto load = [&](int s, uint4 (&bv)[TN], uint4 (&av)[4]) {
+        const int t = 4 * s + grp;
+        const bool tok = t < 27;
+        const int tt = tok ? t : 0;
+        const int kz = tt / 9, ky = (tt / 3) % 3, kx = tt % 3;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            bv[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (tok) bv[j] = *reinterpret_cast<const uint4*>(&sW[((tt * TN + j) * 16 + row) * 8]);
+        }
+        const int slot = kx == 1 ? (kS2X + 1) / 2 + row : row + (kx >> 1);
+        const int base = ((2 * wave + kz) * kS2Y + ky) * kS2X + slot;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            av[m] = make_uint4(0u, 0u, 0u, 0u);
+            if (tok) av[m] = *reinterpret_cast<const uint4*>(&sX[(base + 2 * m * kS2X) * 8]);
+        }
+    };
+    prefetch(0);
+    for (int cb = 0; cb < a.kin; cb += 8) {
+        if (cb > 0) __syncthreads();  // the previous run's matrix loop is done with sX and sW
+        commit();
+        __syncthreads();
+        if (cb + 8 < a.kin) prefetch(cb + 8);  // in flight during this run's matrix loop
+        uint4 bcur[TN], acur[4];
+        load(0, bcur, acur);
+#pragma unroll
+        for (int s = 0; s < kSteps; ++s) {
+            uint4 bnext[TN], anext[4];
+            if (s + 1 < kSteps) load(s + 1, bnext, anext);
+            bf16x8 bfr[TN];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) __builtin_memcpy(&bfr[j], &bcur[j], 16);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                bf16x8 afr;
+                __builtin_memcpy(&afr, &acur[m], 16);
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+            }
+            if (s + 1 < kSteps) {
+#pragma unroll
+                for (int j = 0; j < TN; ++j) bcur[j] = bnext[j];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) acur[m] = anext[m];
+            }
+        }
+    }
+    // epilogue: lane holds output rows 4*grp + q (x within the tile row), column row
+    const int z = z0 + wave;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int nn = nbase + j * 16 + row;
+            const int y = y0 + m;
+            if (nn >= a.nout || y >= a.hd || z >= a.dd) continue;
+            const float bb = a.bias ? a.bias[nn] : 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int x = x0 + grp * 4 + q;
+                if (x >= a.wd) continue;
+                const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
+                float val = acc[m][j][q] + bb;
+                if (a.act == 1) val = val > 0.f ? val : kLeak * val;
+                if (a.outf)
+                    a.outf[v * a.nout + nn] = val;
+                else
+                    a.out[v * a.nout + nn] = F2Bf(val);
+            }
+        }
+}
 
 // Workgroups of a persistent launch: the CUs times the kernel's resident workgroups per CU.
 template <class K>
@@ -1337,20 +1525,32 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
         if (tiles == 0) return;
         bool done = false;
         if constexpr (MODE != kAdj && kConvPersistent) {
-            if (a.nout == 16 && (a.kin == 8 || a.kin == 48)) {
-                // the persistent kernel for the network's 16-channel layers (e0: 8 inputs, d0: 48);
-                // at most one workgroup per tile of 16 x 4 x 8 voxels
-                constexpr int ZW = 2;
-                const unsigned ptiles = static_cast<unsigned>(a.nb) * ((a.dd + 4 * ZW - 1) / (4 * ZW)) *
+            // the persistent kernel for the network's layers of 16 to 64 output channels, in
+            // blocks of 16 or 32 on grid.y; the workgroups of one channel block at most one
+            // per output tile (16 x 4 x 4*ZW voxels)
+            auto persistent = [&](auto kern, int zw, int co) {
+                const unsigned ptiles = static_cast<unsigned>(a.nb) * ((a.dd + 4 * zw - 1) / (4 * zw)) *
                                         ((a.hd + kTY - 1) / kTY) * ((a.wd + kTX - 1) / kTX);
-                if (a.kin == 8)
-                    hipLaunchKernelGGL((KConvTileP16<MODE, 1, ZW>),
-                                       dim3(PersistentGrid(KConvTileP16<MODE, 1, ZW>, ptiles)), dim3(256), 0, s, a);
-                else
-                    hipLaunchKernelGGL((KConvTileP16<MODE, 6, ZW>),
-                                       dim3(PersistentGrid(KConvTileP16<MODE, 6, ZW>, ptiles)), dim3(256), 0, s, a);
+                const unsigned blocks = Blk(a.nout, co);
+                const unsigned g = PersistentGrid(kern, static_cast<long long>(ptiles) * blocks) / blocks;
+                hipLaunchKernelGGL(kern, dim3(g > 0 ? g : 1, blocks), dim3(256), 0, s, a);
                 done = true;
-            }
+            };
+            // z depth per tile (ZW) as measured fastest per layer at batch 4 x 256x128x128
+            if (a.nout == 16 && a.kin == 8)
+                persistent(KConvTileP<MODE, 1, 2, 1>, 2, 16);
+            else if (a.nout == 16 && a.kin == 48)
+                persistent(KConvTileP<MODE, 6, 2, 1>, 2, 16);
+            else if (a.nout == 32 && a.kin == 32 && MODE == kFwd)
+                persistent(KConvTileP<MODE, 4, 1, 2>, 1, 32);
+            else if (a.nout == 32 && a.kin == 96)
+                persistent(KConvTileP<MODE, 12, 2, 2>, 2, 32);
+            else if (a.nout == 64 && a.kin == 64 && MODE == kFwd)
+                persistent(KConvTileP<MODE, 8, 1, 2>, 1, 32);
+            else if (a.nout == 64 && a.kin == 192)
+                persistent(KConvTileP<MODE, 24, 2, 2>, 2, 32);
+            else if (a.nout == 128 && a.kin == 128 && MODE == kFwd)
+                persistent(KConvTileP<MODE, 16, 1, 2>, 1, 32);
         }
         if (done) {
         } else if (a.nout <= 16)
@@ -1361,6 +1561,21 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
             LaunchConvTile<MODE, 3>(a, dim3(tiles, 1), s);
         else
             LaunchConvTile<MODE, 4>(a, dim3(tiles, Blk(a.nout, 64)), s);
+        return;
+    }
+    // stride 2: the tiled kernel re-stages the halo for every 8-channel run, which pays for
+    // the 16-channel layer (e1: 0.37 -> 0.27 ms at batch 4 x 256x128x128) but not for 32
+    // (e2: 0.13 -> 0.16 ms); wider inputs take the direct gather below
+    if (MODE == kFwd && a.stride == 2 && a.kin <= 16 && kConvTiled) {
+        const unsigned tiles = static_cast<unsigned>(a.nb) * ((a.dd + kTZ - 1) / kTZ) * ((a.hd + kTY - 1) / kTY) *
+                               ((a.wd + kTX - 1) / kTX);
+        if (tiles == 0) return;
+        if (a.nout <= 16)
+            hipLaunchKernelGGL((KConvTileS2<1>), dim3(tiles, 1), dim3(256), 0, s, a);
+        else if (a.nout <= 32)
+            hipLaunchKernelGGL((KConvTileS2<2>), dim3(tiles, 1), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((KConvTileS2<4>), dim3(tiles, Blk(a.nout, 64)), dim3(256), 0, s, a);
         return;
     }
     if (MODE == kAdj && a.stride == 2 && a.kin % 32 == 0) {

@@ -154,16 +154,48 @@ def test_stride2_adjoint_odd_dims(cin, cout):
     assert rel(dxd.cpu().numpy().reshape(n, D, H, W, cin), no.conv_adj(dy, w, (n, D, H, W, cin), 2)) < 1e-4
 
 
+@pytest.mark.parametrize("cin,cout", [(8, 16), (16, 32), (8, 48), (16, 80), (32, 64), (64, 128)])
+def test_stride2_forward_several_tiles(cin, cout):
+    """Stride-2 forward on extents that span several 16 x 4 x 4 output tiles with ragged ones
+    in every axis (output 6 x 7 x 19 from 11 x 13 x 37: odd input extents, so the last output
+    voxel's kx = 2 tap falls outside the volume), fp32 and leaky bf16: the LDS-tiled kernel
+    (inputs of 8 or 16 channels; 48 and 80 outputs leave a partly filled 64-channel block)
+    and the direct gather (wider inputs)."""
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    rng = np.random.default_rng(7 * cin + cout)
+    n, D, H, W = 2, 11, 13, 37
+    Do, Ho, Wo = no.out_dims(D, 2), no.out_dims(H, 2), no.out_dims(W, 2)
+    x = bf(rng.normal(size=(n, D, H, W, cin)))
+    w = bf(rng.normal(size=(cout, 27, cin)) / np.sqrt(27 * cin))
+    b = rng.normal(size=cout).astype(np.float32).astype(np.float64)
+    ref = no.conv_fwd(x, w, b, 2)
+    assert ref.shape == (n, Do, Ho, Wo, cout)
+    yd = t.empty(n * Do * Ho * Wo * cout, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_conv(0, ptr(dev_bf16(x)), None, ptr(dev_bf16(w)), ptr(dev_f32(b)), None, ptr(yd), n, D, H, W, Do,
+                      Ho, Wo, cin, 0, cout, 2, 0, stream())
+    assert rc == 0
+    assert rel(yd.cpu().numpy().reshape(ref.shape), ref) < 1e-4
+    yb = t.empty(n * Do * Ho * Wo * cout, dtype=t.bfloat16, device="cuda")
+    rc = L.fc_nn_conv(0, ptr(dev_bf16(x)), None, ptr(dev_bf16(w)), ptr(dev_f32(b)), ptr(yb), None, n, D, H, W, Do,
+                      Ho, Wo, cin, 0, cout, 2, 1, stream())
+    assert rc == 0
+    assert rel(yb.float().cpu().numpy().reshape(ref.shape), no.leaky(ref)) < 2 ** -7
+
+
 @pytest.mark.parametrize("dims", [(1, 10, 4, 6), (2, 6, 6, 18)])
-def test_upcat_conv_ragged_tiles(dims):
-    """The 48 -> 16 upsample-concat layer (d0's shape) on extents that leave ragged tiles:
-    D = 10 and 6 against an 8-deep tile, W = 18 against a 16-wide one."""
+@pytest.mark.parametrize("c1,c2,cout", [(32, 16, 16), (64, 32, 32), (128, 64, 64)])
+def test_upcat_conv_ragged_tiles(dims, c1, c2, cout):
+    """The upsample-concat layers of the persistent kernels (d0: 48 -> 16, d1: 96 -> 32,
+    d2: 192 -> 64 in two 32-channel blocks) on
+    extents that leave ragged tiles: D = 10 and 6 against 8- and 4-deep tiles, W = 18 against
+    a 16-wide one."""
     from flowcraft_amd.surrogate import nn_lib
     t = torch_mod()
     L = nn_lib()
     n, D, H, W = dims
-    c1, c2, cout = 32, 16, 16
-    rng = np.random.default_rng(D * W)
+    rng = np.random.default_rng(D * W + cout)
     xc = bf(rng.normal(size=(n, D // 2, H // 2, W // 2, c1)))
     xs = bf(rng.normal(size=(n, D, H, W, c2)))
     kin = c1 + c2
@@ -175,6 +207,33 @@ def test_upcat_conv_ragged_tiles(dims):
                       H, W, D, H, W, kin, c1, cout, 1, 0, stream())
     assert rc == 0
     assert rel(yd.cpu().numpy().reshape(ref.shape), ref) < 1e-4
+
+
+@pytest.mark.parametrize("dims", [(1, 10, 4, 6), (2, 6, 5, 18)])
+@pytest.mark.parametrize("cin,cout", [(8, 16), (32, 32), (64, 64), (128, 128)])
+def test_forward_conv_ragged_tiles(dims, cin, cout):
+    """The stride-1 forward layers of the persistent kernels (e0: 8 -> 16, e1b: 32 -> 32,
+    e2b: 64 -> 64 and e3b/e3c: 128 -> 128 in 32-channel blocks) on
+    extents that leave ragged tiles in every axis, fp32 and leaky bf16 outputs."""
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    n, D, H, W = dims
+    rng = np.random.default_rng(D * W + cin)
+    x = bf(rng.normal(size=(n, D, H, W, cin)))
+    w = bf(rng.normal(size=(cout, 27, cin)) / np.sqrt(27 * cin))
+    b = rng.normal(size=cout).astype(np.float32).astype(np.float64)
+    ref = no.conv_fwd(x, w, b, 1)
+    yd = t.empty(n * D * H * W * cout, dtype=t.float32, device="cuda")
+    rc = L.fc_nn_conv(0, ptr(dev_bf16(x)), None, ptr(dev_bf16(w)), ptr(dev_f32(b)), None, ptr(yd), n, D, H, W, D, H,
+                      W, cin, 0, cout, 1, 0, stream())
+    assert rc == 0
+    assert rel(yd.cpu().numpy().reshape(ref.shape), ref) < 1e-4
+    yb = t.empty(n * D * H * W * cout, dtype=t.bfloat16, device="cuda")
+    rc = L.fc_nn_conv(0, ptr(dev_bf16(x)), None, ptr(dev_bf16(w)), ptr(dev_f32(b)), ptr(yb), None, n, D, H, W, D, H,
+                      W, cin, 0, cout, 1, 1, stream())
+    assert rc == 0
+    assert rel(yb.float().cpu().numpy().reshape(ref.shape), no.leaky(ref)) < 2 ** -7
 
 
 @pytest.mark.parametrize("c1,c2,cout", [(32, 16, 16), (64, 32, 32), (128, 64, 64)])
@@ -344,18 +403,23 @@ def test_network_layers_match_fp64_oracle_teacher_forced():
         assert e_w < 1e-3 and e_b < 1e-3, (name, e_w, e_b)
         if name == "e0":
             continue
+        # each contribution with the magnitude that bounds its rounding errors
         if mode == 0:
-            contrib.setdefault(ins[0], []).append(no.bf16(no.conv_adj(dz, w, src.shape, stride)))
+            dx = no.bf16(no.conv_adj(dz, w, src.shape, stride))
+            contrib.setdefault(ins[0], []).append((dx, np.abs(dx)))
         else:
             dcat = no.bf16(no.conv_adj(dz, w, src.shape, 1))
-            contrib.setdefault(ins[1], []).append(dcat[..., c1:])
-            contrib.setdefault(ins[0], []).append(no.bf16(no.down2_sum(dcat[..., :c1])))
+            contrib.setdefault(ins[1], []).append((dcat[..., c1:], np.abs(dcat[..., c1:])))
+            # the coarse part sums 8 rounded children: their errors add up to 2^-8 times the
+            # sum of their magnitudes, which cancellation can make far larger than |sum|
+            contrib.setdefault(ins[0], []).append((no.bf16(no.down2_sum(dcat[..., :c1])),
+                                                   no.down2_sum(np.abs(dcat[..., :c1]))))
     for key, parts in contrib.items():
         # summed in the GPU's order; each rounding is bounded by the operands' magnitude
-        acc, mag = parts[0], np.abs(parts[0])
-        for v in parts[1:]:
+        acc, mag = parts[0]
+        for v, m in parts[1:]:
             acc = no.bf16(acc + v)
-            mag = mag + np.abs(v)
+            mag = mag + m
         r = _within_bf16(gg[key], acc, 2.02, mag)
         print(key, f"data gradient {r:.3f} of tolerance")
         assert r <= 1.0, (key, r)
@@ -366,8 +430,10 @@ def test_network_gradients_end_to_end_sanity():
     and with the GPU's bf16 rounding points emulated.  A sanity bound only: rounding
     flips compound through the 11 layers and, at this tiny size, the deepest levels hold
     4-16 voxels, so the free-running gradients differ by up to a few percent from the
-    emulation (2.0e-2 measured at e3b) and ~10% from exact arithmetic.  Mis-wired
-    layers (a transposed weight, a dropped skip gradient) give O(1) errors."""
+    emulation and ~10% from exact arithmetic; a layer may differ from the emulation by
+    5%, or by twice the emulation's own distance from exact arithmetic where that is
+    larger (two independent rounding histories).  Mis-wired layers (a transposed weight,
+    a dropped skip gradient) give O(1) errors."""
     from flowcraft_amd.surrogate import LAYERS, Surrogate
     t = torch_mod()
     net = Surrogate(seed=3)
@@ -397,6 +463,10 @@ def test_network_gradients_end_to_end_sanity():
             e = np.linalg.norm(gv - r) / max(np.linalg.norm(r), 1e-30)
             ex = exact[f"{name}.{kind}"]
             e2 = np.linalg.norm(gv - ex) / max(np.linalg.norm(ex), 1e-30)
-            print(name, kind, f"vs emulated {e:.3e}  vs exact {e2:.3e}")
-            assert e < 5e-2, (name, kind, e)
+            # the emulation's own distance from exact arithmetic is the size of one
+            # independent rounding history; the GPU's (another history) may differ from it by
+            # about twice that
+            eq = np.linalg.norm(r - ex) / max(np.linalg.norm(ex), 1e-30)
+            print(name, kind, f"vs emulated {e:.3e}  vs exact {e2:.3e}  (emulated vs exact {eq:.3e})")
+            assert e < max(5e-2, 2.0 * eq), (name, kind, e, eq)
             assert e2 < 2.5e-1, (name, kind, e2)
