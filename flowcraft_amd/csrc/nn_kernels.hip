@@ -139,9 +139,10 @@ __global__ __launch_bounds__(256) void KConv(const ConvArgs a) {
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int kc = 0; kc < nchunks; ++kc) {
+    // K chunk kc's operands; those of chunk kc+1 are loaded before chunk kc's MFMAs, so the
+    // gathers' latency overlaps one chunk of matrix work
+    auto load = [&](int kc, bf16x8 (&bfr)[TN], bf16x8 (&afr)[TM]) {
         const int k0 = kc * 32 + kq;
-        bf16x8 bfr[TN];
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
             uint4 raw = make_uint4(0u, 0u, 0u, 0u);
@@ -150,10 +151,22 @@ __global__ __launch_bounds__(256) void KConv(const ConvArgs a) {
             __builtin_memcpy(&bfr[j], &raw, 16);
         }
 #pragma unroll
-        for (int m = 0; m < TM; ++m) {
-            const bf16x8 afr = GatherA<MODE>(a, vn[m], vz[m], vy[m], vx[m], k0, vok[m]);
+        for (int m = 0; m < TM; ++m) afr[m] = GatherA<MODE>(a, vn[m], vz[m], vy[m], vx[m], k0, vok[m]);
+    };
+    bf16x8 bcur[TN], acur[TM];
+    load(0, bcur, acur);
+    for (int kc = 0; kc < nchunks; ++kc) {
+        bf16x8 bnext[TN], anext[TM];
+        if (kc + 1 < nchunks) load(kc + 1, bnext, anext);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+        for (int m = 0; m < TM; ++m)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[m], bcur[j], acc[m][j], 0, 0, 0);
+        if (kc + 1 < nchunks) {
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bcur[j] = bnext[j];
+#pragma unroll
+            for (int m = 0; m < TM; ++m) acur[m] = anext[m];
         }
     }
     // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
