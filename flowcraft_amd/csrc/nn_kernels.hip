@@ -106,6 +106,38 @@ __device__ __forceinline__ bf16x8 GatherA(const ConvArgs& a, int n, int z, int y
     return r;
 }
 
+// Bias, activation and store of output channels c0 .. c0+3 of voxel v (those below a.nout),
+// the four results a lane holds when the weights are the MFMA's first operand (rows = output
+// channels): one 8-byte (bf16) or 16-byte (fp32) store when a.nout is a multiple of 4.
+__device__ __forceinline__ void StoreC4(const ConvArgs& a, long long v, int c0, const f32x4& acc) {
+    if (c0 >= a.nout) return;
+    float val[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        val[q] = acc[q] + (a.bias && c0 + q < a.nout ? a.bias[c0 + q] : 0.f);
+        if (a.act == 1) val[q] = val[q] > 0.f ? val[q] : kLeak * val[q];
+    }
+    const long long o = v * a.nout + c0;
+    if ((a.nout & 3) == 0) {  // then c0 + 3 < nout, and o is a multiple of 4
+        if (a.outf) {
+            *reinterpret_cast<float4*>(a.outf + o) = make_float4(val[0], val[1], val[2], val[3]);
+        } else {
+            const uint2 pk = make_uint2(static_cast<uint32_t>(F2Bf(val[0])) | (static_cast<uint32_t>(F2Bf(val[1])) << 16),
+                                        static_cast<uint32_t>(F2Bf(val[2])) | (static_cast<uint32_t>(F2Bf(val[3])) << 16));
+            *reinterpret_cast<uint2*>(a.out + o) = pk;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (c0 + q >= a.nout) continue;
+            if (a.outf)
+                a.outf[o + q] = val[q];
+            else
+                a.out[o + q] = F2Bf(val[q]);
+        }
+    }
+}
+
 // One wave computes (16*TM) destination voxels x (16*TN) output channels; a block is
 // 4 waves along the voxel axis.
 template <int MODE, int TM, int TN>
@@ -161,7 +193,7 @@ __global__ __launch_bounds__(256) void KConv(const ConvArgs a) {
 #pragma unroll
         for (int m = 0; m < TM; ++m)
 #pragma unroll
-            for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[m], bcur[j], acc[m][j], 0, 0, 0);
+            for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bcur[j], acur[m], acc[m][j], 0, 0, 0);
         if (kc + 1 < nchunks) {
 #pragma unroll
             for (int j = 0; j < TN; ++j) bcur[j] = bnext[j];
@@ -169,26 +201,15 @@ __global__ __launch_bounds__(256) void KConv(const ConvArgs a) {
             for (int m = 0; m < TM; ++m) acur[m] = anext[m];
         }
     }
-    // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+    // epilogue: the weights are the MFMA's first operand, so lane (row, g) holds output
+    // channels 16j + 4g .. +3 of voxel vbase + 16m + row
 #pragma unroll
-    for (int m = 0; m < TM; ++m)
+    for (int m = 0; m < TM; ++m) {
+        const long long v = vbase + m * 16 + row;
+        if (v >= V) continue;
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int n = nbase + j * 16 + (lane & 15);
-            if (n >= a.nout) continue;
-            const float b = a.bias ? a.bias[n] : 0.f;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const long long v = vbase + m * 16 + (lane >> 4) * 4 + r;
-                if (v >= V) continue;
-                float y = acc[m][j][r] + b;
-                if (a.act == 1) y = y > 0.f ? y : kLeak * y;
-                if (a.outf)
-                    a.outf[v * a.nout + n] = y;
-                else
-                    a.out[v * a.nout + n] = F2Bf(y);
-            }
-        }
+        for (int j = 0; j < TN; ++j) StoreC4(a, v, nbase + j * 16 + (lane >> 4) * 4, acc[m][j]);
+    }
 }
 
 // Adjoint (data gradient) of a stride-2 convolution, split by output parity.  Along one
@@ -809,7 +830,7 @@ __device__ __forceinline__ void ChunkGemmRuns(const ConvArgs& a, const u16* sX, 
             bf16x8 afr;
             __builtin_memcpy(&afr, &acur[m], 16);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+            for (int j = 0; j < TN; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j], afr, acc[m][j], 0, 0, 0);
         }
 #pragma unroll
         for (int j = 0; j < TN; ++j) bcur[j] = bnext[j];
@@ -881,7 +902,7 @@ __device__ __forceinline__ void ChunkGemmRows16(const ConvArgs& a, const u16* sX
             __builtin_memcpy(&bfr, &bcur[ky], 16);
 #pragma unroll
             for (int m = 0; m < 4; ++m)
-                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[m + oy], bfr, acc[m][0], 0, 0, 0);
+                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr, afr[m + oy], acc[m][0], 0, 0, 0);
         }
 #pragma unroll
         for (int ky = 0; ky < 3; ++ky) bcur[ky] = bnext[ky];
@@ -947,29 +968,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3
         else
             ConvChunk<MODE, TN, R3, CS, LDSW>(a, sX, sW, n, x0, y0, z0, cb, wave, row, grp, nbase, K, acc);
     }
-    // epilogue: lane holds output rows 4*(lane/16)+r (x within the tile row), column lane%16
-    const int z = z0 + wave;
+    // epilogue: the weights are the MFMA's first operand, so lane (row, grp) holds output
+    // channels 16j + 4grp .. +3 of voxel x0 + row
+    const int z = z0 + wave, x = x0 + row;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4; ++m) {
+        const int y = y0 + m;
+        if (x >= a.wd || y >= a.hd || z >= a.dd) continue;
+        const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int nn = nbase + j * 16 + row;
-            const int y = y0 + m;
-            if (nn >= a.nout || y >= a.hd || z >= a.dd) continue;
-            const float bb = a.bias ? a.bias[nn] : 0.f;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int x = x0 + grp * 4 + q;
-                if (x >= a.wd) continue;
-                const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
-                float val = acc[m][j][q] + bb;
-                if (a.act == 1) val = val > 0.f ? val : kLeak * val;
-                if (a.outf)
-                    a.outf[v * a.nout + nn] = val;
-                else
-                    a.out[v * a.nout + nn] = F2Bf(val);
-            }
-        }
+        for (int j = 0; j < TN; ++j) StoreC4(a, v, nbase + j * 16 + 4 * grp, acc[m][j]);
+    }
 }
 
 // Persistent convolution for the network's stride-1 layers of 16 to 64 output channels
@@ -1234,7 +1243,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
 // row would be read at a 32-byte stride; and the run's weights as [tap][output channel][8].
 // The next run's halo and weights are loaded into registers while the current run's matrix
 // loop runs (which reads only LDS, so no wait on those loads stalls it) and written to LDS
-// between two barriers.  The implicit GEMM of a run takes four taps per 32-wide K step (lane
+// between two barriers.  The weights are the MFMA's first operand, so a lane ends with 4
+// consecutive output channels of one voxel (8- or 16-byte stores).  The implicit GEMM of a run takes four taps per 32-wide K step (lane
 // group g takes tap 4s + g; the 28th is zero).  Every input voxel is read from global
 // memory about 1.45 times per run (the halo overlap) instead of once per tap.
 constexpr int kS2X = 2 * kTX + 1, kS2Y = 2 * kTY + 1, kS2Z = 2 * kTZ + 1;  // 33 x 9 x 9 input halo
@@ -1348,7 +1358,7 @@ __global__ __launch_bounds__(256) void KConvTileS2(const ConvArgs a) {
                 __builtin_memcpy(&afr, &acur[m], 16);
 #pragma unroll
                 for (int j = 0; j < TN; ++j)
-                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j], afr, acc[m][j], 0, 0, 0);
             }
             if (s + 1 < kSteps) {
 #pragma unroll
@@ -1358,29 +1368,16 @@ __global__ __launch_bounds__(256) void KConvTileS2(const ConvArgs a) {
             }
         }
     }
-    // epilogue: lane holds output rows 4*grp + q (x within the tile row), column row
-    const int z = z0 + wave;
+    // epilogue: lane (row, grp) holds output channels 16j + 4grp .. +3 of voxel x0 + row
+    const int z = z0 + wave, x = x0 + row;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4; ++m) {
+        const int y = y0 + m;
+        if (x >= a.wd || y >= a.hd || z >= a.dd) continue;
+        const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int nn = nbase + j * 16 + row;
-            const int y = y0 + m;
-            if (nn >= a.nout || y >= a.hd || z >= a.dd) continue;
-            const float bb = a.bias ? a.bias[nn] : 0.f;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int x = x0 + grp * 4 + q;
-                if (x >= a.wd) continue;
-                const long long v = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
-                float val = acc[m][j][q] + bb;
-                if (a.act == 1) val = val > 0.f ? val : kLeak * val;
-                if (a.outf)
-                    a.outf[v * a.nout + nn] = val;
-                else
-                    a.out[v * a.nout + nn] = F2Bf(val);
-            }
-        }
+        for (int j = 0; j < TN; ++j) StoreC4(a, v, nbase + j * 16 + 4 * grp, acc[m][j]);
+    }
 }
 
 // Workgroups of a persistent launch: the CUs times the kernel's resident workgroups per CU.

@@ -154,13 +154,13 @@ def test_stride2_adjoint_odd_dims(cin, cout):
     assert rel(dxd.cpu().numpy().reshape(n, D, H, W, cin), no.conv_adj(dy, w, (n, D, H, W, cin), 2)) < 1e-4
 
 
-@pytest.mark.parametrize("cin,cout", [(8, 16), (16, 32), (8, 48), (16, 80), (32, 64), (64, 128)])
+@pytest.mark.parametrize("cin,cout", [(8, 16), (16, 32), (8, 18), (8, 48), (16, 80), (32, 64), (64, 128)])
 def test_stride2_forward_several_tiles(cin, cout):
     """Stride-2 forward on extents that span several 16 x 4 x 4 output tiles with ragged ones
     in every axis (output 6 x 7 x 19 from 11 x 13 x 37: odd input extents, so the last output
     voxel's kx = 2 tap falls outside the volume), fp32 and leaky bf16: the LDS-tiled kernel
-    (inputs of 8 or 16 channels; 48 and 80 outputs leave a partly filled 64-channel block)
-    and the direct gather (wider inputs)."""
+    (inputs of 8 or 16 channels; 48 and 80 outputs leave a partly filled 64-channel block,
+    18 takes the scalar stores) and the direct gather (wider inputs)."""
     from flowcraft_amd.surrogate import nn_lib
     t = torch_mod()
     L = nn_lib()
