@@ -988,7 +988,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3
 // selects the block of CO channels).  A workgroup loops over 16 (x) x 4 (y) x 4*ZW (z) output
 // tiles; wave w owns z slices w*ZW .. w*ZW+ZW-1, each as four MFMA tiles of 16 x-voxels
 // (one per y row) by the CO output channels.  Input channels go in chunks of RC 8-channel
-// runs (RC = 2 unless the layer has one run).  Each chunk's halo (18 x 6 x (4*ZW+2) voxels)
+// runs (RC = RCH, 2 or 3, unless the layer has fewer runs; 48 bytes per halo voxel hold two
+// runs and a pad, or three runs, either way a stride at which the 16 lanes of a fragment
+// read hit distinct banks).  Each chunk's halo (18 x 6 x (4*ZW+2) voxels)
 // and weights are loaded into registers while the previous chunk's matrix loop runs and
 // written to LDS between two barriers; the matrix loop reads only LDS.  (Weights read from
 // global memory inside the loop, issued after the halo prefetch, make every wait on them a
@@ -1001,17 +1003,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3
 // halo address and zero weights, so the loop has no branches: with 16 output channels the
 // weights come from a zero block in LDS (no extra instructions: d0 0.97 against 1.00 ms
 // with selects), with 32 the fragments are replaced by zeros (the block would take LDS
-// that the second workgroup per CU needs).  The instruction's operands are the weights (rows = output channels) and the
-// input (columns = voxels), so a lane ends with 4 consecutive output channels of one voxel
-// per 16-channel tile and the epilogue writes 8 bytes (bf16) or 16 bytes (fp32) per lane
-// and tile.
-template <int MODE, int NRUNS, int ZW, int NT>
+// that the second workgroup per CU needs).  The instruction's operands are the weights
+// (rows = output channels) and the input (columns = voxels), so a lane ends with 4
+// consecutive output channels of one voxel per 16-channel tile and the epilogue writes 8
+// bytes (bf16) or 16 bytes (fp32) per lane and tile.
+template <int MODE, int NRUNS, int ZW, int NT, int RCH = 2>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void KConvTileP(const ConvArgs a) {
     static_assert(MODE == kFwd || MODE == kUpcat, "forward gathers only");
     constexpr int CO = 16 * NT;                        // output channels of the workgroup
-    constexpr int CS = HaloStride(2);                  // 24 bf16 (48 B) per halo voxel
+    static_assert(RCH == 2 || RCH == 3, "chunks of 2 or 3 runs");
+    constexpr int CS = 24;                             // 48 B per halo voxel (2 or 3 runs)
     constexpr int TZ = 4 * ZW, HZ = TZ + 2, HV = kHX * kHY * HZ;
-    constexpr int RC = NRUNS < 2 ? NRUNS : 2;          // 8-channel runs per chunk
+    constexpr int RC = NRUNS < RCH ? NRUNS : RCH;      // 8-channel runs per chunk
     static_assert(NRUNS % RC == 0, "whole chunks");
     constexpr int kChunks = NRUNS / RC;
     constexpr int kPF = (HV * RC + 255) / 256;         // halo prefetch registers (16 B each)
@@ -1549,8 +1552,8 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
             // z depth per tile (ZW) as measured fastest per layer at batch 4 x 256x128x128
             if (a.nout == 16 && a.kin == 8)
                 persistent(KConvTileP<MODE, 1, 2, 1>, 2, 16);
-            else if (a.nout == 16 && a.kin == 48)
-                persistent(KConvTileP<MODE, 6, 2, 1>, 2, 16);
+            else if (a.nout == 16 && a.kin == 48)  // two chunks of 24 channels: 3.6% of the K steps
+                persistent(KConvTileP<MODE, 6, 2, 1, 3>, 2, 16);  // padded instead of 10% with 16
             else if (a.nout == 32 && a.kin == 32 && MODE == kFwd)
                 persistent(KConvTileP<MODE, 4, 1, 2>, 1, 32);
             else if (a.nout == 32 && a.kin == 96)

@@ -328,9 +328,10 @@ def field_from_prediction(pred, u_in, V):
 
 
 def time_layers(nb=4, dims=(128, 128, 256), reps=10, train_reps=5, log=print):
-    """Times each forward convolution of the network (HIP events over `reps` launches)
-    and a whole training step on random inputs of nb samples of dims = (D, H, W);
-    returns {"layers": [(name, voxels, kin, cout, ms, TFLOP/s)], "train_step_ms", "fwd_flops"}."""
+    """Times each forward convolution of the network (HIP events over `reps` launches),
+    the forward pass (the convolutions back to back, `reps` passes) and a whole training
+    step on random inputs of nb samples of dims = (D, H, W); returns {"layers": [(name,
+    voxels, kin, cout, ms, TFLOP/s)], "forward_ms", "train_step_ms", "fwd_flops"}."""
     import torch
     L = nn_lib()
     net = Surrogate(seed=0)
@@ -365,8 +366,21 @@ def time_layers(nb=4, dims=(128, 128, 256), reps=10, train_reps=5, log=print):
         tot_f += flops
         rows.append((name, V, kin, cout, ms, flops / ms / 1e9))
         log(f"{name:4s} V={V:9d} K={27 * kin:5d} N={cout:4d}  {ms:8.3f} ms  {flops / ms / 1e9:8.1f} TFLOP/s")
+    # the forward pass as the network runs it: the 11 convolutions back to back
+    for _ in range(2):
+        net.forward(x, nb, dims)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        net.forward(x, nb, dims)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    log(f"forward pass (11 convolutions back to back): {fwd_ms:.3f} ms  {tot_f / fwd_ms / 1e9:8.1f} TFLOP/s; "
+        f"layers timed one by one: {sum(r[4] for r in rows):.3f} ms")
     if train_reps <= 0:
-        return {"layers": rows, "train_step_ms": None, "fwd_flops": tot_f}
+        return {"layers": rows, "train_step_ms": None, "fwd_flops": tot_f, "forward_ms": fwd_ms}
     tgt = torch.zeros(V0 * 4, device=net.dev)
     wm = torch.ones(V0, device=net.dev)
     for _ in range(2):
@@ -380,7 +394,7 @@ def time_layers(nb=4, dims=(128, 128, 256), reps=10, train_reps=5, log=print):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / train_reps
     log(f"train step (batch {nb}): {ms:.2f} ms; 3 x forward FLOPs -> {3 * tot_f / ms / 1e9:.1f} TFLOP/s")
-    return {"layers": rows, "train_step_ms": ms, "fwd_flops": tot_f}
+    return {"layers": rows, "train_step_ms": ms, "fwd_flops": tot_f, "forward_ms": fwd_ms}
 
 
 if __name__ == "__main__":
