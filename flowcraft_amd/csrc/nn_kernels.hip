@@ -468,14 +468,16 @@ __global__ void KLeakyBackward8(const u16* __restrict__ y, const u16* __restrict
 }
 
 // Wt[ci][t][co] = W[co][t][ci]
-__global__ void KTransposeW(const u16* __restrict__ w, u16* __restrict__ wt, int cout, int cin) {
+// With flip, Wt[ci][26 - t][co] = W[co][t][ci]: the weights of the forward convolution of
+// dZ that equals the stride-1 adjoint (tap (kz, ky, kx) mirrored to (2-kz, 2-ky, 2-kx)).
+__global__ void KTransposeW(const u16* __restrict__ w, u16* __restrict__ wt, int cout, int cin, int flip) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long n = static_cast<long long>(cout) * 27 * cin;
     if (i >= n) return;
     const int ci = static_cast<int>(i % cin);
     const int t = static_cast<int>((i / cin) % 27);
     const int co = static_cast<int>(i / (27LL * cin));
-    wt[(static_cast<long long>(ci) * 27 + t) * cout + co] = w[i];
+    wt[(static_cast<long long>(ci) * 27 + (flip ? 26 - t : t)) * cout + co] = w[i];
 }
 
 // Split the data gradient of a kUpcat conv input ([V][c1+c2] at full resolution):
@@ -1550,19 +1552,23 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
                 done = true;
             };
             // z depth per tile (ZW) as measured fastest per layer at batch 4 x 256x128x128
+            // (the forward-mode shapes with 16 to 128 input channels also serve the stride-1
+            // adjoints, run as forward convolutions with mirrored weights)
             if (a.nout == 16 && a.kin == 8)
                 persistent(KConvTileP<MODE, 1, 2, 1>, 2, 16);
             else if (a.nout == 16 && a.kin == 48)  // two chunks of 24 channels: 3.6% of the K steps
                 persistent(KConvTileP<MODE, 6, 2, 1, 3>, 2, 16);  // padded instead of 10% with 16
-            else if (a.nout == 32 && a.kin == 32 && MODE == kFwd)
-                persistent(KConvTileP<MODE, 4, 1, 2>, 1, 32);
-            else if (a.nout == 32 && a.kin == 96)
+            else if (MODE == kUpcat && a.nout == 32 && a.kin == 96)
                 persistent(KConvTileP<MODE, 12, 2, 2>, 2, 32);
-            else if (a.nout == 64 && a.kin == 64 && MODE == kFwd)
-                persistent(KConvTileP<MODE, 8, 1, 2>, 1, 32);
-            else if (a.nout == 64 && a.kin == 192)
+            else if (MODE == kUpcat && a.nout == 64 && a.kin == 192)
                 persistent(KConvTileP<MODE, 24, 2, 2>, 2, 32);
-            else if (a.nout == 128 && a.kin == 128 && MODE == kFwd)
+            else if (MODE == kFwd && a.nout == 48 && a.kin == 16)
+                persistent(KConvTileP<MODE, 2, 1, 3>, 1, 48);
+            else if (MODE == kFwd && a.nout % 32 == 0 && a.kin == 32)
+                persistent(KConvTileP<MODE, 4, 1, 2>, 1, 32);
+            else if (MODE == kFwd && a.nout % 32 == 0 && a.kin == 64)
+                persistent(KConvTileP<MODE, 8, 1, 2>, 1, 32);
+            else if (MODE == kFwd && a.nout % 32 == 0 && a.kin == 128)
                 persistent(KConvTileP<MODE, 16, 1, 2>, 1, 32);
         }
         if (done) {
@@ -1712,7 +1718,16 @@ int fc_nn_leaky_backward(const void* y, const void* dy, void* dz, long long n, v
 int fc_nn_transpose_w(const void* w, void* wt, int cout, int cin, void* stream) {
     const long long n = static_cast<long long>(cout) * 27 * cin;
     hipLaunchKernelGGL(KTransposeW, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const u16*>(w), static_cast<u16*>(wt), cout, cin);
+                       static_cast<const u16*>(w), static_cast<u16*>(wt), cout, cin, 0);
+    return static_cast<int>(hipGetLastError());
+}
+
+// Wt[ci][26 - t][co] = W[co][t][ci]: a mode-0 (forward, stride 1) convolution of dZ with Wt
+// is the stride-1 adjoint, and runs on the forward kernels.
+int fc_nn_flip_transpose_w(const void* w, void* wt, int cout, int cin, void* stream) {
+    const long long n = static_cast<long long>(cout) * 27 * cin;
+    hipLaunchKernelGGL(KTransposeW, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const u16*>(w), static_cast<u16*>(wt), cout, cin, 1);
     return static_cast<int>(hipGetLastError());
 }
 

@@ -67,6 +67,7 @@ def nn_lib() -> ctypes.CDLL:
     L.fc_nn_colsum.argtypes = [vp, ll, i, vp, vp, i, vp]
     L.fc_nn_leaky_backward.argtypes = [vp, vp, vp, ll, vp]
     L.fc_nn_transpose_w.argtypes = [vp, vp, i, i, vp]
+    L.fc_nn_flip_transpose_w.argtypes = [vp, vp, i, i, vp]
     L.fc_nn_upcat_split.argtypes = [vp, i, i, i, i, i, i, vp, vp, vp]
     L.fc_nn_head_loss.argtypes = [vp, i, vp, vp, vp, vp, ll, f, vp, vp, vp, vp, vp]
     L.fc_nn_adam.argtypes = [vp, vp, vp, vp, vp, ll, f, f, f, f, i, f, vp]
@@ -251,15 +252,24 @@ class Surrogate:
             _chk(L.fc_nn_colsum(_p(dz), V, cout, _p(p["gb"]), _p(cws), 0, s), name + " bias")
             if name == "e0":
                 continue  # no gradient needed for the input features
-            # data gradient through the transposed weights
+            # data gradient through the transposed weights: the stride-2 adjoint gathers with
+            # them; a stride-1 adjoint is the forward convolution of dZ with the transposed,
+            # mirrored weights, and runs on the forward kernels
             wt = t.empty_like(p["wbf"])
-            _chk(L.fc_nn_transpose_w(_p(p["wbf"]), _p(wt), cout, kin, s), "transpose")
+            if stride == 2:
+                _chk(L.fc_nn_transpose_w(_p(p["wbf"]), _p(wt), cout, kin, s), "transpose")
+            else:
+                _chk(L.fc_nn_flip_transpose_w(_p(p["wbf"]), _p(wt), cout, kin, s), "transpose")
             if mode == FWD:
                 src = ins[0]
                 si = self.level_dims(dims, lvl - (1 if stride == 2 else 0))
                 dx = t.empty(nb * si[0] * si[1] * si[2] * kin, dtype=t.bfloat16, device=self.dev)
-                _chk(L.fc_nn_conv(ADJ, _p(dz), None, _p(wt), None, _p(dx), None, nb, dd, hd, wd, si[0], si[1], si[2],
-                                  cout, 0, kin, stride, 0, s), name + " dgrad")
+                if stride == 2:
+                    _chk(L.fc_nn_conv(ADJ, _p(dz), None, _p(wt), None, _p(dx), None, nb, dd, hd, wd, si[0], si[1],
+                                      si[2], cout, 0, kin, stride, 0, s), name + " dgrad")
+                else:
+                    _chk(L.fc_nn_conv(FWD, _p(dz), None, _p(wt), None, _p(dx), None, nb, dd, hd, wd, dd, hd, wd,
+                                      cout, 0, kin, 1, 0, s), name + " dgrad")
                 if src in grads:
                     _chk(L.fc_nn_add_bf16(_p(dx), _p(grads[src]), dx.numel(), s), "add")
                 else:
@@ -267,7 +277,7 @@ class Surrogate:
             else:
                 coarse, skip = ins
                 dcat = t.empty(V * kin, dtype=t.bfloat16, device=self.dev)
-                _chk(L.fc_nn_conv(ADJ, _p(dz), None, _p(wt), None, _p(dcat), None, nb, dd, hd, wd, dd, hd, wd,
+                _chk(L.fc_nn_conv(FWD, _p(dz), None, _p(wt), None, _p(dcat), None, nb, dd, hd, wd, dd, hd, wd,
                                   cout, 0, kin, 1, 0, s), name + " dgrad")
                 if skip not in grads:
                     grads[skip] = t.zeros(V * (kin - c1), dtype=t.bfloat16, device=self.dev)

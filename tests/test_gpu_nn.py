@@ -236,6 +236,34 @@ def test_forward_conv_ragged_tiles(dims, cin, cout):
     assert rel(yb.float().cpu().numpy().reshape(ref.shape), no.leaky(ref)) < 2 ** -7
 
 
+@pytest.mark.parametrize("cout,cin", [(16, 48), (32, 96), (64, 192), (32, 32), (128, 128), (16, 8)])
+def test_stride1_adjoint_as_mirrored_forward(cout, cin):
+    """The stride-1 data gradient the training step computes: a forward convolution of dZ
+    (cout channels) with the transposed, mirrored weights (fc_nn_flip_transpose_w), on the
+    adjoint shapes of d0, d1, d2, e1b, e3b and e0 and extents that leave ragged tiles, against
+    the oracle's adjoint and against the adjoint-mode kernel."""
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    rng = np.random.default_rng(cout * 1000 + cin)
+    n, D, H, W = 2, 6, 5, 18
+    w = bf(rng.normal(size=(cout, 27, cin)) / np.sqrt(27 * cin))
+    dy = bf(rng.normal(size=(n, D, H, W, cout)))
+    ref = no.conv_adj(dy, w, (n, D, H, W, cin), 1)
+    wf = t.empty(cout * 27 * cin, dtype=t.bfloat16, device="cuda")
+    assert L.fc_nn_flip_transpose_w(ptr(dev_bf16(w)), ptr(wf), cout, cin, stream()) == 0
+    got = t.empty(n * D * H * W * cin, dtype=t.float32, device="cuda")
+    assert L.fc_nn_conv(0, ptr(dev_bf16(dy)), None, ptr(wf), None, None, ptr(got), n, D, H, W, D, H, W, cout, 0,
+                        cin, 1, 0, stream()) == 0
+    assert rel(got.cpu().numpy().reshape(ref.shape), ref) < 1e-4
+    wt = t.empty(cout * 27 * cin, dtype=t.bfloat16, device="cuda")
+    assert L.fc_nn_transpose_w(ptr(dev_bf16(w)), ptr(wt), cout, cin, stream()) == 0
+    adj = t.empty(n * D * H * W * cin, dtype=t.float32, device="cuda")
+    assert L.fc_nn_conv(2, ptr(dev_bf16(dy)), None, ptr(wt), None, None, ptr(adj), n, D, H, W, D, H, W, cout, 0,
+                        cin, 1, 0, stream()) == 0
+    assert rel(got.cpu().numpy(), adj.cpu().numpy().astype(np.float64)) < 1e-4
+
+
 @pytest.mark.parametrize("c1,c2,cout", [(32, 16, 16), (64, 32, 32), (128, 64, 64)])
 def test_upcat_conv_and_split(c1, c2, cout):
     from flowcraft_amd.surrogate import nn_lib
