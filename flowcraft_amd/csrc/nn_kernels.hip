@@ -574,7 +574,9 @@ __global__ __launch_bounds__(256) void KHeadLoss(const u16* __restrict__ h, cons
                                                  const float* __restrict__ bh, const float* __restrict__ tgt,
                                                  const float* __restrict__ wmask, long long V, float inv_norm,
                                                  u16* __restrict__ dh, float* __restrict__ part, float* pred_out) {
-    // part[block][4*cin + 4 + 1]
+    // part[block][4*cin + 4 + 1]; a voxel's cin bf16 activations are cin/8 16-byte loads, its
+    // targets and prediction one float4, its activation gradient cin/8 16-byte stores
+    static_assert(cin % 8 == 0, "16-byte activation runs");
     constexpr int P = 4 * cin + 5;
     float gw[4 * cin];
     float gb[4] = {0, 0, 0, 0};
@@ -585,44 +587,65 @@ __global__ __launch_bounds__(256) void KHeadLoss(const u16* __restrict__ h, cons
          v += static_cast<long long>(kHeadBlocks) * 256) {
         float hv[cin];
 #pragma unroll
-        for (int c = 0; c < cin; ++c) hv[c] = Bf2F(h[v * cin + c]);
-        float g[4];
+        for (int r = 0; r < cin / 8; ++r) {
+            const uint4 raw = *reinterpret_cast<const uint4*>(h + v * cin + 8 * r);
+            const uint32_t w4[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                hv[8 * r + 2 * q] = Bf2F(static_cast<u16>(w4[q] & 0xFFFFu));
+                hv[8 * r + 2 * q + 1] = Bf2F(static_cast<u16>(w4[q] >> 16));
+            }
+        }
+        const float4 tv = tgt ? *reinterpret_cast<const float4*>(tgt + v * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float tg[4] = {tv.x, tv.y, tv.z, tv.w};
+        const float m = wmask ? wmask[v] : 1.f;
+        float g[4], pr[4];
 #pragma unroll
         for (int o = 0; o < 4; ++o) {
             float p = bh[o];
 #pragma unroll
             for (int c = 0; c < cin; ++c) p += wh[o * cin + c] * hv[c];
-            if (pred_out) pred_out[v * 4 + o] = p;
-            const float e = p - (tgt ? tgt[v * 4 + o] : 0.f);
-            const float m = wmask ? wmask[v] : 1.f;
+            pr[o] = p;
+            const float e = p - tg[o];
             ls += m * e * e * inv_norm;
             g[o] = 2.f * m * e * inv_norm;
             gb[o] += g[o];
 #pragma unroll
             for (int c = 0; c < cin; ++c) gw[o * cin + c] += g[o] * hv[c];
         }
-        if (dh)
+        if (pred_out) *reinterpret_cast<float4*>(pred_out + v * 4) = make_float4(pr[0], pr[1], pr[2], pr[3]);
+        if (dh) {
 #pragma unroll
-            for (int c = 0; c < cin; ++c) {
-                float s = 0.f;
+            for (int r = 0; r < cin / 8; ++r) {
+                uint32_t w4[4];
 #pragma unroll
-                for (int o = 0; o < 4; ++o) s += g[o] * wh[o * cin + c];
-                dh[v * cin + c] = F2Bf(s);
+                for (int q = 0; q < 4; ++q) {
+                    float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) {
+                        s0 += g[o] * wh[o * cin + 8 * r + 2 * q];
+                        s1 += g[o] * wh[o * cin + 8 * r + 2 * q + 1];
+                    }
+                    w4[q] = static_cast<uint32_t>(F2Bf(s0)) | (static_cast<uint32_t>(F2Bf(s1)) << 16);
+                }
+                *reinterpret_cast<uint4*>(dh + v * cin + 8 * r) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
             }
+        }
     }
-    __shared__ float red[256];
+    // block partials, in a fixed order: a butterfly over each wave's 64 lanes, then the four
+    // waves' sums in wave order
+    __shared__ float red[4][P];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         float val = k < 4 * cin ? gw[k] : (k < 4 * cin + 4 ? gb[k - 4 * cin] : ls);
-        red[threadIdx.x] = val;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[static_cast<long long>(blockIdx.x) * P + k] = red[0];
-        __syncthreads();
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off, 64);
+        if (lane == 0) red[wave][k] = val;
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < P; k += 256)
+        part[static_cast<long long>(blockIdx.x) * P + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
 }
 
 // Adam (Kingma & Ba 2015) on fp32 master weights, refreshing the bf16 copy.
