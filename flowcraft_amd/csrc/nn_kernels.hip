@@ -1037,9 +1037,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
     static_assert(MODE == kFwd || MODE == kUpcat, "forward gathers only");
     constexpr int CO = 16 * NT;                        // output channels of the workgroup
     static_assert(RCH == 2 || RCH == 3, "chunks of 2 or 3 runs");
-    constexpr int CS = 24;                             // 48 B per halo voxel (2 or 3 runs)
-    constexpr int TZ = 4 * ZW, HZ = TZ + 2, HV = kHX * kHY * HZ;
     constexpr int RC = NRUNS < RCH ? NRUNS : RCH;      // 8-channel runs per chunk
+    // bf16 per halo voxel: 48 B for 2 or 3 runs, 16 B for one (16 lanes then read 256
+    // contiguous bytes; the smaller halo leaves LDS for more workgroups per CU)
+    constexpr int CS = RC == 1 ? 8 : 24;
+    constexpr int TZ = 4 * ZW, HZ = TZ + 2, HV = kHX * kHY * HZ;
     static_assert(NRUNS % RC == 0, "whole chunks");
     constexpr int kChunks = NRUNS / RC;
     constexpr int kPF = (HV * RC + 255) / 256;         // halo prefetch registers (16 B each)
