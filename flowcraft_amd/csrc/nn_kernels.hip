@@ -1008,9 +1008,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 1 ? 3
 
 // Persistent convolution for the network's stride-1 layers of 16 to 64 output channels
 // (e0: 8 inputs -> 16; e1b: 32 -> 32; e2b: 64 -> 64; d2: 192 = 128 upsampled + 64 skip ->
-// 64; d1: 96 = 64 upsampled + 32 skip -> 32; d0: 48 = 32 upsampled + 16 skip -> 16),
-// forward and upsample-concat modes, CO = 16*NT output channels per workgroup (blockIdx.y
-// selects the block of CO channels).  A workgroup loops over 16 (x) x 4 (y) x 4*ZW (z) output
+// 64; d1: 96 = 64 upsampled + 32 skip -> 32; d0: 48 = 32 upsampled + 16 skip -> 16) and
+// the stride-1 adjoints run as forward convolutions, forward and upsample-concat modes,
+// CO = 16*NT output channels per workgroup (blockIdx.y selects the block of CO channels;
+// a.nout must be a multiple of CO, which the launcher checks: bias reads and output stores
+// are not bounded by a.nout).  A workgroup loops over 16 (x) x 4 (y) x 4*ZW (z) output
 // tiles; wave w owns z slices w*ZW .. w*ZW+ZW-1, each as four MFMA tiles of 16 x-voxels
 // (one per y row) by the CO output channels.  Input channels go in chunks of RC 8-channel
 // runs (RC = RCH, 2 or 3, unless the layer has fewer runs; 48 bytes per halo voxel hold two
@@ -1569,6 +1571,7 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
             // blocks of 16 or 32 on grid.y; the workgroups of one channel block at most one
             // per output tile (16 x 4 x 4*ZW voxels)
             auto persistent = [&](auto kern, int zw, int co) {
+                if (a.nout % co != 0) return;  // a whole number of channel blocks only
                 const unsigned ptiles = static_cast<unsigned>(a.nb) * ((a.dd + 4 * zw - 1) / (4 * zw)) *
                                         ((a.hd + kTY - 1) / kTY) * ((a.wd + kTX - 1) / kTX);
                 const unsigned blocks = Blk(a.nout, co);
