@@ -1046,7 +1046,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
     constexpr int TZ = 4 * ZW, HZ = TZ + 2, HV = kHX * kHY * HZ;
     static_assert(NRUNS % RC == 0, "whole chunks");
     constexpr int kChunks = NRUNS / RC;
-    constexpr int kPF = (HV * RC + 255) / 256;         // halo prefetch registers (16 B each)
+    constexpr int kPV = (HV + 255) / 256;              // halo voxels per thread (RC runs each)
     constexpr int WE = 27 * RC * CO;                   // weight fragments of a chunk (16 B each)
     constexpr int kPW = (WE + 255) / 256;
     constexpr int kCombos = 9 * RC, kSteps = (kCombos + 3) / 4;
@@ -1073,18 +1073,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
         z0 = (t % tiles_z) * TZ;
         n = t / tiles_z;
     };
-    uint4 pf[kPF], pw[kPW];
+    // Halo prefetch: a thread takes whole halo voxels and loads their RC runs with one address
+    // computation per voxel (the runs of a chunk are contiguous channels; for upsample-concat
+    // a run comes from the coarse or the skip tensor, the same for every lane).
+    uint4 pf[kPV][RC], pw[kPW];
     auto prefetch_halo = [&](int t, int c) {
         int n, x0, y0, z0;
         decode(t, n, x0, y0, z0);
 #pragma unroll
-        for (int j = 0; j < kPF; ++j) {
-            const int e = threadIdx.x + 256 * j;
-            pf[j] = make_uint4(0u, 0u, 0u, 0u);
-            if (e < HV * RC) {
-                const int hv = e / RC, r8 = e - hv * RC;
+        for (int j = 0; j < kPV; ++j) {
+            const int hv = threadIdx.x + 256 * j;
+#pragma unroll
+            for (int r = 0; r < RC; ++r) pf[j][r] = make_uint4(0u, 0u, 0u, 0u);
+            if (hv < HV) {
                 const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-                pf[j] = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + hx, 8 * RC * c + r8 * 8);
+                const int ix = x0 - 1 + hx, iy = y0 - 1 + hy, iz = z0 - 1 + hz;
+                if (ix >= 0 && iy >= 0 && iz >= 0 && ix < a.wi && iy < a.hi && iz < a.di) {
+                    const int cs = MODE == kUpcat ? a.kin - a.c1 : a.kin;
+                    const u16* fine = a.in + (((static_cast<long long>(n) * a.di + iz) * a.hi + iy) * a.wi + ix) * cs;
+                    const u16* coarse = nullptr;
+                    if constexpr (MODE == kUpcat)
+                        coarse = a.in2 + (((static_cast<long long>(n) * (a.di >> 1) + (iz >> 1)) * (a.hi >> 1) + (iy >> 1)) *
+                                              (a.wi >> 1) + (ix >> 1)) * a.c1;
+#pragma unroll
+                    for (int r = 0; r < RC; ++r) {
+                        const int ci = 8 * RC * c + 8 * r;
+                        if (MODE == kUpcat && ci < a.c1)
+                            pf[j][r] = *reinterpret_cast<const uint4*>(coarse + ci);
+                        else
+                            pf[j][r] = *reinterpret_cast<const uint4*>(fine + (MODE == kUpcat ? ci - a.c1 : ci));
+                    }
+                }
             }
         }
     };
@@ -1104,11 +1123,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
     };
     auto commit_halo = [&]() {
 #pragma unroll
-        for (int j = 0; j < kPF; ++j) {
-            const int e = threadIdx.x + 256 * j;
-            if (e < HV * RC) {
-                const int hv = e / RC, r8 = e - hv * RC;
-                *reinterpret_cast<uint4*>(&sX[hv * CS + r8 * 8]) = pf[j];
+        for (int j = 0; j < kPV; ++j) {
+            const int hv = threadIdx.x + 256 * j;
+            if (hv < HV) {
+#pragma unroll
+                for (int r = 0; r < RC; ++r) *reinterpret_cast<uint4*>(&sX[hv * CS + r * 8]) = pf[j][r];
             }
         }
     };
