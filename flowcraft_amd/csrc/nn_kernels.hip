@@ -45,6 +45,14 @@ __device__ __forceinline__ u16 F2Bf(float f) {
     return static_cast<u16>(b >> 16);
 }
 
+// Two floats to two bf16 (round to nearest even) in one word, low half first: one
+// v_cvt_pk_bf16_f32 on gfx950 (F2Bf spells the same rounding out for finite values).
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t PackBf2(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{lo, hi}, bf16x2_t));
+}
+
 struct ConvArgs {
     const u16* in;    // kFwd/kAdj: input; kUpcat: full-resolution part
     const u16* in2;   // kUpcat: half-resolution part
@@ -122,9 +130,7 @@ __device__ __forceinline__ void StoreC4(const ConvArgs& a, long long v, int c0, 
         if (a.outf) {
             *reinterpret_cast<float4*>(a.outf + o) = make_float4(val[0], val[1], val[2], val[3]);
         } else {
-            const uint2 pk = make_uint2(static_cast<uint32_t>(F2Bf(val[0])) | (static_cast<uint32_t>(F2Bf(val[1])) << 16),
-                                        static_cast<uint32_t>(F2Bf(val[2])) | (static_cast<uint32_t>(F2Bf(val[3])) << 16));
-            *reinterpret_cast<uint2*>(a.out + o) = pk;
+            *reinterpret_cast<uint2*>(a.out + o) = make_uint2(PackBf2(val[0], val[1]), PackBf2(val[2], val[3]));
         }
     } else {
 #pragma unroll
@@ -626,7 +632,7 @@ __global__ __launch_bounds__(256) void KHeadLoss(const u16* __restrict__ h, cons
                         s0 += g[o] * wh[o * cin + 8 * r + 2 * q];
                         s1 += g[o] * wh[o * cin + 8 * r + 2 * q + 1];
                     }
-                    w4[q] = static_cast<uint32_t>(F2Bf(s0)) | (static_cast<uint32_t>(F2Bf(s1)) << 16);
+                    w4[q] = PackBf2(s0, s1);
                 }
                 *reinterpret_cast<uint4*>(dh + v * cin + 8 * r) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
             }
@@ -1271,9 +1277,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
                     if (a.outf) {
                         *reinterpret_cast<float4*>(a.outf + o) = make_float4(val[0], val[1], val[2], val[3]);
                     } else {
-                        const uint2 pk = make_uint2(static_cast<uint32_t>(F2Bf(val[0])) | (static_cast<uint32_t>(F2Bf(val[1])) << 16),
-                                                    static_cast<uint32_t>(F2Bf(val[2])) | (static_cast<uint32_t>(F2Bf(val[3])) << 16));
-                        *reinterpret_cast<uint2*>(a.out + o) = pk;
+                        *reinterpret_cast<uint2*>(a.out + o) = make_uint2(PackBf2(val[0], val[1]), PackBf2(val[2], val[3]));
                     }
                 }
             }

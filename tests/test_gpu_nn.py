@@ -234,6 +234,10 @@ def test_forward_conv_ragged_tiles(dims, cin, cout):
                       W, cin, 0, cout, 1, 1, stream())
     assert rc == 0
     assert rel(yb.float().cpu().numpy().reshape(ref.shape), no.leaky(ref)) < 2 ** -7
+    # the bf16 store rounds the same fp32 accumulators to nearest even, bit for bit as torch does
+    y32 = yd.cpu()
+    expect = t.where(y32 > 0, y32, y32 * np.float32(0.05)).to(t.bfloat16)
+    assert t.equal(yb.cpu(), expect)
 
 
 @pytest.mark.parametrize("cout,cin", [(16, 48), (32, 96), (64, 192), (32, 32), (128, 128), (16, 8)])
