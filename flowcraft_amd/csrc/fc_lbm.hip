@@ -65,9 +65,9 @@ struct fc_lbm {
     uint8_t* dsolid = nullptr;
     int* bidx = nullptr;
     void* recs = nullptr;
-    int* nearlist = nullptr;
+    long long* nearlist = nullptr;  // padded indices of the near-solid fluid cells (64-bit)
     int nrec = 0;
-    int* surf = nullptr;
+    long long* surf = nullptr;      // padded indices of the surface solid cells
     int nsurf = 0;
     void* side_mom = nullptr;
     void* sforce = nullptr;
@@ -113,6 +113,11 @@ const char* kNoHandle = "null fc_lbm handle";
 // cached stores lose (192x96x96 FP32 ping-pong, 280 MB: 58.8 -> 60.0 us; 256x128x128
 // FP16 ping-pong, 328 MB: 67.3 -> 71.1 us).  FC_LBM_RESIDENT=0/1 overrides the choice.
 constexpr long long kResidentBytes = 256LL << 20;
+// Largest lattice, in cells: every one-work-item-per-cell launch stays below 2^32
+// work-items (HIP's limit for one grid dimension), with room for a partial block.
+constexpr long long kMaxCells = (1LL << 32) - (1LL << 20);
+// Step launches are split by z range when one would exceed this many work-items.
+constexpr long long kMaxStepItems = 1LL << 31;
 constexpr size_t kPopMargin = 256;
 
 // Elements of one population copy: the front margin, nz+2 padded layers of ny+2 rows of
@@ -214,7 +219,8 @@ void FreeGeometry(fc_lbm* h) {
     DevFree(h->surf);
     DevFree(h->sforce);
     h->flags = h->dsolid = nullptr;
-    h->bidx = h->nearlist = h->surf = nullptr;
+    h->bidx = nullptr;
+    h->nearlist = h->surf = nullptr;
     h->recs = h->sforce = nullptr;
     h->nrec = h->nsurf = 0;
     h->has_geometry = false;
@@ -223,25 +229,29 @@ void FreeGeometry(fc_lbm* h) {
 
 // Deterministic compaction of the cells whose flag has `mask` (padded indices, in
 // global cell order).  Allocates *out (at least one element).
-int Compact(fc_lbm* h, uint8_t mask, int** out, int* count) {
+int Compact(fc_lbm* h, uint8_t mask, long long** out, int* count) {
     const unsigned chunks = Blocks(h->N, kChunk);
     DevTmp counts_buf{h->stream};
     FC_HIP(h, hipMalloc(&counts_buf.p, sizeof(int) * chunks));
     int* dcounts = static_cast<int*>(counts_buf.p);
     hipLaunchKernelGGL(KCountFlags, dim3(chunks), dim3(256), 0, h->stream, h->flags, mask, h->nx, h->ny, h->nz,
                        dcounts);
-    std::vector<int> counts(chunks), offs(chunks);
+    std::vector<int> counts(chunks);
+    std::vector<long long> offs(chunks);
     FC_HIP(h, hipMemcpyAsync(counts.data(), dcounts, sizeof(int) * chunks, hipMemcpyDeviceToHost, h->stream));
     FC_HIP(h, hipStreamSynchronize(h->stream));
     long long total = 0;
     for (unsigned i = 0; i < chunks; ++i) {
-        offs[i] = static_cast<int>(total);
+        offs[i] = total;
         total += counts[i];
     }
-    FC_HIP(h, hipMemcpyAsync(dcounts, offs.data(), sizeof(int) * chunks, hipMemcpyHostToDevice, h->stream));
-    FC_HIP(h, hipMalloc(out, sizeof(int) * std::max<long long>(total, 1)));
+    if (total >= (1LL << 31)) return Fail(h, FC_LBM_EINVAL, "more than 2^31 - 1 boundary cells");
+    DevTmp offs_buf{h->stream};
+    FC_HIP(h, hipMalloc(&offs_buf.p, sizeof(long long) * chunks));
+    FC_HIP(h, hipMemcpyAsync(offs_buf.p, offs.data(), sizeof(long long) * chunks, hipMemcpyHostToDevice, h->stream));
+    FC_HIP(h, hipMalloc(out, sizeof(long long) * std::max<long long>(total, 1)));
     hipLaunchKernelGGL(KScatterFlags, dim3(chunks), dim3(256), 0, h->stream, h->flags, mask, h->nx, h->ny, h->nz,
-                       dcounts, *out);
+                       static_cast<const long long*>(offs_buf.p), *out);
     FC_HIP(h, hipStreamSynchronize(h->stream));
     *count = static_cast<int>(total);
     return FC_LBM_OK;
@@ -375,7 +385,10 @@ int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out) {
     h->perz = desc->periodic_z ? 1 : 0;
     h->N = static_cast<long long>(h->nx) * h->ny * h->nz;
     h->P = static_cast<long long>(h->nx) * (h->ny + 2) * (h->nz + 2);
-    if (h->P >= (1LL << 31)) return Fail(h, FC_LBM_EINVAL, "lattice too large (padded cells must be < 2^31)");
+    // Kernels over every cell launch one work-item per cell in a one-dimensional grid, whose
+    // size is a 32-bit count of work-items; cell and padded-cell indices are 64-bit.
+    if (h->N > kMaxCells)
+        return Fail(h, FC_LBM_EINVAL, "lattice too large (at most " + std::to_string(kMaxCells) + " cells)");
     h->tbytes = h->storage == FC_LBM_STORE_F64 ? 8 : (h->storage == FC_LBM_STORE_F32 ? 4 : 2);
     // The step kernel addresses three padded z layers of all planes with 32-bit byte
     // offsets from one base.
@@ -688,12 +701,19 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     a.cs_d = h->params.smagorinsky_cs;
     const int cpt = h->cells_per_thread;
     const int tx = TileX(static_cast<int>(h->tbytes)), ty = TileY(static_cast<int>(h->tbytes));
-    const long long tiles = static_cast<long long>((h->nx + tx - 1) / tx) * ((h->ny + ty * cpt - 1) / (ty * cpt)) * zcount;
+    const long long layer_tiles = static_cast<long long>((h->nx + tx - 1) / tx) * ((h->ny + ty * cpt - 1) / (ty * cpt));
+    // Lattices past 2^31 work-items per launch step in z ranges (one step's ranges are
+    // independent: each cell's update reads only the previous state).
+    const int zmax = static_cast<int>(std::max<long long>(1, kMaxStepItems / (256 * layer_tiles)));
     hipError_t e = hipSuccess;
     DispatchStep(h, [&]<int Q, class S, class Lay>() {
         InletPops<Q, S>(h, a);
-        e = LaunchStep<Q, S, Lay>(a, static_cast<unsigned>(tiles), cpt, (h->pery | h->perz) != 0, h->resident,
-                                  h->stream);
+        for (int z = z0; z < z0 + zcount && e == hipSuccess; z += zmax) {
+            a.z0 = z;
+            a.zc = std::min(zmax, z0 + zcount - z);
+            e = LaunchStep<Q, S, Lay>(a, static_cast<unsigned>(layer_tiles * a.zc), cpt, (h->pery | h->perz) != 0,
+                                      h->resident, h->stream);
+        }
     });
     FC_HIP(h, e);
     return FC_LBM_OK;
@@ -1087,17 +1107,18 @@ int fc_lbm_boundary_links(fc_lbm* h, long long max, long long* cell, int* dir, i
                           long long* count) {
     if (!h || !count) return FC_LBM_EINVAL;
     if (!h->has_geometry) return Fail(h, FC_LBM_ESTATE, "no geometry: call fc_lbm_set_geometry first");
-    std::vector<int> list(std::max(h->nrec, 1));
+    std::vector<long long> list(std::max(h->nrec, 1));
     const size_t rb = h->rbytes == 8 ? sizeof(BRec<double>) : sizeof(BRec<float>);
     std::vector<unsigned char> recs(rb * std::max(h->nrec, 1));
-    FC_HIP(h, hipMemcpy(list.data(), h->nearlist, sizeof(int) * h->nrec, hipMemcpyDeviceToHost));
+    FC_HIP(h, hipMemcpy(list.data(), h->nearlist, sizeof(long long) * h->nrec, hipMemcpyDeviceToHost));
     FC_HIP(h, hipMemcpy(recs.data(), h->recs, rb * h->nrec, hipMemcpyDeviceToHost));
     long long k = 0;
     for (int r = 0; r < h->nrec; ++r) {
-        const int n = list[r];
-        const int x = n % h->nx;
-        const int y = (n / h->nx) % (h->ny + 2) - 1;
-        const int z = n / (h->nx * (h->ny + 2)) - 1;
+        const long long n = list[r];
+        const long long layer = static_cast<long long>(h->nx) * (h->ny + 2);
+        const int x = static_cast<int>(n % h->nx);
+        const int y = static_cast<int>((n / h->nx) % (h->ny + 2)) - 1;
+        const int z = static_cast<int>(n / layer) - 1;
         const long long c = (static_cast<long long>(z) * h->ny + y) * h->nx + x;
         const uint32_t* kinds = reinterpret_cast<const uint32_t*>(recs.data() + rb * r);
         for (int d = 1; d < h->Q; ++d) {

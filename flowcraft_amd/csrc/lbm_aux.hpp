@@ -18,6 +18,23 @@ __device__ __forceinline__ void Coords(long long c, int nx, int ny, int& x, int&
     x = static_cast<int>(r - static_cast<long long>(y) * nx);
 }
 
+// Decodes a padded cell index ((z+1)*(ny+2) + (y+1))*nx + x, the index of the per-cell
+// arrays (flags, boundary-record index) and of the cell lists.  64-bit: a lattice may hold
+// more than 2^31 padded cells.
+__device__ __forceinline__ void PaddedCoords(long long n, int nx, int ny, int& x, int& y, int& z) {
+    const long long layer = static_cast<long long>(nx) * (ny + 2);
+    const long long zp = n / layer;
+    const long long r = n - zp * layer;
+    const long long yp = r / nx;
+    x = static_cast<int>(r - yp * nx);
+    y = static_cast<int>(yp) - 1;
+    z = static_cast<int>(zp) - 1;
+}
+
+__device__ __forceinline__ long long PaddedIndex(int x, int y, int z, int nx, int ny) {
+    return (static_cast<long long>(z + 1) * (ny + 2) + (y + 1)) * nx + x;
+}
+
 // In-domain test with the periodic wrap (applied in place).
 __device__ __forceinline__ bool Resolve(int& x, int& y, int& z, int nx, int ny, int nz, int pery, int perz) {
     if (x < 0 || x >= nx) return false;
@@ -42,7 +59,7 @@ __global__ void KClassify(const uint8_t* __restrict__ solid, uint8_t* __restrict
     if (c >= N) return;
     int x, y, z;
     Coords(c, nx, ny, x, y, z);
-    const int n = ((z + 1) * (ny + 2) + (y + 1)) * nx + x;
+    const long long n = PaddedIndex(x, y, z, nx, ny);
     uint8_t fl = 0;
     if (solid[c]) {
         fl = kSolid;
@@ -72,10 +89,10 @@ __global__ void KClassify(const uint8_t* __restrict__ solid, uint8_t* __restrict
 constexpr int kChunkIters = 16;
 constexpr int kChunk = 256 * kChunkIters;
 
-__device__ __forceinline__ int PaddedOf(long long c, int nx, int ny) {
+__device__ __forceinline__ long long PaddedOf(long long c, int nx, int ny) {
     int x, y, z;
     Coords(c, nx, ny, x, y, z);
-    return ((z + 1) * (ny + 2) + (y + 1)) * nx + x;
+    return PaddedIndex(x, y, z, nx, ny);
 }
 
 __global__ void KCountFlags(const uint8_t* __restrict__ flags, uint8_t mask, int nx, int ny, int nz,
@@ -98,15 +115,15 @@ __global__ void KCountFlags(const uint8_t* __restrict__ flags, uint8_t mask, int
 }
 
 __global__ void KScatterFlags(const uint8_t* __restrict__ flags, uint8_t mask, int nx, int ny, int nz,
-                              const int* __restrict__ offsets, int* __restrict__ out) {
+                              const long long* __restrict__ offsets, long long* __restrict__ out) {
     const long long N = static_cast<long long>(nx) * ny * nz;
     const long long base = static_cast<long long>(blockIdx.x) * kChunk;
     __shared__ int wave_tot[4];
-    int running = offsets[blockIdx.x];
+    long long running = offsets[blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int it = 0; it < kChunkIters; ++it) {
         long long c = base + it * 256 + threadIdx.x;
-        int n = 0;
+        long long n = 0;
         bool p = false;
         if (c < N) {
             n = PaddedOf(c, nx, ny);
@@ -130,14 +147,13 @@ __global__ void KScatterFlags(const uint8_t* __restrict__ flags, uint8_t mask, i
 // q = clamp(phi_f / (phi_f - phi_s), 0, 1); q < 1/2 needs x + e_i in-domain fluid.
 template <int Q, class R>
 __global__ void KBuildRecords(const uint8_t* __restrict__ solid, const float* __restrict__ sdf,
-                              const int* __restrict__ list, int count, int* __restrict__ bidx,
+                              const long long* __restrict__ list, int count, int* __restrict__ bidx,
                               BRec<R>* __restrict__ recs, int nx, int ny, int nz, int pery, int perz) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
-    const int n = list[k];
-    const int x = n % nx;
-    const int yy = (n / nx) % (ny + 2) - 1;
-    const int zz = n / (nx * (ny + 2)) - 1;
+    const long long n = list[k];
+    int x, yy, zz;
+    PaddedCoords(n, nx, ny, x, yy, zz);
     const long long c = (static_cast<long long>(zz) * ny + yy) * nx + x;
     bidx[n] = k;
     BRec<R> r;
@@ -257,17 +273,15 @@ __global__ void KGetPops(const void* pops0, const void* pops1, long long P, long
 // from the current state after a seed or a restore.
 template <int Q, class S, class Lay>
 __global__ void KRefreshSide(const void* pops0, const void* pops1, long long P, long long t, int nx, int ny, int nz,
-                             int pery, int perz, const int* __restrict__ list, int count, BRec<typename S::R>* recs,
+                             int pery, int perz, const long long* __restrict__ list, int count, BRec<typename S::R>* recs,
                              typename S::R* side_mom) {
     using R = typename S::R;
     const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = 2 * ny * nz;
     if (k < count) {
-        const int n = list[k];
-        const int x = n % nx;
-        const int y = (n / nx) % (ny + 2) - 1;
-        const int z = n / (nx * (ny + 2)) - 1;
+        int x, y, z;
+        PaddedCoords(list[k], nx, ny, x, y, z);
         BRec<R>& r = recs[k];
         for (int d = 1; d < Q; ++d)
             if (KindOf(r.kind, d) == kBouzidiHi) r.side[d] = CurrentRaw<Q, S, Lay>(pops0, pops1, t, g, x, y, z, d);
@@ -299,21 +313,19 @@ __global__ void KRefreshSide(const void* pops0, const void* pops1, long long P, 
 template <int Q, class S, class Lay>
 __global__ void KSurfaceForce(const void* pops0, const void* pops1, long long P, long long t, int nx, int ny, int nz,
                               int pery, int perz, const uint8_t* __restrict__ flags, const int* __restrict__ bidx,
-                              const BRec<typename S::R>* __restrict__ recs, const int* __restrict__ surf, int nsurf,
+                              const BRec<typename S::R>* __restrict__ recs, const long long* __restrict__ surf, int nsurf,
                               typename S::R* __restrict__ out) {
     using R = typename S::R;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsurf) return;
     const Geo g{nx, ny, nz, pery, perz, nx, nx * (ny + 2), Q, PopRowStride(nx, Q)};
-    const int n = surf[k];
-    const int x = n % nx;
-    const int y = (n / nx) % (ny + 2) - 1;
-    const int z = n / (nx * (ny + 2)) - 1;
+    int x, y, z;
+    PaddedCoords(surf[k], nx, ny, x, y, z);
     R fx = 0, fy = 0, fz = 0, wsum = 0;
     for (int d = 1; d < Q; ++d) {
         int a = x + Lat<Q>::ex[d], b = y + Lat<Q>::ey[d], e = z + Lat<Q>::ez[d];
         if (!Resolve(a, b, e, nx, ny, nz, pery, perz)) continue;
-        const int fn = g.Idx(a, b, e);
+        const long long fn = g.Idx(a, b, e);
         if (flags[fn] & kSolid) continue;
         const int o = Opp(d);
         const R plain = CurrentPop<Q, S, Lay>(pops0, pops1, t, g, a, b, e, o);
@@ -406,14 +418,12 @@ __global__ void KFieldAt(const void* pops0, const void* pops1, long long t, int 
 }
 
 template <class R>
-__global__ void KScatterSurface(const int* __restrict__ surf, int nsurf, const R* __restrict__ sforce, int nx, int ny,
-                                int z0, int zc, float* __restrict__ out) {
+__global__ void KScatterSurface(const long long* __restrict__ surf, int nsurf, const R* __restrict__ sforce, int nx,
+                                int ny, int z0, int zc, float* __restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsurf) return;
-    const int n = surf[k];
-    const int x = n % nx;
-    const int y = (n / nx) % (ny + 2) - 1;
-    const int z = n / (nx * (ny + 2)) - 1;
+    int x, y, z;
+    PaddedCoords(surf[k], nx, ny, x, y, z);
     if (z < z0 || z >= z0 + zc) return;
     const long long j = (static_cast<long long>(z - z0) * ny + y) * nx + x;
     out[j * 4 + 0] = static_cast<float>(sforce[k * 4 + 0]);
@@ -501,7 +511,7 @@ __device__ __forceinline__ void BlockReduce256(double (&v)[V][256], const bool (
 // Surface pass: part[b*kAeroSVals + i] = 0-2 sum F, 3 sum |Fz|, 4-6 sum |Fz| pos,
 // 7 sum |F|, 8-10 sum |F| pos, 11 max positive stress (-1 if none), 12 non-finite seen.
 template <class R>
-__global__ __launch_bounds__(256) void KAeroSurface(const int* __restrict__ surf, int nsurf,
+__global__ __launch_bounds__(256) void KAeroSurface(const long long* __restrict__ surf, int nsurf,
                                                     const R* __restrict__ sforce, int nx, int ny, float cs, float ox,
                                                     float oy, float oz, double* __restrict__ part) {
     __shared__ double v[kAeroSVals][256];
@@ -510,10 +520,8 @@ __global__ __launch_bounds__(256) void KAeroSurface(const int* __restrict__ surf
     for (int i = 0; i < kAeroSVals; ++i) a[i] = 0.0;
     a[11] = -1.0;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < nsurf; k += kAeroBlocks * 256) {
-        const int n = surf[k];
-        const int x = n % nx;
-        const int y = (n / nx) % (ny + 2) - 1;
-        const int z = n / (nx * (ny + 2)) - 1;
+        int x, y, z;
+        PaddedCoords(surf[k], nx, ny, x, y, z);
         const float st = static_cast<float>(sforce[k * 4 + 0]);
         const float fx = static_cast<float>(sforce[k * 4 + 1]);
         const float fy = static_cast<float>(sforce[k * 4 + 2]);
@@ -748,12 +756,13 @@ __global__ void KProbe(const void* pops0, const void* pops1, long long P, long l
 }
 
 template <class R>
-__global__ void KProbeSurface(const int* __restrict__ surf, int nsurf, const R* __restrict__ sforce, int nx, int ny,
-                              int z, int* __restrict__ bad) {
+__global__ void KProbeSurface(const long long* __restrict__ surf, int nsurf, const R* __restrict__ sforce, int nx,
+                              int ny, int z, int* __restrict__ bad) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsurf) return;
-    const int n = surf[k];
-    if (n / (nx * (ny + 2)) - 1 != z) return;
+    int x, y, zc;
+    PaddedCoords(surf[k], nx, ny, x, y, zc);
+    if (zc != z) return;
     for (int i = 0; i < 4; ++i)
         if (!__builtin_isfinite(static_cast<float>(sforce[k * 4 + i]))) atomicOr(bad, 1);
 }

@@ -362,8 +362,11 @@ struct Geo {
     int sy, sz;  // padded strides of the per-cell arrays: sy = nx, sz = LS = nx*(ny+2)
     int Q;
     long long rs;  // population row stride, PopRowStride(nx, Q)
-    // index of a padded cell in per-cell arrays (flags, boundary-record index)
-    __device__ __forceinline__ int Idx(int x, int y, int z) const { return (z + 1) * sz + (y + 1) * sy + x; }
+    // index of a padded cell in per-cell arrays (flags, boundary-record index); 64-bit, a
+    // lattice may hold more than 2^31 padded cells
+    __device__ __forceinline__ long long Idx(int x, int y, int z) const {
+        return static_cast<long long>(z + 1) * sz + (y + 1) * sy + x;
+    }
     // element index of population plane `pl` of padded cell (x, y, z) (x may be -1 or nx)
     __device__ __forceinline__ long long Pop(int pl, int x, int y, int z) const {
         return PopFront(rs, Q) + (static_cast<long long>(z + 1) * (ny + 2) + (y + 1)) * rs +
@@ -382,7 +385,7 @@ struct Geo {
     // Neighbour index with the layout's wrap rules: x always wraps (only the
     // inlet/outlet columns ever reach across it, and they never read a slot);
     // y/z wrap when periodic, otherwise land in the ghost shell.
-    __device__ __forceinline__ int Nbr(int x, int y, int z, int dx, int dy, int dz) const {
+    __device__ __forceinline__ long long Nbr(int x, int y, int z, int dx, int dy, int dz) const {
         int ax = x + dx, ay = y + dy, az = z + dz;
         ax = ax >= nx ? ax - nx : (ax < 0 ? ax + nx : ax);
         if (pery) ay = ay >= ny ? ay - ny : (ay < 0 ? ay + ny : ay);
@@ -817,7 +820,11 @@ __device__ __forceinline__ void EncodeCell(const typename S::R (&fo)[Q], typenam
 template <class S>
 __device__ __forceinline__ uint32_t CellSrHash(const StepArgs& a, int x, int y, int z) {
     if constexpr (S::kStochastic)
-        return SrHash(static_cast<uint32_t>((z * a.ny + y) * a.nx + x), a.step);
+        // the global cell index (z*ny + y)*nx + x, exact below 2^32 cells (unsigned wrap)
+        return SrHash((static_cast<uint32_t>(z) * static_cast<uint32_t>(a.ny) + static_cast<uint32_t>(y)) *
+                              static_cast<uint32_t>(a.nx) +
+                          static_cast<uint32_t>(x),
+                      a.step);
     else
         return 0u;
 }

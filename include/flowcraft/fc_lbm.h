@@ -89,11 +89,15 @@ typedef struct fc_lbm_stats {
 } fc_lbm_stats;
 
 /* Replaces LbmSolver::Init's allocation half (FlowCraft/LbmSolver.cpp:205-366).
- * The device is the HIP current device at the call.  Lattice limit: fewer than 2^31
- * padded cells nx*(ny+2)*(nz+2) (about 1290^3), and three padded z layers of
- * populations under 2 GiB; larger lattices fail with FC_LBM_EINVAL before any device
- * call.  (FlowCraft's own ceiling is lower: its solid mask is one 4-byte-per-cell
- * buffer under 4 GiB, about 1.07e9 cells, FlowCraft/LbmSolver.h:22-58.) */
+ * The device is the HIP current device at the call.  Lattice limit: at most
+ * 2^32 - 2^20 cells nx*ny*nz (about 1620^3; one work-item per cell stays within one
+ * 32-bit grid dimension), and three padded z layers of populations under 2 GiB; larger
+ * lattices fail with FC_LBM_EINVAL before any device call.  Memory bounds it first for
+ * most formats: per cell, 2*Q*s population bytes ping-pong (Q*s in place) plus 6 bytes
+ * (solid mask, flags, boundary-record index), so one 288 GB MI355X holds about 1.8e9 cells of
+ * FP32 D3Q19 ping-pong, 3.5e9 in place or in FP16 ping-pong, and the full 4.29e9 in
+ * FP16 in place.  (FlowCraft's own ceiling is lower: its solid mask is one
+ * 4-byte-per-cell buffer under 4 GiB, about 1.07e9 cells, FlowCraft/LbmSolver.h:22-58.) */
 int fc_lbm_create(const fc_lbm_desc* desc, fc_lbm** out);
 
 /* Replaces LbmSolver::Shutdown (FlowCraft/LbmSolver.cpp:368-387). */
@@ -172,7 +176,7 @@ int fc_lbm_readback_field_range_noalloc(fc_lbm* h, int z0, int zcount, float* ou
 int fc_lbm_field_device(fc_lbm* h, float* out_device);
 
 /* Extension: the field at n listed cells (device array of global cell indices
- * (z*ny + y)*nx + x) into a device buffer of 4*n floats: fluid cells as
+ * (z*ny + y)*nx + x, 32-bit: cells below 2^31) into a device buffer of 4*n floats: fluid cells as
  * fc_lbm_field_device gives them, solid cells and indices outside [0, N) zero.  Enqueued
  * on the handle's stream (asynchronous); for consumers that sample a few cells (smoke
  * trails). */
@@ -292,7 +296,8 @@ const char* fc_voxelize_last_error(void);
 typedef struct fc_smoke fc_smoke;
 
 /* A particle system over a grid: dimensions, cell size, origin and solid mask (host,
- * nx*ny*nz bytes, copied).  On a device failure *out still receives the handle, for
+ * nx*ny*nz bytes, copied; grids below 2^31 cells, FC_LBM_EINVAL otherwise).  On a
+ * device failure *out still receives the handle, for
  * fc_smoke_last_error; destroy it either way. */
 int fc_smoke_create(int nx, int ny, int nz, float cell_size, const float origin[3], const uint8_t* solid,
                     fc_smoke** out);

@@ -69,15 +69,18 @@ def _create(nx, ny, nz, storage=_lib.STORE_F16, layout=_lib.LAYOUT_INPLACE):
 
 
 def test_lattice_size_limit_is_checked_before_the_device():
-    """The engine's lattice limit: fewer than 2^31 padded cells nx*(ny+2)*(nz+2), and three
+    """The engine's lattice limit: at most 2^32 - 2^20 cells nx*ny*nz (one work-item per
+    cell in one 32-bit grid dimension; cell and padded-cell indices are 64-bit), and three
     padded z layers of populations under 2 GiB (32-bit offsets from a per-workgroup base).
-    2048 x 1026 x 1022 = 2147475456 padded cells is the last nz that fits at this
-    cross-section; one more layer is refused with EINVAL before any device call."""
-    rc, msg = _create(2048, 1024, 1021)
-    assert rc == _lib.FC_LBM_EINVAL and "padded cells must be < 2^31" in msg
+    At this cross-section nz = 2047 (2^32 - 2^21 cells, well past 2^31 padded cells) is the
+    last that fits; one more layer is refused with EINVAL before any device call."""
+    assert 2048 * 1024 * 2047 <= (1 << 32) - (1 << 20) < 2048 * 1024 * 2048
+    assert 2048 * (1024 + 2) * (2047 + 2) > (1 << 31)
+    rc, msg = _create(2048, 1024, 2048)
+    assert rc == _lib.FC_LBM_EINVAL and "lattice too large (at most 4293918720 cells)" in msg
     rc, msg = _create(65536, 8192, 1)
     assert rc == _lib.FC_LBM_EINVAL and "cross-section too large" in msg
     if torch.cuda.device_count() == 0:
         # the size checks pass; without a device the next step (the device query) fails
-        rc, msg = _create(2048, 1024, 1020)
+        rc, msg = _create(2048, 1024, 2047)
         assert rc == _lib.FC_LBM_EDEVICE, (rc, msg)
