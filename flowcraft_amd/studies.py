@@ -2,6 +2,8 @@
 
     python -m flowcraft_amd.studies precision [--steps 5000] [--every 50] [--window 1000]
     python -m flowcraft_amd.studies large [--steps 200] [--warmup 50]
+    python -m flowcraft_amd.studies capacity [--steps 10] [--warmup 3]
+    python -m flowcraft_amd.studies depth [--nz 64 256 1024 2048] [--steps 10]
     python -m flowcraft_amd.studies drain [--steps 20000] [--every 1000]
     python -m flowcraft_amd.studies sensitivity [--t0 1000] [--steps 4000] [--eps 1e-9] [--tau 0.5005]
 
@@ -27,6 +29,19 @@ large — the 768x512x576 bandwidth lattice (226M cells): an ellipsoid of semi-a
     steps, then `steps` steps timed with HIP events on the solver's stream.  FP32 in
     place (one 17 GB population copy) and FP16 offsets.
 
+capacity — the largest lattices one GPU holds, in place: 2048 x 1024 x nz with nz = 2047
+    in stochastically rounded FP16 (4.29e9 cells, the engine's cell limit, 189 GB) and
+    nz = 1400 in FP32 (2.94e9 cells, 241 GB); a circular cylinder of radius 48 about
+    (500, 512) extruded over z, plain bounce-back (no signed distance: its 4 bytes per cell
+    would not fit beside the FP32 populations), D3Q19 Cs = 0.1, u = 0.05, tau = 0.6,
+    free-slip y/z; the geometry is built on the device, `warmup` untimed steps, then
+    `steps` steps timed with HIP events on the solver's stream.
+
+depth — the step's rate against the lattice depth: 512 x 256 x nz around a circular
+    cylinder of radius 25.6 about (128, 128) extruded over z (plain bounce-back), D3Q19 FP32
+    ping-pong, Cs = 0.1, u = 0.05, tau = 0.6; free-slip y/z walls, and the same periodic in
+    y and z; 3 untimed steps, then `steps` steps timed with HIP events.
+
 sensitivity — whether the developed wing flow amplifies a small local perturbation (and
     FP32 rounding) in FP64: sensitivity_study.
 
@@ -45,7 +60,7 @@ import time
 import numpy as np
 
 from . import geometry
-from .lbm import LatticeModel, Layout, LbmSolver, Params, Storage
+from .lbm import LatticeModel, Layout, LbmSolver, Params, Storage, VoxelGrid
 
 HBM_PEAK_GBS = 8000.0
 
@@ -199,6 +214,95 @@ def large_lattice(steps=200, warmup=50, storages=("f32", "f16"), layout=Layout.I
     return out
 
 
+def capacity(runs=(("f16sr", 2047), ("f32", 1400)), nx=2048, ny=1024, steps=10, warmup=3, log=print):
+    import torch
+    st_map = {"f32": (Storage.F32, 4), "f16": (Storage.F16, 2), "f16sr": (Storage.F16_SR, 2),
+              "bf16sr": (Storage.BF16_SR, 2)}
+    params = Params(tau=0.6, inletVelocity=0.05, smagorinskyCs=0.1, freeSlipY=True, freeSlipZ=True)
+    out = {"workload": f"{nx}x{ny}xnz circular cylinder r 48 about (500, 512) extruded over z, plain bounce-back, "
+                       "D3Q19 Cs 0.1, u 0.05, tau 0.6, free-slip y/z, in place", "steps": steps, "warmup": warmup,
+           "runs": {}}
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    layer = torch.from_numpy((np.hypot(xx + 0.5 - 500.0, yy + 0.5 - 512.0) < 48.0).astype(np.uint8))
+    for sname, nz in runs:
+        st, b = st_map[sname]
+        cells = nx * ny * nz
+        t0 = time.time()
+        s = LbmSolver()
+        ok = s.Init(None, VoxelGrid(nx, ny, nz, np.zeros(cells, np.uint8), None), LatticeModel.D3Q19, storage=st,
+                    layout=Layout.INPLACE)
+        if not ok:
+            raise RuntimeError(s.LastError())
+        solid = layer.cuda().reshape(1, ny, nx).expand(nz, ny, nx).contiguous()
+        s.SetGeometryDevice(solid.reshape(-1))
+        del solid
+        torch.cuda.empty_cache()
+        s.Reset(params, True)
+        stream = torch.cuda.ExternalStream(s.Stream())
+        s.Step(None, warmup)
+        s.Sync()
+        setup = time.time() - t0
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        s.Step(None, steps)
+        e1.record(stream)
+        s.Sync()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        mlups = cells * steps / (ms * 1e-3) / 1e6
+        gbs = mlups * 1e6 * 2 * 19 * b / 1e9
+        st_ = s.ReadbackStats()
+        r = {"nz": nz, "cells": cells, "population_bytes": s.PopulationBytes(),
+             "device_bytes_in_use": int(torch.cuda.mem_get_info()[1] - torch.cuda.mem_get_info()[0]),
+             "mlups": round(mlups, 1), "ms_per_step": round(ms / steps, 3), "achieved_GBs": round(gbs, 1),
+             "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "setup_s": round(setup, 1), "finite": not st_.hasNaN}
+        s.Shutdown()
+        torch.cuda.empty_cache()
+        out["runs"][sname] = r
+        log(f"  {sname}: {r}")
+    return out
+
+
+def depth_sweep(nzs=(64, 256, 1024, 2048), nx=512, ny=256, steps=10, warmup=3, log=print):
+    import torch
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    layer = torch.from_numpy((np.hypot(xx + 0.5 - nx / 4, yy + 0.5 - ny / 2) < ny / 10).astype(np.uint8))
+    out = {"workload": f"{nx}x{ny}xnz circular cylinder r {ny / 10} about ({nx / 4}, {ny / 2}) extruded over z, plain "
+                       "bounce-back, D3Q19 FP32 ping-pong, Cs 0.1, u 0.05, tau 0.6", "steps": steps, "warmup": warmup,
+           "runs": []}
+    for periodic in (False, True):
+        for nz in nzs:
+            cells = nx * ny * nz
+            s = LbmSolver()
+            ok = s.Init(None, VoxelGrid(nx, ny, nz, np.zeros(cells, np.uint8), None), LatticeModel.D3Q19,
+                        storage=Storage.F32, layout=Layout.PINGPONG, periodicY=periodic, periodicZ=periodic)
+            if not ok:
+                raise RuntimeError(s.LastError())
+            solid = layer.cuda().reshape(1, ny, nx).expand(nz, ny, nx).contiguous()
+            s.SetGeometryDevice(solid.reshape(-1))
+            del solid
+            torch.cuda.empty_cache()
+            s.Reset(Params(tau=0.6, inletVelocity=0.05, smagorinskyCs=0.1, freeSlipY=True, freeSlipZ=True), True)
+            stream = torch.cuda.ExternalStream(s.Stream())
+            s.Step(None, warmup)
+            s.Sync()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            s.Step(None, steps)
+            e1.record(stream)
+            s.Sync()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            r = {"nz": nz, "walls": "periodic y/z" if periodic else "free-slip y/z", "ms_per_step": round(ms, 4),
+                 "glups": round(cells / ms / 1e6, 2), "TBs": round(cells * 152 / ms / 1e9, 3),
+                 "us_per_layer": round(ms * 1000 / nz, 3)}
+            s.Shutdown()
+            torch.cuda.empty_cache()
+            out["runs"].append(r)
+            log(f"  {r}")
+    return out
+
+
 def density_drain(steps=20000, every=1000, model="D3Q15", tau=TAU_D3Q15, storages=("f32", "f16"), log=print):
     """The 512x256x256 wing from rest (u 0.08, Cs 0.1), mean density, mean speed and Cd
     every `every` steps per storage: how the channel's excess mass drains through the
@@ -305,6 +409,14 @@ def main(argv=None):
     sv.add_argument("--tau", type=float, default=0.5005)
     sv.add_argument("--dims", type=int, nargs=3, default=[512, 256, 256])
     sv.add_argument("--out", default=None)
+    c = sub.add_parser("capacity")
+    c.add_argument("--steps", type=int, default=10)
+    c.add_argument("--warmup", type=int, default=3)
+    c.add_argument("--out", default=None)
+    dp = sub.add_parser("depth")
+    dp.add_argument("--nz", type=int, nargs="+", default=[64, 256, 1024, 2048])
+    dp.add_argument("--steps", type=int, default=10)
+    dp.add_argument("--out", default=None)
     d = sub.add_parser("drain")
     d.add_argument("--steps", type=int, default=20000)
     d.add_argument("--every", type=int, default=1000)
@@ -322,6 +434,10 @@ def main(argv=None):
                             tau15=a.tau15, dims=tuple(a.dims))
     elif a.cmd == "sensitivity":
         r = sensitivity_study(a.t0, a.steps, a.every, a.eps, a.tau, tuple(a.dims), log=log)
+    elif a.cmd == "depth":
+        r = depth_sweep(tuple(a.nz), steps=a.steps, log=log)
+    elif a.cmd == "capacity":
+        r = capacity(steps=a.steps, warmup=a.warmup, log=log)
     elif a.cmd == "drain":
         r = density_drain(a.steps, a.every, storages=tuple(a.storages), log=log)
     else:
