@@ -31,6 +31,11 @@ grids are identical.
 solver's state against FlowCraft's path (field readback plus its own Step on the host)
 and checks that the trails agree.
 
+`largest_lattice` times the step on the largest FP32 lattice one GPU holds in place:
+2048x1024x1400 (2.94e9 cells, 224 GB of populations) around an extruded cylinder, the
+geometry built on the device (flowcraft_amd.studies.capacity); it is not the metric's
+workload and does not enter `value`.
+
 `pacing` times an interactive frame (the field for display plus 60 steps) in FlowCraft's
 blocking order against the asynchronous snapshot that overlaps the copy with the steps.
 
@@ -428,6 +433,21 @@ def voxelize_block(cpu=True, reps=3, detail=2, res=640):
     return out
 
 
+def largest_lattice_block(torch, nz=1400):
+    """FP32 in place on 2048x1024xnz: 3 untimed steps, 10 timed (HIP events).  A lattice that
+    does not fit (another tenant on the device) is reported as an error, not a failure."""
+    from flowcraft_amd import studies
+    torch.cuda.empty_cache()
+    try:
+        r = studies.capacity(runs=(("f32", nz),), steps=10, warmup=3, log=lambda m: None)
+    except Exception as e:  # noqa: BLE001 - allocation failures are reported in the line
+        torch.cuda.empty_cache()
+        return {"error": str(e)[:200]}
+    out = dict(r["runs"]["f32"])
+    out["workload"] = r["workload"]
+    return out
+
+
 def warm_start_block(torch, n_shapes, settled=False):
     """Live warm-vs-plain comparison on held-out shapes of the warm-start distribution
     (256x128x128 D3Q19 Smagorinsky Cs = 0.1, u = 0.06, tau = 0.503, FP16 storage with
@@ -628,6 +648,7 @@ def main():
     vox = None if args.no_variants else timed("voxelize", lambda: voxelize_block(cpu=not args.no_cpu))
     smoke = None if args.no_variants else timed("smoke_trails", lambda: smoke_block(torch, grid, cpu=not args.no_cpu))
     pacing = None if args.no_variants else timed("pacing", lambda: pacing_block(torch, grid))
+    largest = None if args.no_variants else timed("largest_lattice", lambda: largest_lattice_block(torch))
     fp64_err = None if args.no_variants else timed("error_vs_fp64", lambda: fp64_error_block(torch, grid))
     # the precision target on a flow where FP32 tracks FP64: the same wing at tau 0.6, where FP32
     # stays within 0.03% of FP64, over 20000 steps (trailing 10000-step means)
@@ -646,7 +667,7 @@ def main():
                                f"free-slip y/z, {layout.name.lower()} layout",
                    "lattice": [NX, NY, NZ], "model": "D3Q19", "storage": "f32", "layout": layout.name.lower(),
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "error_vs_fp64": fp64_err, "error_vs_fp64_tau06": fp64_tau06, "warm_start": warm, "warm_start_settled": warm_settled,
+        "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "aero_forces": aero, "voxelize": vox, "smoke_trails": smoke, "pacing": pacing, "largest_lattice": largest, "error_vs_fp64": fp64_err, "error_vs_fp64_tau06": fp64_tau06, "warm_start": warm, "warm_start_settled": warm_settled,
         "finite": main_r["finite"],
     }
     print(json.dumps(line), flush=True)
