@@ -1462,6 +1462,30 @@ void LaunchConvTile(const ConvArgs& a, dim3 grid, hipStream_t s) {
         hipLaunchKernelGGL((KConvTile<MODE, TN, 4>), grid, dim3(256), 0, s, a);
 }
 
+// 8 x 8 transpose of 16-bit values: in[j] holds elements (j, 0..7) two to a word, low half
+// first; out[c] receives elements (0..7, c) in the same packing.
+__device__ __forceinline__ void Transpose8x8(const uint4* in, uint4 (&out)[8]) {
+    uint32_t w[8][4];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        w[j][0] = in[j].x;
+        w[j][1] = in[j].y;
+        w[j][2] = in[j].z;
+        w[j][3] = in[j].w;
+    }
+    uint32_t o[8][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t lo = w[2 * k][m], hi = w[2 * k + 1][m];
+            o[2 * m][k] = (lo & 0xffffu) | (hi << 16);
+            o[2 * m + 1][k] = (lo >> 16) | (hi & 0xffff0000u);
+        }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) out[c] = make_uint4(o[c][0], o[c][1], o[c][2], o[c][3]);
+}
+
 // Weight gradient of a stride-1 convolution (forward or upsample-concat), LDS-tiled:
 //   dW[co][tap][ci] = sum_v dZ[v][co] * X[v + off(tap)][ci]
 // as MFMAs with M = 16 output channels, N = 16 input channels, K = 32 voxels.  A workgroup
@@ -1507,35 +1531,51 @@ __global__ __launch_bounds__(256) void KWgradTile(const WgradArgs w) {
         b /= tiles_y;
         const int z0 = static_cast<int>(b % tiles_z) * kTZ;
         const int n = static_cast<int>(b / tiles_z);
-        // input halo, transposed, three x-shifted copies
-        for (int e = threadIdx.x; e < kHV * runs; e += 256) {
-            const int hv = e / runs, r8 = e - hv * runs;
-            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-            const uint4 raw = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + hx, cb + r8 * 8);
-            u16 v8[8];
-            __builtin_memcpy(v8, &raw, 16);
-            const int off = (hz * kHY + hy) * kTX;
+        // Staging, 16-byte LDS rows: a halo item (hz, hy, half xh of the 16 x positions, an
+        // 8-channel run) loads the 10 voxels x0-1+8xh .. +9, which feed 8 consecutive positions
+        // of all three x-shifted copies; a dZ item loads 8 consecutive voxels of one 8-channel
+        // run.  Each 8 x 8 block (voxels x channels) is transposed in registers and stored as
+        // 8 rows of one channel.  Halo and dZ items share one index space.
+        const int halo_items = kHZ * kHY * 2 * runs;
+        const int z_items = (kTX * kTY * kTZ / 8) * TI * 2;
+        for (int e = threadIdx.x; e < halo_items + z_items; e += 256) {
+            if (e < halo_items) {
+                const int r8 = e % runs;
+                int it = e / runs;
+                const int xh = it & 1;
+                it >>= 1;
+                const int hy = it % kHY, hz = it / kHY;
+                uint4 raw[10];
 #pragma unroll
-            for (int kx = 0; kx < 3; ++kx) {
-                const int xx = hx - kx;
-                if (xx < 0 || xx >= kTX) continue;
+                for (int j = 0; j < 10; ++j)
+                    raw[j] = InputRun<MODE>(a, n, z0 - 1 + hz, y0 - 1 + hy, x0 - 1 + 8 * xh + j, cb + r8 * 8);
+                u16* dst = sX + r8 * 8 * CIS + (hz * kHY + hy) * kTX + 8 * xh;
 #pragma unroll
-                for (int c = 0; c < 8; ++c) sX[(kx * 16 + r8 * 8 + c) * CIS + off + xx] = v8[c];
+                for (int kx = 0; kx < 3; ++kx) {
+                    uint4 t[8];
+                    Transpose8x8(raw + kx, t);
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) *reinterpret_cast<uint4*>(dst + (kx * 16 + c) * CIS) = t[c];
+                }
+            } else {
+                const int ez = e - halo_items;
+                const int g = ez / (TI * 2), q = ez - g * (TI * 2);
+                const int v0 = 8 * g;
+                const int xb = x0 + (v0 & (kTX - 1)), y = y0 + (v0 / kTX) % kTY, z = z0 + v0 / (kTX * kTY);
+                const bool row_in = y < a.hd && z < a.dd && co0 + q * 8 < w.cout;
+                const long long vrow = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd;
+                uint4 raw[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    raw[j] = make_uint4(0u, 0u, 0u, 0u);
+                    if (row_in && xb + j < a.wd)
+                        raw[j] = *reinterpret_cast<const uint4*>(w.dz + (vrow + xb + j) * w.cout + co0 + q * 8);
+                }
+                uint4 t[8];
+                Transpose8x8(raw, t);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) *reinterpret_cast<uint4*>(&sZ[(q * 8 + c) * ZS + v0]) = t[c];
             }
-        }
-        // dZ tile, transposed (zeros outside the volume)
-        for (int e = threadIdx.x; e < kTX * kTY * kTZ * TI * 2; e += 256) {
-            const int v = e / (TI * 2), q = e - v * (TI * 2);
-            const int x = x0 + (v & (kTX - 1)), y = y0 + (v / kTX) % kTY, z = z0 + v / (kTX * kTY);
-            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-            if (x < a.wd && y < a.hd && z < a.dd && co0 + q * 8 < w.cout) {
-                const long long vg = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
-                raw = *reinterpret_cast<const uint4*>(w.dz + vg * w.cout + co0 + q * 8);
-            }
-            u16 v8[8];
-            __builtin_memcpy(v8, &raw, 16);
-#pragma unroll
-            for (int c = 0; c < 8; ++c) sZ[(q * 8 + c) * ZS + v] = v8[c];
         }
         __syncthreads();
         // K block kb: voxels 32kb..32kb+31 (tile rows 2kb, 2kb+1 of 16 x-voxels); lane
