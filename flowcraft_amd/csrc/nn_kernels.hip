@@ -221,19 +221,22 @@ __global__ __launch_bounds__(256) void KConv(const ConvArgs a) {
 // Adjoint (data gradient) of a stride-2 convolution, split by output parity.  Along one
 // axis an even output coordinate c receives only tap 1 (from input c/2) and an odd one taps
 // 0 and 2 (from (c+1)/2 and (c-1)/2), so a voxel has 1 to 8 valid taps of 27 (3.4 on
-// average); KConv's gather visits all 27 and zeroes the rest.  blockIdx.y is the parity
-// class (pz, py, px); the class's voxels are enumerated at half resolution, so the rows of
-// every wave share a class and its K loop visits only the class's valid taps, 32 gathered
-// channels per step (kin % 32 == 0).  blockIdx.z selects 16*TN output channels.
+// average); KConv's gather visits all 27 and zeroes the rest.  Wave w of a workgroup takes
+// parity class w = (pz, py, px); a class's voxels are enumerated at half resolution, so each
+// wave's K loop visits only its class's valid taps, 32 gathered channels per step (kin % 32
+// == 0).  The eight waves cover the same 16*TM half-resolution positions, so on even extents
+// a workgroup writes whole 2 x 2 x 2 blocks of output voxels and no output line is left
+// partly written for another workgroup.  The weights are the MFMA's first operand: a lane
+// holds four consecutive output channels of one voxel and stores them at once (StoreC4).
+// blockIdx.z selects 16*TN output channels.
 template <int TM, int TN>
-__global__ __launch_bounds__(256) void KConvAdj2(const ConvArgs a) {
+__global__ __launch_bounds__(512) void KConvAdj2(const ConvArgs a) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int cls = blockIdx.y;
+    const int cls = threadIdx.x >> 6;
     const int px = cls & 1, py = (cls >> 1) & 1, pz = cls >> 2;
     const int wx = (a.wd - px + 1) / 2, wy = (a.hd - py + 1) / 2, wz = (a.dd - pz + 1) / 2;
     const long long Vc = static_cast<long long>(a.nb) * wz * wy * wx;
-    const long long vbase = (static_cast<long long>(blockIdx.x) * 4 + wave) * (16 * TM);
+    const long long vbase = static_cast<long long>(blockIdx.x) * (16 * TM);
     if (vbase >= Vc) return;  // wave-uniform: classes with odd extents hold fewer voxels
     const int nbase = blockIdx.z * (16 * TN);
     const int K = 27 * a.kin;
@@ -247,13 +250,19 @@ __global__ __launch_bounds__(256) void KConvAdj2(const ConvArgs a) {
         z = 2 * static_cast<int>(r % wz) + pz;
         n = static_cast<int>(r / wz);
     };
+    // An output voxel (z, y, x) = (2zc + pz, 2yc + py, 2xc + px) reads, through tap k along an
+    // axis, the input at (c + 1 - k) / 2: c/2 = zc for an even c (k = 1), and zc + 1 (k = 0)
+    // or zc (k = 2) for an odd one.  So a tap's input voxel is (zc + dz, yc + dy, xc + dx)
+    // with d in {0, 1}: the lane's coarse index plus an offset the whole wave shares.
     int vn[TM], vz[TM], vy[TM], vx[TM];
+    long long vin[TM];
     bool vok[TM];
 #pragma unroll
     for (int m = 0; m < TM; ++m) {
         const long long v = vbase + m * 16 + row;
         vok[m] = v < Vc;
         coords(vok[m] ? v : 0, vn[m], vz[m], vy[m], vx[m]);
+        vin[m] = ((static_cast<long long>(vn[m]) * a.di + (vz[m] >> 1)) * a.hi + (vy[m] >> 1)) * a.wi + (vx[m] >> 1);
     }
     f32x4 acc[TM][TN];
 #pragma unroll
@@ -266,46 +275,42 @@ __global__ __launch_bounds__(256) void KConvAdj2(const ConvArgs a) {
         const int jx = ti % nkx, jy = (ti / nkx) % nky, jz = ti / (nkx * nky);
         const int kx = px ? 2 * jx : 1, ky = py ? 2 * jy : 1, kz = pz ? 2 * jz : 1;
         const int t = (kz * 3 + ky) * 3 + kx;
+        const int dx = kx == 0, dy = ky == 0, dz = kz == 0;
+        const long long toff = (static_cast<long long>(dz) * a.hi + dy) * a.wi + dx;
+        bool tok[TM];
+#pragma unroll
+        for (int m = 0; m < TM; ++m)
+            tok[m] = vok[m] && (vz[m] >> 1) + dz < a.di && (vy[m] >> 1) + dy < a.hi && (vx[m] >> 1) + dx < a.wi;
         for (int kc = 0; kc < kblocks; ++kc) {
             const int k0 = t * a.kin + kc * 32 + kq;
-            bf16x8 bfr[TN];
+            bf16x8 wfr[TN];
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
                 uint4 raw = make_uint4(0u, 0u, 0u, 0u);
                 const int n = nbase + j * 16 + row;
                 if (n < a.nout) raw = *reinterpret_cast<const uint4*>(a.w + static_cast<long long>(n) * K + k0);
-                __builtin_memcpy(&bfr[j], &raw, 16);
+                __builtin_memcpy(&wfr[j], &raw, 16);
             }
 #pragma unroll
             for (int m = 0; m < TM; ++m) {
-                const bf16x8 afr = GatherA<kAdj>(a, vn[m], vz[m], vy[m], vx[m], k0, vok[m]);
+                uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+                if (tok[m]) raw = *reinterpret_cast<const uint4*>(a.in + (vin[m] + toff) * a.kin + kc * 32 + kq);
+                bf16x8 gfr;
+                __builtin_memcpy(&gfr, &raw, 16);
 #pragma unroll
                 for (int j = 0; j < TN; ++j)
-                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[m][j], 0, 0, 0);
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfr[j], gfr, acc[m][j], 0, 0, 0);
             }
         }
     }
-    // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+    // epilogue: lane holds output channels 4*(lane/16)+r of class voxel vbase + 16m + lane%16
 #pragma unroll
-    for (int m = 0; m < TM; ++m)
+    for (int m = 0; m < TM; ++m) {
+        if (!vok[m]) continue;
+        const long long vo = ((static_cast<long long>(vn[m]) * a.dd + vz[m]) * a.hd + vy[m]) * a.wd + vx[m];
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int nn = nbase + j * 16 + row;
-            if (nn >= a.nout) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const long long v = vbase + m * 16 + (lane >> 4) * 4 + r;
-                if (v >= Vc) continue;
-                int n, z, y, x;
-                coords(v, n, z, y, x);
-                const long long vo = ((static_cast<long long>(n) * a.dd + z) * a.hd + y) * a.wd + x;
-                const float val = acc[m][j][r];
-                if (a.outf)
-                    a.outf[vo * a.nout + nn] = val;
-                else
-                    a.out[vo * a.nout + nn] = F2Bf(val);
-            }
-        }
+        for (int j = 0; j < TN; ++j) StoreC4(a, vo, nbase + j * 16 + 4 * (lane >> 4), acc[m][j]);
+    }
 }
 
 // Weight gradient of a kFwd/kUpcat conv: dW[co][k] = sum_v dZ[v][co] * A[v][k], split over
@@ -1692,11 +1697,11 @@ void LaunchConv(const ConvArgs& a, hipStream_t s) {
         // class 0 (all coordinates even) is the largest parity class
         const long long V0 = static_cast<long long>(a.nb) * ((a.dd + 1) / 2) * ((a.hd + 1) / 2) * ((a.wd + 1) / 2);
         if (a.nout <= 16)
-            hipLaunchKernelGGL((KConvAdj2<4, 1>), dim3(Blk(V0, 256), 8, 1), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((KConvAdj2<4, 1>), dim3(Blk(V0, 64), 1, 1), dim3(512), 0, s, a);
         else if (a.nout <= 32)
-            hipLaunchKernelGGL((KConvAdj2<2, 2>), dim3(Blk(V0, 128), 8, 1), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((KConvAdj2<2, 2>), dim3(Blk(V0, 32), 1, 1), dim3(512), 0, s, a);
         else
-            hipLaunchKernelGGL((KConvAdj2<2, 4>), dim3(Blk(V0, 128), 8, Blk(a.nout, 64)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((KConvAdj2<2, 4>), dim3(Blk(V0, 32), 1, Blk(a.nout, 64)), dim3(512), 0, s, a);
         return;
     }
     if (a.nout <= 16) {
