@@ -423,7 +423,7 @@ __global__ __launch_bounds__(256) void KReduceSplits(const float* __restrict__ p
 
 // Per-channel sums over voxels of a [V][C] bf16 tensor: part[block][c] (fixed grid).  A
 // thread reads 8 consecutive channels of a voxel with one 16-byte load.
-constexpr int kColBlocks = 256;
+constexpr int kColBlocks = 1024;  // partial sums per channel (fc_nn_col_blocks)
 __global__ __launch_bounds__(256) void KColSumPartial(const u16* __restrict__ x, long long V, int C,
                                                       float* __restrict__ part) {
     const int octs = C / 8;                  // 16-byte groups per voxel
@@ -437,6 +437,58 @@ __global__ __launch_bounds__(256) void KColSumPartial(const u16* __restrict__ x,
         for (int q = 0; q < 4; ++q) {
             s[2 * q] += __uint_as_float(w4[q] << 16);
             s[2 * q + 1] += __uint_as_float(w4[q] & 0xFFFF0000u);
+        }
+    }
+    __shared__ float red[256 * 8];
+    for (int q = 0; q < 8; ++q) red[threadIdx.x * 8 + q] = s[q];
+    __syncthreads();
+    if (threadIdx.x < C) {
+        const int oc = threadIdx.x / 8, q = threadIdx.x % 8;
+        float t = 0.f;
+        for (int k = 0; k < lanes; ++k) t += red[(k * octs + oc) * 8 + q];
+        part[blockIdx.x * C + threadIdx.x] = t;
+    }
+}
+
+// dZ = dY * leaky'(Y) fused with the per-channel sums of dZ (the bias gradient): the thread
+// mapping, grid and summation order of KColSumPartial, which reads back the stored bf16 dZ,
+// so dZ and the partial sums are bit-identical to the two kernels run apart.  Each thread
+// keeps the loads of kLcUnroll voxels in flight and accumulates them in voxel order.
+constexpr int kLcUnroll = 8;
+__global__ __launch_bounds__(256) void KLeakyColSum(const u16* __restrict__ y, const u16* __restrict__ dy,
+                                                    u16* __restrict__ dz, long long V, int C,
+                                                    float* __restrict__ part) {
+    const int octs = C / 8;
+    const int lanes = 256 / octs;
+    const int o = threadIdx.x % octs, sub = threadIdx.x / octs;
+    const long long stride = static_cast<long long>(kColBlocks) * lanes;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (long long v0 = static_cast<long long>(blockIdx.x) * lanes + sub; v0 < V; v0 += kLcUnroll * stride) {
+        uint4 ry[kLcUnroll], rg[kLcUnroll];
+#pragma unroll
+        for (int u = 0; u < kLcUnroll; ++u) {
+            const long long v = v0 + u * stride;
+            if (v < V) {
+                ry[u] = *reinterpret_cast<const uint4*>(y + v * C + o * 8);
+                rg[u] = *reinterpret_cast<const uint4*>(dy + v * C + o * 8);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLcUnroll; ++u) {
+            const long long v = v0 + u * stride;
+            if (v >= V) break;
+            u16 yv[8], gv[8], out[8];
+            __builtin_memcpy(yv, &ry[u], 16);
+            __builtin_memcpy(gv, &rg[u], 16);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float g = Bf2F(gv[q]);
+                out[q] = F2Bf(Bf2F(yv[q]) > 0.f ? g : kLeak * g);
+                s[q] += Bf2F(out[q]);
+            }
+            uint4 w;
+            __builtin_memcpy(&w, out, 16);
+            *reinterpret_cast<uint4*>(dz + v * C + o * 8) = w;
         }
     }
     __shared__ float red[256 * 8];
@@ -1796,6 +1848,21 @@ int fc_nn_colsum(const void* x, long long V, int C, float* out, float* ws, int a
     if (C <= 0 || C % 8 != 0 || 256 % (C / 8) != 0) return -1;
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(KColSumPartial, dim3(kColBlocks), dim3(256), 0, s, static_cast<const u16*>(x), V, C, ws);
+    hipLaunchKernelGGL(KReduceSplits, dim3(Blk(C, kRedOut)), dim3(256), 0, s, ws, kColBlocks, static_cast<long long>(C),
+                       out, accumulate);
+    return static_cast<int>(hipGetLastError());
+}
+
+// Number of per-block partial sums behind fc_nn_colsum and fc_nn_leaky_backward_colsum: their
+// workspace holds fc_nn_col_blocks() * C floats.
+int fc_nn_col_blocks() { return kColBlocks; }
+
+int fc_nn_leaky_backward_colsum(const void* y, const void* dy, void* dz, long long V, int C, float* out, float* ws,
+                                int accumulate, void* stream) {
+    if (C <= 0 || C % 8 != 0 || 256 % (C / 8) != 0) return -1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(KLeakyColSum, dim3(kColBlocks), dim3(256), 0, s, static_cast<const u16*>(y),
+                       static_cast<const u16*>(dy), static_cast<u16*>(dz), V, C, ws);
     hipLaunchKernelGGL(KReduceSplits, dim3(Blk(C, kRedOut)), dim3(256), 0, s, ws, kColBlocks, static_cast<long long>(C),
                        out, accumulate);
     return static_cast<int>(hipGetLastError());

@@ -66,6 +66,8 @@ def nn_lib() -> ctypes.CDLL:
     L.fc_nn_wgrad.argtypes = [i, vp, vp, vp, vp, vp, i, i] + [i] * 11 + [vp]
     L.fc_nn_colsum.argtypes = [vp, ll, i, vp, vp, i, vp]
     L.fc_nn_leaky_backward.argtypes = [vp, vp, vp, ll, vp]
+    L.fc_nn_leaky_backward_colsum.argtypes = [vp, vp, vp, ll, i, vp, vp, i, vp]
+    L.fc_nn_col_blocks.argtypes = []
     L.fc_nn_transpose_w.argtypes = [vp, vp, i, i, vp]
     L.fc_nn_flip_transpose_w.argtypes = [vp, vp, i, i, vp]
     L.fc_nn_upcat_split.argtypes = [vp, i, i, i, i, i, i, vp, vp, vp]
@@ -236,7 +238,12 @@ class Surrogate:
             V = nb * dd * hd * wd
             y = acts[name]
             dz = t.empty_like(y)
-            _chk(L.fc_nn_leaky_backward(_p(y), _p(grads[name]), _p(dz), V * cout, s), "leaky")
+            # dZ and the bias gradient (its per-channel sums) in one pass
+            cws = t.empty(L.fc_nn_col_blocks() * cout, dtype=t.float32, device=self.dev)
+            fused = L.fc_nn_leaky_backward_colsum(_p(y), _p(grads[name]), _p(dz), V, cout, _p(p["gb"]), _p(cws), 0,
+                                                   s) == 0
+            if not fused:
+                _chk(L.fc_nn_leaky_backward(_p(y), _p(grads[name]), _p(dz), V * cout, s), "leaky")
             # weight and bias gradients
             K = 27 * kin
             splits = wgrad_splits(V, K, cout)
@@ -248,8 +255,8 @@ class Surrogate:
             else:
                 _chk(L.fc_nn_wgrad(UPCAT, _p(acts[ins[1]]), _p(acts[ins[0]]), _p(dz), _p(p["gw"]), _p(ws), splits, 0,
                                    nb, dd, hd, wd, dd, hd, wd, kin, c1, cout, 1, s), name + " wgrad")
-            cws = t.empty(256 * cout, dtype=t.float32, device=self.dev)
-            _chk(L.fc_nn_colsum(_p(dz), V, cout, _p(p["gb"]), _p(cws), 0, s), name + " bias")
+            if not fused:
+                _chk(L.fc_nn_colsum(_p(dz), V, cout, _p(p["gb"]), _p(cws), 0, s), name + " bias")
             if name == "e0":
                 continue  # no gradient needed for the input features
             # data gradient through the transposed weights: the stride-2 adjoint gathers with

@@ -332,7 +332,7 @@ def test_head_loss_colsum_adam():
     assert rel(dh.float().cpu().numpy().reshape(V, C), g @ wh) < 2 ** -7
     # bias-gradient column sums
     cs = t.empty(C, dtype=t.float32, device="cuda")
-    cws = t.empty(256 * C, dtype=t.float32, device="cuda")
+    cws = t.empty(L.fc_nn_col_blocks() * C, dtype=t.float32, device="cuda")
     assert L.fc_nn_colsum(ptr(dev_bf16(h)), V, C, ptr(cs), ptr(cws), 0, stream()) == 0
     assert rel(cs.cpu().numpy(), h.sum(0)) < 1e-5
     # Adam, two steps, against the textbook update (Kingma & Ba 2015)
@@ -502,3 +502,34 @@ def test_network_gradients_end_to_end_sanity():
             print(name, kind, f"vs emulated {e:.3e}  vs exact {e2:.3e}  (emulated vs exact {eq:.3e})")
             assert e < max(5e-2, 2.0 * eq), (name, kind, e, eq)
             assert e2 < 2.5e-1, (name, kind, e2)
+
+
+@pytest.mark.parametrize("C,V", [(16, 1000), (32, 4099), (64, 70001), (128, 3), (8, 300000)])
+def test_leaky_backward_colsum_fused_equals_separate(C, V):
+    """The fused leaky-ReLU backward and bias-gradient sums give the same dZ and the same
+    per-channel sums, bit for bit, as the two kernels run one after the other (same grid,
+    thread mapping and summation order), on voxel counts that leave ragged last rounds."""
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    rng = np.random.default_rng(C * 7 + V)
+    y = dev_bf16(rng.normal(size=(V, C)))
+    dy = dev_bf16(rng.normal(size=(V, C)))
+    dz1 = t.empty(V * C, dtype=t.bfloat16, device="cuda")
+    dz2 = t.empty(V * C, dtype=t.bfloat16, device="cuda")
+    g1 = t.zeros(C, dtype=t.float32, device="cuda")
+    g2 = t.zeros(C, dtype=t.float32, device="cuda")
+    ws1 = t.empty(L.fc_nn_col_blocks() * C, dtype=t.float32, device="cuda")
+    ws2 = t.empty(L.fc_nn_col_blocks() * C, dtype=t.float32, device="cuda")
+    assert L.fc_nn_leaky_backward(ptr(y), ptr(dy), ptr(dz1), V * C, stream()) == 0
+    assert L.fc_nn_colsum(ptr(dz1), V, C, ptr(g1), ptr(ws1), 0, stream()) == 0
+    assert L.fc_nn_leaky_backward_colsum(ptr(y), ptr(dy), ptr(dz2), V, C, ptr(g2), ptr(ws2), 0, stream()) == 0
+    t.cuda.synchronize()
+    assert t.equal(dz1.view(t.int16), dz2.view(t.int16))
+    assert t.equal(g1.view(t.int32), g2.view(t.int32))
+    # and both against the definition in float64
+    yn = y.float().cpu().numpy().reshape(V, C)
+    dyn = dy.float().cpu().numpy().reshape(V, C)
+    want = no.leaky_grad(yn.astype(np.float64), dyn.astype(np.float64))
+    assert np.abs(dz2.float().cpu().numpy().reshape(V, C) - want).max() <= 2 ** -7 * np.abs(want).max()
+    assert np.abs(g2.cpu().numpy() - want.sum(axis=0)).max() <= 1e-3 * max(np.abs(want).sum(axis=0).max(), 1.0)
