@@ -734,6 +734,21 @@ __global__ void KAddBf16(const u16* __restrict__ x, u16* __restrict__ y, long lo
     if (i < n) y[i] = F2Bf(Bf2F(y[i]) + Bf2F(x[i]));
 }
 
+// The same, 8 elements per thread (n % 8 == 0): one 16-byte load of each input and one store.
+__global__ void KAddBf16x8(const u16* __restrict__ x, u16* __restrict__ y, long long n8) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    u16 xv[8], yv[8];
+    const uint4 rx = reinterpret_cast<const uint4*>(x)[i], ry = reinterpret_cast<const uint4*>(y)[i];
+    __builtin_memcpy(xv, &rx, 16);
+    __builtin_memcpy(yv, &ry, 16);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) yv[q] = F2Bf(Bf2F(yv[q]) + Bf2F(xv[q]));
+    uint4 w;
+    __builtin_memcpy(&w, yv, 16);
+    reinterpret_cast<uint4*>(y)[i] = w;
+}
+
 __global__ void KF32ToBf16(const float* __restrict__ x, u16* __restrict__ y, long long n) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) y[i] = F2Bf(x[i]);
@@ -1939,8 +1954,12 @@ int fc_nn_adam(float* p, const float* g, float* m, float* v, void* pbf, long lon
 }
 
 int fc_nn_add_bf16(const void* x, void* y, long long n, void* stream) {
-    hipLaunchKernelGGL(KAddBf16, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const u16*>(x), static_cast<u16*>(y), n);
+    if (n % 8 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0)
+        hipLaunchKernelGGL(KAddBf16x8, dim3(Blk(n / 8, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                           static_cast<const u16*>(x), static_cast<u16*>(y), n / 8);
+    else
+        hipLaunchKernelGGL(KAddBf16, dim3(Blk(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                           static_cast<const u16*>(x), static_cast<u16*>(y), n);
     return static_cast<int>(hipGetLastError());
 }
 

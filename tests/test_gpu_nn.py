@@ -533,3 +533,20 @@ def test_leaky_backward_colsum_fused_equals_separate(C, V):
     want = no.leaky_grad(yn.astype(np.float64), dyn.astype(np.float64))
     assert np.abs(dz2.float().cpu().numpy().reshape(V, C) - want).max() <= 2 ** -7 * np.abs(want).max()
     assert np.abs(g2.cpu().numpy() - want.sum(axis=0)).max() <= 1e-3 * max(np.abs(want).sum(axis=0).max(), 1.0)
+
+
+@pytest.mark.parametrize("n", [8, 4096, 1 << 20, 1003])
+def test_add_bf16_equals_rounded_fp32_sum(n):
+    """y += x on bf16 tensors (the gradient accumulation of a tensor with two consumers): each
+    result is the fp32 sum rounded to nearest even, bit for bit torch's own rounding, on the
+    16-byte path (n % 8 == 0) and the one-element path."""
+    from flowcraft_amd.surrogate import nn_lib
+    t = torch_mod()
+    L = nn_lib()
+    g = t.Generator().manual_seed(n)
+    x = t.randn(n, generator=g).to(t.bfloat16).cuda()
+    y = t.randn(n, generator=g).to(t.bfloat16).cuda()
+    want = (y.float() + x.float()).to(t.bfloat16)
+    assert L.fc_nn_add_bf16(ptr(x), ptr(y), n, stream()) == 0
+    t.cuda.synchronize()
+    assert t.equal(y.view(t.int16), want.view(t.int16))
