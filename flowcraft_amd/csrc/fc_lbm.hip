@@ -704,7 +704,7 @@ int fc_lbm_step_range(fc_lbm* h, int z0, int zcount) {
     const long long layer_tiles = static_cast<long long>((h->nx + tx - 1) / tx) * ((h->ny + ty * cpt - 1) / (ty * cpt));
     // Lattices past 2^31 work-items per launch step in z ranges (one step's ranges are
     // independent: each cell's update reads only the previous state).
-    const int zmax = static_cast<int>(std::max<long long>(1, kMaxStepItems / (256 * layer_tiles)));
+    const int zmax = static_cast<int>(std::max<long long>(1, kMaxStepItems / (StepThreads(static_cast<int>(h->tbytes)) * layer_tiles)));
     hipError_t e = hipSuccess;
     DispatchStep(h, [&]<int Q, class S, class Lay>() {
         InletPops<Q, S>(h, a);

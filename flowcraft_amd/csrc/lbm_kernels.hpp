@@ -775,11 +775,20 @@ __device__ __forceinline__ bool CollideOffset(const float (&g)[Q], float (&go)[Q
 #ifndef FC_LBM_TILE_X
 #define FC_LBM_TILE_X 0
 #endif
+#ifndef FC_LBM_THREADS_16
+#define FC_LBM_THREADS_16 256
+#endif
+// Threads per workgroup of the step: 256, or FC_LBM_THREADS_16 for 2-byte populations.
+// Whole 512-cell rows as one 512-thread tile (fewer x-shifted pulls re-fetching the
+// neighbour tile's lines) measured slower on the 512x256x256 wing, same fields: FP16
+// ping-pong 70.8 -> 63.9 GLUPS, in place 62.1 -> 60.3, BF16 ping-pong 69.4 -> 59.1.
+__host__ __device__ constexpr int StepThreads(int tbytes) { return tbytes == 2 ? FC_LBM_THREADS_16 : 256; }
 __host__ __device__ constexpr int TileX(int tbytes) {
-    return FC_LBM_TILE_X > 0 ? FC_LBM_TILE_X : (tbytes == 2 ? 256 : 128);
+    return FC_LBM_TILE_X > 0 ? FC_LBM_TILE_X : (tbytes == 2 ? StepThreads(2) : 128);
 }
-__host__ __device__ constexpr int TileY(int tbytes) { return 256 / TileX(tbytes); }
-static_assert(TileX(2) % 64 == 0 && 256 % TileX(2) == 0 && TileX(4) % 64 == 0 && 256 % TileX(4) == 0,
+__host__ __device__ constexpr int TileY(int tbytes) { return StepThreads(tbytes) / TileX(tbytes); }
+static_assert(TileX(2) % 64 == 0 && StepThreads(2) % TileX(2) == 0 && TileX(4) % 64 == 0 &&
+                  StepThreads(4) % TileX(4) == 0,
               "a tile row is whole waves");
 
 __device__ __forceinline__ unsigned XcdContiguous(unsigned hw, unsigned n) {
@@ -1180,7 +1189,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
 // RESIDENT: the populations fit in the last-level cache, so the stores stay cached for
 // the next step's pulls instead of streaming to HBM.
 template <int Q, class S, class Lay, int CPT, bool PER, bool RESIDENT = false>
-__global__ __launch_bounds__(256) FC_LBM_STEP_ATTR void KStep(const StepArgs a) {
+__global__ __launch_bounds__(StepThreads(sizeof(typename S::T))) FC_LBM_STEP_ATTR void KStep(const StepArgs a) {
     constexpr int SA = RESIDENT ? 0 : kStoreAux;
     using R = typename S::R;
     using T = typename S::T;

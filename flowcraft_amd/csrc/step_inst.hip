@@ -14,7 +14,7 @@ namespace fc {
 
 template <int Q, class S, class Lay>
 hipError_t LaunchStep(const StepArgs& a, unsigned tiles, int cpt, bool periodic, bool resident, hipStream_t stream) {
-    const dim3 grid(tiles), block(256);
+    const dim3 grid(tiles), block(StepThreads(sizeof(typename S::T)));
     if (periodic) {
         if (cpt == 2)
             hipLaunchKernelGGL((KStep<Q, S, Lay, 2, true>), grid, block, 0, stream, a);
