@@ -1177,7 +1177,9 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
 // The per-direction scalar offsets hold ~100 SGPRs, which caps the step at 7 waves per
 // SIMD.  Forcing 8 (FC_LBM_WAVES_PER_EU=8: 78 SGPRs, the rest spilled to VGPR lanes)
 // measured slower on the 512x256x256 wing (FP32 ping-pong 37.06 -> 36.9 GLUPS, FP16
-// 70.1 -> 68.2), so no limit is set by default.
+// 70.1 -> 68.2), so no limit is set by default.  FC_LBM_WAVES_PER_EU=7 (the periodic
+// FP32 ping-pong build 73 -> 70 VGPRs, 6 -> 7 waves) changed nothing beyond run-to-run
+// spread: periodic 128x64x64 sphere FP32 18.8-19.1 -> 18.6-18.7 us, FP16 15.6 -> 15.9 us.
 #ifndef FC_LBM_WAVES_PER_EU
 #define FC_LBM_WAVES_PER_EU 0
 #endif
