@@ -71,20 +71,25 @@ constexpr int kStoreAux = 2;  // non-temporal
 #else
 constexpr int kStoreAux = 0;
 #endif
-// Cache policy of the population loads (0: default).
+// Cache policy of the population loads: FC_LBM_LOAD_AUX (default 0) for the ping-pong
+// layout and for lattices held in the last-level cache, FC_LBM_INPLACE_LOAD_AUX (default
+// 2, non-temporal) for the streamed in-place step (see StepLoadAux).
 #ifndef FC_LBM_LOAD_AUX
 #define FC_LBM_LOAD_AUX 0
 #endif
 constexpr int kLoadAux = FC_LBM_LOAD_AUX;
+#ifndef FC_LBM_INPLACE_LOAD_AUX
+#define FC_LBM_INPLACE_LOAD_AUX 2
+#endif
 
-template <class T>
+template <class T, int Aux = kLoadAux>
 __device__ __forceinline__ T BufLoad(BufRsrc r, uint32_t vo, uint32_t so) {
     if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, kLoadAux));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, Aux));
     else if constexpr (sizeof(T) == 4)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, kLoadAux));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, Aux));
     else
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, kLoadAux));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, Aux));
 }
 
 template <class T, int Aux = kStoreAux>
@@ -881,12 +886,12 @@ __host__ __device__ constexpr int StoreDx(int d) {
 // every direction.  The offsets are those of an interior cell and stay inside the
 // padded allocation for every cell of the lattice; values for cells that take another
 // branch are discarded.
-template <int Q, class S, class Lay>
+template <int Q, class S, class Lay, int LA>
 __device__ __forceinline__ void SpecLoad(const StepArgs& a, BufRsrc src, const PopLane& ln, typename S::R (&fs)[Q]) {
     using T = typename S::T;
     StaticFor<0, Q>([&](auto I) {
         constexpr int d = I;
-        fs[d] = S::Raw(BufLoad<T>(src, LaneSel<LoadDx<Q, Lay>(d)>(ln), a.ld_off[d]));
+        fs[d] = S::Raw(BufLoad<T, LA>(src, LaneSel<LoadDx<Q, Lay>(d)>(ln), a.ld_off[d]));
     });
 }
 
@@ -941,7 +946,7 @@ __device__ __forceinline__ void StoreCell(const StepArgs& a, const Geo& g, BufRs
 
 // Everything after the bulk pull for one cell: branch on the cell type, patch wall and
 // solid directions, collide, store.
-template <int Q, class S, class Lay, bool PER, int SA>
+template <int Q, class S, class Lay, bool PER, int SA, int LA>
 __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufRsrc src_layer, BufRsrc dst_layer,
                                            int x, int y, int z, const typename S::R (&fs)[Q], uint8_t fl) {
     using T = typename S::T;
@@ -1000,7 +1005,7 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
             const uint32_t nvo = nbx < x ? ln.m - static_cast<uint32_t>(sizeof(T)) : ln.p + static_cast<uint32_t>(sizeof(T));
             StaticFor<0, Q>([&](auto I) {
                 constexpr int d = I;
-                v[d] = S::Raw(BufLoad<T>(src_layer, nvo, a.st_off[d]));
+                v[d] = S::Raw(BufLoad<T, LA>(src_layer, nvo, a.st_off[d]));
             });
             MomentSums<Q, R>(v, nr, nu, nv, nw);
         } else {
@@ -1058,13 +1063,13 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
             if constexpr (!IP) {
                 StaticFor<1, Q>([&](auto I) {
                     constexpr int j = I;
-                    own[j] = S::Raw(BufLoad<T>(src_layer, ln.c, a.st_off[j]));
+                    own[j] = S::Raw(BufLoad<T, LA>(src_layer, ln.c, a.st_off[j]));
                 });
             } else {
                 auto own_slots = [&]() {
                     StaticFor<1, Q>([&](auto I) {
                         constexpr int j = I;
-                        own[j] = S::Raw(BufLoad<T>(dst_layer, LaneSel<(j & 1) ? Lat<Q>::ex[j] : 0>(ln), a.prev_off[j]));
+                        own[j] = S::Raw(BufLoad<T, LA>(dst_layer, LaneSel<(j & 1) ? Lat<Q>::ex[j] : 0>(ln), a.prev_off[j]));
                     });
                 };
                 if constexpr (PER) {
@@ -1188,11 +1193,21 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
 #else
 #define FC_LBM_STEP_ATTR
 #endif
+// Load cache policy of the step.  Non-temporal loads measured faster for the streamed
+// in-place step and slower for ping-pong (512x256x256 wing, identical fields: FP32 in
+// place 32.1 -> 32.6 GLUPS, stochastically rounded FP16 in place 62.0 -> 66.8; ping-pong
+// FP32 37.2 -> 35.5, FP16 68.4 -> 66.8; periodic 128x64x64 sphere held in the cache
+// 18.8 -> 22.5 us).
+template <class Lay, bool RESIDENT>
+__host__ __device__ constexpr int StepLoadAux() {
+    return Lay::kInPlace && !RESIDENT ? FC_LBM_INPLACE_LOAD_AUX : kLoadAux;
+}
 // RESIDENT: the populations fit in the last-level cache, so the stores stay cached for
 // the next step's pulls instead of streaming to HBM.
 template <int Q, class S, class Lay, int CPT, bool PER, bool RESIDENT = false>
 __global__ __launch_bounds__(StepThreads(sizeof(typename S::T))) FC_LBM_STEP_ATTR void KStep(const StepArgs a) {
     constexpr int SA = RESIDENT ? 0 : kStoreAux;
+    constexpr int LA = StepLoadAux<Lay, RESIDENT>();
     using R = typename S::R;
     using T = typename S::T;
     S::StepMode();
@@ -1227,7 +1242,7 @@ __global__ __launch_bounds__(StepThreads(sizeof(typename S::T))) FC_LBM_STEP_ATT
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int y = min(y0 + k * kTileY, a.ny - 1);
-        SpecLoad<Q, S, Lay>(a, src_layer, MakePopLane<Q, T>(x, y, g.rs), fs[k]);
+        SpecLoad<Q, S, Lay, LA>(a, src_layer, MakePopLane<Q, T>(x, y, g.rs), fs[k]);
     }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
@@ -1248,7 +1263,7 @@ __global__ __launch_bounds__(StepThreads(sizeof(typename S::T))) FC_LBM_STEP_ATT
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int y = y0 + k * kTileY;
-        if (y < a.ny) UpdateCell<Q, S, Lay, PER, SA>(a, g, src_layer, dst_layer, x, y, z, fs[k], fl[k]);
+        if (y < a.ny) UpdateCell<Q, S, Lay, PER, SA, LA>(a, g, src_layer, dst_layer, x, y, z, fs[k], fl[k]);
     }
 }
 
