@@ -1196,8 +1196,8 @@ __device__ __forceinline__ void UpdateCell(const StepArgs& a, const Geo& g, BufR
 // Load cache policy of the step.  Non-temporal loads measured faster for the streamed
 // in-place step and slower for ping-pong (512x256x256 wing, identical fields: FP32 in
 // place 32.1 -> 32.6 GLUPS, stochastically rounded FP16 in place 62.0 -> 66.8; ping-pong
-// FP32 37.2 -> 35.5, FP16 68.4 -> 66.8; periodic 128x64x64 sphere held in the cache
-// 18.8 -> 22.5 us).
+// FP32 37.2 -> 35.5, FP16 68.4 -> 66.8, and 37.2 -> 36.95 with only the pulls without
+// an x shift non-temporal; periodic 128x64x64 sphere held in the cache 18.8 -> 22.5 us).
 template <class Lay, bool RESIDENT>
 __host__ __device__ constexpr int StepLoadAux() {
     return Lay::kInPlace && !RESIDENT ? FC_LBM_INPLACE_LOAD_AUX : kLoadAux;
